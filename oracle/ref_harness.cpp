@@ -1,0 +1,422 @@
+// ORACLE — test infrastructure only.
+//
+// Fixture writer linked against the reference library compiled from its own
+// sources (oracle/Makefile target `ref`).  It only drives the reference's
+// public/protected API (TrimBase hooks, MaterialBase, samples); every number
+// it prints is computed by the reference.  Runs only where the reference is
+// mounted; its output is committed under tests/golden/.
+//
+//   ref_harness stop Z1 m1 rho nel Z m t ...   (energies on stdin)  -> getrstop
+//   ref_harness avg  Z1 m1 tmin rho nel Z m t ...                   -> average()
+//   ref_harness trace SPECFILE h max_rows                           -> per collision
+//   ref_harness tally SPECFILE                                      -> tallies
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <queue>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "element.h"
+#include "ion.h"
+#include "material.h"
+#include "sample_clusters.h"
+#include "sample_layers.h"
+#include "sample_solid.h"
+#include "simconf.h"
+#include "trim.h"
+
+#include "src/philox_ref.h"
+
+using namespace MyTRIM_NS;
+
+namespace {
+
+struct MatSpec {
+  double rho;
+  std::vector<int> Z;
+  std::vector<double> m, t, Ed, Eb;
+};
+
+struct Spec {
+  std::string geo = "solid";
+  double w[3] = {10000, 10000, 10000};
+  int bc[3] = {0, 0, 0};
+  std::vector<MatSpec> mats;
+  std::vector<double> layers;
+  int kn[3] = {1, 1, 1};
+  std::vector<double> cl;  // x y z r
+  double tmin = 0.2, tau = 0.0, cw = 0.001, scale = 1.0, Ef = 3.0;
+  int potential = 0;
+  int pZ = 1;
+  double pm = 1, pE = 1, ppos[3] = {0, 0, 0}, pdir[3] = {1, 0, 0};
+  long long nhist = 1, hist0 = 0;
+  unsigned seed = 1;
+  int follow = -1, kp = 0;
+  double range_x0 = 0, range_bw = 10, vac_x0 = 0, vac_bw = 10, lat_y0 = 0, lat_z0 = 0, lat_bw = 10;
+  int range_nb = 100, vac_nb = 100, lat_nb = 100;
+};
+
+Spec read_spec(const char* path) {
+  Spec s;
+  std::ifstream in(path);
+  if (!in) {
+    std::cerr << "cannot open " << path << "\n";
+    exit(2);
+  }
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    std::string k;
+    if (!(ss >> k) || k[0] == '#') continue;
+    if (k == "geo") ss >> s.geo;
+    else if (k == "w") ss >> s.w[0] >> s.w[1] >> s.w[2];
+    else if (k == "bc") ss >> s.bc[0] >> s.bc[1] >> s.bc[2];
+    else if (k == "mat") {
+      MatSpec m;
+      int n;
+      ss >> m.rho >> n;
+      for (int i = 0; i < n; ++i) {
+        int z;
+        double mm, t, ed, eb;
+        ss >> z >> mm >> t >> ed >> eb;
+        m.Z.push_back(z);
+        m.m.push_back(mm);
+        m.t.push_back(t);
+        m.Ed.push_back(ed);
+        m.Eb.push_back(eb);
+      }
+      s.mats.push_back(m);
+    } else if (k == "layers") {
+      double t;
+      while (ss >> t) s.layers.push_back(t);
+    } else if (k == "kn") ss >> s.kn[0] >> s.kn[1] >> s.kn[2];
+    else if (k == "cluster") {
+      double v;
+      for (int i = 0; i < 4; ++i) {
+        ss >> v;
+        s.cl.push_back(v);
+      }
+    } else if (k == "tmin") ss >> s.tmin;
+    else if (k == "tau") ss >> s.tau;
+    else if (k == "cw") ss >> s.cw;
+    else if (k == "scale") ss >> s.scale;
+    else if (k == "Ef") ss >> s.Ef;
+    else if (k == "potential") ss >> s.potential;
+    else if (k == "pka")
+      ss >> s.pZ >> s.pm >> s.pE >> s.ppos[0] >> s.ppos[1] >> s.ppos[2] >> s.pdir[0] >> s.pdir[1] >>
+          s.pdir[2];
+    else if (k == "nhist") ss >> s.nhist;
+    else if (k == "hist0") ss >> s.hist0;
+    else if (k == "seed") ss >> s.seed;
+    else if (k == "follow") ss >> s.follow;
+    else if (k == "kp") ss >> s.kp;
+    else if (k == "range") ss >> s.range_x0 >> s.range_bw >> s.range_nb;
+    else if (k == "vac") ss >> s.vac_x0 >> s.vac_bw >> s.vac_nb;
+    else if (k == "lat") ss >> s.lat_y0 >> s.lat_z0 >> s.lat_bw >> s.lat_nb;
+  }
+  return s;
+}
+
+SampleBase* build(const Spec& s, SimconfType* sc) {
+  SampleBase* smp;
+  if (s.geo == "solid") {
+    smp = new SampleSolid(s.w[0], s.w[1], s.w[2]);
+  } else if (s.geo == "layers") {
+    SampleLayers* sl = new SampleLayers(s.w[0], s.w[1], s.w[2]);
+    for (double t : s.layers) sl->layerThickness.push_back(t);
+    smp = sl;
+  } else {
+    sampleClusters* c = new sampleClusters(s.w[0], s.w[1], s.w[2]);
+    for (int i = 0; i < 3; ++i) c->bc[i] = SampleBase::sampleBoundary(s.bc[i]);
+    c->initSpatialhash(s.kn[0], s.kn[1], s.kn[2]);
+    for (size_t i = 0; i + 3 < s.cl.size(); i += 4) c->addCluster(s.cl[i], s.cl[i + 1], s.cl[i + 2], s.cl[i + 3]);
+    smp = c;
+  }
+  for (int i = 0; i < 3; ++i) smp->bc[i] = SampleBase::sampleBoundary(s.bc[i]);
+  for (const auto& ms : s.mats) {
+    MaterialBase* m = new MaterialBase(sc, ms.rho);
+    for (size_t i = 0; i < ms.Z.size(); ++i) {
+      Element e;
+      e._Z = ms.Z[i];
+      e._m = ms.m[i];
+      e._t = ms.t[i];
+      e._Edisp = ms.Ed[i];
+      e._Elbind = ms.Eb[i];
+      m->_element.push_back(e);
+    }
+    m->prepare();
+    smp->material.push_back(m);
+  }
+  return smp;
+}
+
+int bin_of(double x, double x0, double bw, int nb) {
+  const double t = (x - x0) / bw;
+  if (!(t >= 0.0)) return 0;
+  if (t >= double(nb)) return nb + 1;
+  return 1 + int(t);
+}
+
+struct Tallies {
+  long long n_coll = 0, n_flight = 0, vac = 0, repl = 0, subst = 0, inter = 0, lost = 0, vacuum = 0,
+            disp = 0;
+  double E0 = 0, Eel = 0, Enuc = 0, Eunf = 0, Elost = 0, Edrop = 0, Evac = 0;
+  long long range_n = 0;
+  double range_sum = 0, range_sum2 = 0;
+  std::vector<long long> range_hist, lat_y, lat_z, vac_hist, repl_hist;
+};
+
+// Tally definitions identical to oracle/src/runs.cpp (TallyHooks)
+class TallyTrim : public TrimBase {
+ public:
+  TallyTrim(SimconfType* sc, SampleBase* smp, const Spec& s, Tallies& t)
+      : TrimBase(sc, smp), _s(s), _t(t) {}
+
+ protected:
+  bool followRecoil() override {
+    _t.disp++;
+    const bool f = _s.follow < 0 || _recoil->_gen < _s.follow;
+    if (f)
+      _t.Enuc += _element->_Elbind;
+    else
+      _t.Eunf += _den;
+    return f;
+  }
+  void vacancyCreation() override {
+    _t.vac++;
+    _t.vac_hist[bin_of(_recoil->_pos(0), _s.vac_x0, _s.vac_bw, _s.vac_nb)]++;
+    if (_s.kp && _recoil->_gen == _s.follow) {
+      const Real ed = 0.0115 * std::pow(_material->_az, -7.0 / 3.0) * _recoil->_E;
+      const Real g = 3.4008 * std::pow(ed, 1.0 / 6.0) + 0.40244 * std::pow(ed, 3.0 / 4.0) + ed;
+      const Real kd = 0.1337 * std::pow(_material->_az, 2.0 / 3.0) / std::sqrt(_material->_am);
+      const Real Ev = _recoil->_E / (1.0 + kd * g);
+      _t.vac += int(0.8 * Ev / (2.0 * _element->_Edisp));
+    }
+  }
+  void replacementCollision() override {
+    _t.repl_hist[bin_of(_recoil->_pos(0), _s.vac_x0, _s.vac_bw, _s.vac_nb)]++;
+  }
+  void dissipateRecoilEnergy() override { _t.Enuc += _recoil->_E + _element->_Elbind; }
+  void checkPKAState() override {
+    _t.n_coll++;
+    _t.Eel += _dee;
+    switch (_pka->_state) {
+      case IonBase::LOST:
+        _t.lost++;
+        _t.Elost += _pka->_E;
+        _t.Edrop += _den;
+        break;
+      case IonBase::REPLACEMENT:
+        _t.repl++;
+        _t.Enuc += _pka->_E;
+        break;
+      case IonBase::SUBSTITUTIONAL:
+        _t.subst++;
+        _t.Enuc += _pka->_E;
+        break;
+      case IonBase::INTERSTITIAL:
+        _t.inter++;
+        _t.Enuc += _pka->_E;
+        break;
+      default:
+        break;
+    }
+  }
+  const Spec& _s;
+  Tallies& _t;
+};
+
+// Per-collision trace in the row layout of oracle/src/runs.cpp (orc_trace)
+class TraceTrim : public TrimBase {
+ public:
+  TraceTrim(SimconfType* sc, SampleBase* smp) : TrimBase(sc, smp) {}
+  std::vector<double> rows;
+  int flight = 0, ic = 0;
+
+ protected:
+  double _rE = 0, _rdir[3] = {0, 0, 0};
+  void grab() {
+    _rE = _recoil->_E;
+    for (int i = 0; i < 3; ++i) _rdir[i] = _recoil->_dir(i);
+  }
+  bool followRecoil() override {
+    grab();
+    return true;
+  }
+  void dissipateRecoilEnergy() override { grab(); }
+  void checkPKAState() override {
+    ++ic;
+    if (_pka->_state == IonBase::LOST) grab();
+    const double r[21] = {double(flight), double(ic), double(_pka->_Z), _pka->_E, _pka->_pos(0),
+                          _pka->_pos(1), _pka->_pos(2), _pka->_dir(0), _pka->_dir(1), _pka->_dir(2),
+                          double(_pka->_state), _ls, _dee, _den, double(_element->_Z), _rE,
+                          _rdir[0], _rdir[1], _rdir[2], double(_pka->_gen), 0.0};
+    rows.insert(rows.end(), r, r + 21);
+  }
+};
+
+IonBase* primary(const Spec& s) {
+  IonBase* p = new IonBase;
+  p->_Z = s.pZ;
+  p->_m = s.pm;
+  p->_E = s.pE;
+  p->_pos = Point(s.ppos[0], s.ppos[1], s.ppos[2]);
+  p->_dir = Point(s.pdir[0], s.pdir[1], s.pdir[2]);
+  p->_gen = 0;
+  p->_Ef = s.Ef;
+  return p;
+}
+
+unsigned history_seed(unsigned master, long long h) {
+  return unsigned(orc::flight_key_child(master, unsigned(h)));
+}
+
+void print_vec(const char* name, const std::vector<long long>& v) {
+  printf("%s", name);
+  for (long long x : v) printf(" %lld", x);
+  printf("\n");
+}
+
+int cmd_tally(const Spec& s) {
+  SimconfType sc;
+  sc.tmin = s.tmin;
+  sc.tau = s.tau;
+  sc.cw = s.cw;
+  sc.setLengthScale(s.scale);
+  SampleBase* smp = build(s, &sc);
+  Tallies t;
+  t.range_hist.assign(s.range_nb + 2, 0);
+  t.lat_y.assign(s.lat_nb + 2, 0);
+  t.lat_z.assign(s.lat_nb + 2, 0);
+  t.vac_hist.assign(s.vac_nb + 2, 0);
+  t.repl_hist.assign(s.vac_nb + 2, 0);
+  TallyTrim trim(&sc, smp, s, t);
+  trim._potential = TrimBase::Potential(s.potential);
+  std::vector<long long> hvac;
+  for (long long h = s.hist0; h < s.hist0 + s.nhist; ++h) {
+    const long long vac0 = t.vac;
+    sc.seed(history_seed(s.seed, h));
+    std::queue<IonBase*> q;
+    IonBase* p = primary(s);
+    t.E0 += p->_E;
+    q.push(p);
+    while (!q.empty()) {
+      IonBase* ion = q.front();
+      q.pop();
+      smp->averages(ion);
+      trim.trim(ion, q);
+      t.n_flight++;
+      const bool vacuum = ion->_state == IonBase::MOVING;
+      if (vacuum) {
+        t.vacuum++;
+        t.Evac += ion->_E;
+      }
+      if (ion->_gen == 0 && !vacuum && ion->_state != IonBase::LOST) {
+        t.range_n++;
+        t.range_sum += ion->_pos(0);
+        t.range_sum2 += ion->_pos(0) * ion->_pos(0);
+        t.range_hist[bin_of(ion->_pos(0), s.range_x0, s.range_bw, s.range_nb)]++;
+        t.lat_y[bin_of(ion->_pos(1) - s.lat_y0, -0.5 * s.lat_nb * s.lat_bw, s.lat_bw, s.lat_nb)]++;
+        t.lat_z[bin_of(ion->_pos(2) - s.lat_z0, -0.5 * s.lat_nb * s.lat_bw, s.lat_bw, s.lat_nb)]++;
+      }
+      delete ion;
+    }
+    hvac.push_back(t.vac - vac0);
+  }
+  printf("n_coll %lld\nn_flight %lld\nvac %lld\nrepl %lld\nsubst %lld\ninter %lld\nlost %lld\n",
+         t.n_coll, t.n_flight, t.vac, t.repl, t.subst, t.inter, t.lost);
+  printf("vacuum %lld\ndisp %lld\n", t.vacuum, t.disp);
+  printf("E0 %.17g\nEel %.17g\nEnuc %.17g\nEunf %.17g\nElost %.17g\nEdrop %.17g\nEvac %.17g\n", t.E0,
+         t.Eel, t.Enuc, t.Eunf, t.Elost, t.Edrop, t.Evac);
+  printf("range_n %lld\nrange_sum %.17g\nrange_sum2 %.17g\n", t.range_n, t.range_sum, t.range_sum2);
+  print_vec("range_hist", t.range_hist);
+  print_vec("lat_y_hist", t.lat_y);
+  print_vec("lat_z_hist", t.lat_z);
+  print_vec("vac_hist", t.vac_hist);
+  print_vec("repl_hist", t.repl_hist);
+  print_vec("h_vac", hvac);
+  return 0;
+}
+
+int cmd_trace(const Spec& s, long long h, int max_rows) {
+  SimconfType sc;
+  sc.tmin = s.tmin;
+  sc.tau = s.tau;
+  sc.cw = s.cw;
+  sc.setLengthScale(s.scale);
+  SampleBase* smp = build(s, &sc);
+  TraceTrim trim(&sc, smp);
+  trim._potential = TrimBase::Potential(s.potential);
+  sc.seed(history_seed(s.seed, h));
+  std::queue<IonBase*> q;
+  q.push(primary(s));
+  while (!q.empty() && int(trim.rows.size() / 21) < max_rows) {
+    IonBase* ion = q.front();
+    q.pop();
+    smp->averages(ion);
+    trim.trim(ion, q);
+    trim.flight++;
+    trim.ic = 0;
+    delete ion;
+  }
+  const int n = std::min<int>(max_rows, int(trim.rows.size() / 21));
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < 21; ++j) printf(j ? " %.17g" : "%.17g", trim.rows[21 * i + j]);
+    printf("\n");
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: ref_harness stop|avg|trace|tally ...\n");
+    return 2;
+  }
+  const std::string cmd = argv[1];
+  if (cmd == "stop" || cmd == "avg") {
+    SimconfType sc;
+    int a = 2;
+    const int Z1 = atoi(argv[a++]);
+    const double m1 = atof(argv[a++]);
+    double tmin = 0.2;
+    if (cmd == "avg") tmin = atof(argv[a++]);
+    const double rho = atof(argv[a++]);
+    const int nel = atoi(argv[a++]);
+    MaterialBase mat(&sc, rho);
+    for (int i = 0; i < nel; ++i) {
+      Element e;
+      e._Z = atoi(argv[a++]);
+      e._m = atof(argv[a++]);
+      e._t = atof(argv[a++]);
+      mat._element.push_back(e);
+    }
+    mat.prepare();
+    IonBase ion(Z1, m1, 0.0);
+    if (cmd == "stop") {
+      double E;
+      while (std::cin >> E) {
+        ion._E = E;
+        printf("%.17g\n", mat.getrstop(&ion));
+      }
+    } else {
+      sc.tmin = tmin;
+      mat.average(&ion);
+      printf("%.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g", mat._am, mat._az, mat._arho,
+             mat.mu, mat.a, mat.f, mat.epsdg, mat.fd, mat.kd);
+      for (const auto& e : mat._element) printf(" %.17g %.17g %.17g %.17g", e.my, e.ec, e.ai, e.fi);
+      printf("\n");
+    }
+    return 0;
+  }
+  if (cmd == "trace") return cmd_trace(read_spec(argv[2]), atoll(argv[3]), atoi(argv[4]));
+  if (cmd == "tally") return cmd_tally(read_spec(argv[2]));
+  fprintf(stderr, "unknown command %s\n", argv[1]);
+  return 2;
+}

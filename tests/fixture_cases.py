@@ -1,0 +1,50 @@
+"""Inputs of the golden fixtures (tests/golden/make_fixtures.py writes the
+outputs from the compiled reference; the tests re-run the same inputs)."""
+import numpy as np
+
+from mytrim_amd import configs
+from mytrim_amd.problem import Primaries
+
+# (Z, mass) of ions; mass 0 takes the table mass (material.C:256-259)
+IONS = [(1, 1.008), (2, 4.0026), (5, 10.81), (6, 12.011), (14, 28.086), (26, 55.845),
+        (29, 63.546), (47, 107.87), (54, 131.29), (74, 183.84), (92, 238.03), (29, 0.0)]
+# (name, rho, [(Z, m, t)])
+TARGETS = [("O", 1.429, [(8, 15.999, 1.0)]), ("C", 2.253, [(6, 12.011, 1.0)]),
+           ("Si", 2.33, [(14, 28.086, 1.0)]), ("Fe", 7.87, [(26, 55.845, 1.0)]),
+           ("Cu", 8.96, [(29, 63.546, 1.0)]), ("Ag", 10.5, [(47, 107.87, 1.0)]),
+           ("Xe", 3.5, [(54, 131.29, 1.0)]), ("W", 19.3, [(74, 183.84, 1.0)]),
+           ("U", 19.05, [(92, 238.03, 1.0)]), ("Ge", 5.32, [(32, 72.63, 1.0)]),
+           ("UO2", 10.96, [(92, 238.03, 1.0), (8, 15.999, 2.0)])]
+ENERGIES = np.logspace(0, 8, 97)  # 1 eV .. 100 MeV
+
+
+def c4_uniform(w, n_hist):
+    """UO2 + Xe bubbles with a single 300 keV Xe primary species."""
+    c4 = configs.c4_mixed_uo2(n_hist, w=w)
+    return c4.with_(primaries=Primaries.single(54, 131.29, 3.0e5, (w / 2, w / 2, w / 2),
+                                               (0.6, 0.8, 0.0)))
+
+
+def trace_cases():
+    return {"c1": configs.c1_he_fe(4), "c2": configs.c2_cu_cu(4), "c3": configs.c3_ag_layers(4),
+            "c4": c4_uniform(2000.0, 4), "c5": configs.c5_c_w(4),
+            "moliere": configs.c2_cu_cu(4).with_(potential=1),
+            "ckr": configs.c5_c_w(4).with_(potential=2)}
+
+
+def tally_cases(quick=False):
+    """name -> (problem, processes used to write it)"""
+    s = 0.1 if quick else 1.0
+    return {
+        "c1": (configs.c1_he_fe(2000), 1),
+        "c2": (configs.c2_cu_cu(int(800 * s)), 8),
+        "c3": (configs.c3_ag_layers(int(800 * s)), 8),
+        "c4": (c4_uniform(4000.0, int(400 * s)), 8),
+        "c5": (configs.c5_c_w(int(2000 * s)), 8),
+        "c2_primaries_kp": (configs.c2_cu_cu(int(4000 * s)).with_(follow_max_gen=1, kp=True), 4),
+        "vac_cu_1keV": (configs.vacancy_case("cu", 1e3, 4000), 1),
+        "vac_cu_10keV": (configs.vacancy_case("cu", 1e4, 2000), 2),
+        "vac_cu_100keV": (configs.vacancy_case("cu", 1e5, int(800 * s)), 8),
+        "vac_si_10keV": (configs.vacancy_case("si", 1e4, 2000), 2),
+        "vac_xe_10keV": (configs.vacancy_case("xe", 1e4, 2000), 2),
+    }

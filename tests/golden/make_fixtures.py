@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures from the reference compiled from its own sources.
+
+Runs only where the reference is mounted (oracle/Makefile target `ref` builds
+oracle/_ref/ref_harness from /root/reference).  Every number stored here is
+computed by the reference library; the harness only drives its API.
+
+  stopping_grid.npz   MaterialBase::getrstop (material.C:114-122) on an
+                      (ion, target, energy) grid
+  average.npz         MaterialBase::prepare/average constants (material.C:37-110)
+  trace_<case>.npz    per-collision state of TrimBase::trim (trim.C:35-425) for
+                      the first collisions of one history, std::mt19937 stream
+  tally_<case>.json   full-cascade tallies (FIFO drain of apps/runmytrim.C:71-93),
+                      one std::mt19937 seed per history
+Usage:  python tests/golden/make_fixtures.py [--quick]
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle_lib as O  # noqa: E402
+
+HARNESS = O.REF_HARNESS
+
+from fixture_cases import ENERGIES, IONS, TARGETS, tally_cases, trace_cases  # noqa: E402
+
+
+def run(args, stdin=None):
+    return subprocess.run([HARNESS] + [str(a) for a in args], input=stdin, capture_output=True,
+                          text=True, check=True).stdout
+
+
+def target_args(rho, els):
+    a = [rho, len(els)]
+    for Z, m, t in els:
+        a += [Z, m, t]
+    return a
+
+
+def stopping_grid():
+    e_in = "\n".join("%.17g" % e for e in ENERGIES) + "\n"
+    out = np.zeros((len(IONS), len(TARGETS), len(ENERGIES)))
+    for i, (Z1, m1) in enumerate(IONS):
+        for j, (_, rho, els) in enumerate(TARGETS):
+            txt = run(["stop", Z1, "%.17g" % m1] + target_args(rho, els), e_in)
+            out[i, j] = [float(x) for x in txt.split()]
+    np.savez_compressed(os.path.join(HERE, "stopping_grid.npz"), se=out, energies=ENERGIES,
+                        ions=np.array(IONS), targets=np.array([t[0] for t in TARGETS]))
+
+
+def average_table():
+    rows = []
+    for Z1, m1 in IONS:
+        if m1 == 0.0:
+            continue
+        for _, rho, els in TARGETS:
+            txt = run(["avg", Z1, "%.17g" % m1, 0.2] + target_args(rho, els))
+            v = [float(x) for x in txt.split()]
+            rows.append(v + [0.0] * (17 - len(v)))
+    np.savez_compressed(os.path.join(HERE, "average.npz"), avg=np.array(rows))
+
+
+def spec_file(prob):
+    f = tempfile.NamedTemporaryFile("w", suffix=".spec", delete=False)
+    f.write(prob.spec_text())
+    f.close()
+    return f.name
+
+
+def traces(max_rows=1500):
+    for name, prob in trace_cases().items():
+        path = spec_file(prob)
+        try:
+            for h in (0, 1):
+                txt = run(["trace", path, h, max_rows])
+                rows = np.array([[float(x) for x in l.split()] for l in txt.splitlines()])
+                np.savez_compressed(os.path.join(HERE, "trace_%s_h%d.npz" % (name, h)), rows=rows)
+        finally:
+            os.unlink(path)
+
+
+def parse_tally(txt):
+    res = {}
+    for line in txt.splitlines():
+        k, *v = line.split()
+        if len(v) == 1:
+            res[k] = v[0]
+        else:
+            res[k] = [int(x) for x in v]
+    return res
+
+
+def tallies(quick):
+    for name, (prob, nproc) in tally_cases(quick).items():
+        chunk = (prob.n_hist + nproc - 1) // nproc
+        parts = []
+        specs = []
+        for i in range(nproc):
+            h0 = i * chunk
+            n = min(chunk, prob.n_hist - h0)
+            if n <= 0:
+                continue
+            specs.append(spec_file(prob.with_(hist0=h0, n_hist=n)))
+        try:
+            with ThreadPoolExecutor(len(specs)) as ex:
+                parts = list(ex.map(lambda p: parse_tally(run(["tally", p])), specs))
+        finally:
+            for p in specs:
+                os.unlink(p)
+        tot = {}
+        for k in parts[0]:
+            if isinstance(parts[0][k], list):
+                if k == "h_vac":
+                    tot[k] = sum((p[k] for p in parts), [])
+                else:
+                    tot[k] = [int(x) for x in np.sum([p[k] for p in parts], axis=0)]
+            elif "." in parts[0][k] or "e" in parts[0][k]:
+                tot[k] = float(sum(float(p[k]) for p in parts))
+            else:
+                tot[k] = int(sum(int(p[k]) for p in parts))
+        tot["n_hist"] = prob.n_hist
+        tot["seed"] = prob.seed
+        tot["problem"] = prob.name
+        with open(os.path.join(HERE, "tally_%s.json" % name), "w") as f:
+            json.dump(tot, f)
+        print(name, "n_hist", prob.n_hist, "coll", tot["n_coll"], "vac/ion",
+              tot["vac"] / prob.n_hist, flush=True)
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    quick = "--quick" in sys.argv
+    stopping_grid()
+    average_table()
+    traces()
+    tallies(quick)
+
+
+if __name__ == "__main__":
+    main()
