@@ -1,0 +1,200 @@
+/*
+ * mytrim_gpu.h — C ABI of the MI355X (gfx950) binary-collision cascade engine.
+ *
+ * Plain pointers, sizes and POD structs only.  Every function returns 0 on
+ * success or a negative MT_E* code; mt_last_error() then holds a message.
+ * Host arrays are read during the call and may be freed afterwards.
+ *
+ * What each entry point replaces in the reference (MyTRIM):
+ *   mt_ctx_create / mt_ctx_set_tables   SimconfType construction and
+ *                                       readDataFiles (simconf.C:36-137)
+ *   mt_ctx_set_target                   MaterialBase + Element set-up and
+ *                                       prepare() (material.C:37-74), the
+ *                                       Sample* geometry (sample*.h) and the
+ *                                       SimconfType knobs (simconf.h:66)
+ *   mt_run_cascades                     the drain loop of apps/runmytrim.C:71-93
+ *                                       (averages + TrimBase::trim per ion,
+ *                                       trim.C:35-425) for a batch of primaries,
+ *                                       with the tally hooks of trim.C:439-527
+ *                                       and apps/src/TrimVacCount.C:31-53
+ *   mt_trim_flight                      one TrimBase::trim(pka, recoils) call
+ *                                       (trim.C:35-425) driven by the caller's
+ *                                       SimconfType::drand() stream, with a
+ *                                       per-collision event log so the caller
+ *                                       can replay the virtual hooks
+ *   mt_stopping_batch                   MaterialBase::getrstop (material.C:114-122)
+ *   mt_scatter_batch                    the scattering block of trim.C:172-272
+ */
+#ifndef MYTRIM_GPU_H
+#define MYTRIM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MT_OK = 0,
+  MT_E_ARG = -1,      /* invalid argument */
+  MT_E_HIP = -2,      /* HIP runtime error */
+  MT_E_STATE = -3,    /* call out of order (e.g. no target set) */
+  MT_E_DEVICE = -4,   /* device-side failure flag (queue overflow, runaway loop, NaN) */
+  MT_E_NOMEM = -5,
+  MT_E_LIMIT = -6     /* input exceeds a documented limit */
+};
+
+/* boundary conditions (SampleBase::sampleBoundary, sample.h:49-54) */
+enum { MT_PBC = 0, MT_INF = 1, MT_CUT = 2 };
+/* geometry kinds */
+enum { MT_GEO_SOLID = 0, MT_GEO_LAYERS = 1, MT_GEO_CLUSTERS = 2 };
+/* interatomic potentials (TrimBase::Potential, trim.h:63-68) */
+enum { MT_UNIVERSAL = 0, MT_MOLIERE = 1, MT_CKR = 2 };
+/* arithmetic of the cascade kernel */
+enum { MT_FP64 = 0, MT_FP32 = 1, MT_FP32_SURROGATE = 2 };
+
+typedef struct mt_ctx mt_ctx;
+
+/* Target: materials, geometry and the SimconfType knobs. */
+typedef struct {
+  int n_mat;
+  const double* mat_rho;  /* g/cm^3 */
+  const int* mat_el_off;  /* n_mat + 1 offsets into the element arrays */
+  const int* el_Z;
+  const double* el_m;
+  const double* el_t;     /* stoichiometry (normalised by prepare()) */
+  const double* el_Edisp;
+  const double* el_Elbind;
+  int geo;                /* MT_GEO_* */
+  double w[3];            /* sample size [A] */
+  int bc[3];              /* MT_PBC / MT_INF / MT_CUT */
+  int n_layers;           /* MT_GEO_LAYERS: thickness of layer i is layer_thick[i] */
+  const double* layer_thick;
+  int kn[3];              /* MT_GEO_CLUSTERS: spatial hash cells per axis */
+  int n_cl;               /* clusters (sphere centre, radius); material 1 inside */
+  const double* cl_x;
+  const double* cl_y;
+  const double* cl_z;
+  const double* cl_r;
+  double tmin, tau, cw, scale; /* SimconfType (simconf.C:44-49), length scale */
+  int potential;               /* MT_UNIVERSAL / MT_MOLIERE / MT_CKR */
+} mt_target;
+
+/* A batch of primaries (generation 0).  With pka_uniform != 0 the arrays hold
+ * one ion used for every history, else one entry per history. */
+typedef struct {
+  long long n_hist;
+  long long hist0;   /* index of the first history (selects its random streams) */
+  int pka_uniform;
+  const int* Z;
+  const double* m;
+  const double* E;   /* eV */
+  const double* pos; /* 3 per entry */
+  const double* dir; /* 3 per entry */
+  double Ef;         /* IonBase::_Ef, final energy (ion.C:26) */
+  uint32_t seed;
+} mt_primaries;
+
+/* Which recoils are followed and which tallies are binned. */
+typedef struct {
+  int follow_max_gen; /* < 0: every recoil (TrimBase); k: gen < k (TrimPrimaries/TrimRecoils) */
+  int kp;             /* modified Kinchin-Pease vacancies at gen == follow_max_gen */
+  int arith;          /* MT_FP64 / MT_FP32 / MT_FP32_SURROGATE */
+  double range_x0, range_bw; int range_nb;           /* final x of resting gen-0 ions */
+  double lat_y0, lat_z0, lat_bw; int lat_nb;         /* final y - y0, z - z0 of the same */
+  double vac_x0, vac_bw; int vac_nb;                 /* recoil x at vacancy / replacement */
+  int per_history;    /* fill the per-history arrays of mt_tallies */
+} mt_run_opts;
+
+/* Tallies.  Histogram arrays have nb + 2 bins: [0] underflow, [1..nb], [nb+1]
+ * overflow; they and the per-history arrays are caller-allocated (may be NULL). */
+typedef struct {
+  long long n_coll, n_flight, vac, repl, subst, inter, lost, vacuum, rutherford, newton, disp;
+  double E0, Eel, Enuc, Eunf, Elost, Edrop, Evac;
+  long long range_n;
+  double range_sum, range_sum2;
+  long long* range_hist;
+  long long* lat_y_hist;
+  long long* lat_z_hist;
+  long long* vac_hist;
+  long long* repl_hist;
+  int* h_vac;            /* per history, index h - hist0 */
+  long long* h_coll;
+  int* h_flights;
+  double* h_x;           /* final position and state of the gen-0 ion */
+  double* h_y;
+  double* h_z;
+  int* h_state;          /* IonBase::StateType, -1 = ended in vacuum */
+  double* h_Eel;
+  /* launch record */
+  double kernel_ms;      /* cascade kernel time (HIP events on its stream) */
+  int grid, block, waves;
+  long long queue_peak;  /* most recoils held in the HBM queue */
+  long long queue_ops;   /* wave-level HBM queue transfers */
+} mt_tallies;
+
+/* Tuning knobs of the persistent kernel (0 = default). */
+typedef struct {
+  int block;          /* threads per workgroup (multiple of 64) */
+  int blocks_per_cu;  /* resident workgroups per CU */
+  int stack_cap;      /* per-wave LDS recoil stack entries */
+  long long queue_cap;/* HBM queue entries (power of two) */
+  int lds_hist;       /* bin vacancy/replacement depths in LDS first */
+} mt_tuning;
+
+/* One flight (TrimBase::trim) for the drop-in single-call path. */
+typedef struct {
+  int Z;
+  double m, E;
+  double pos[3], dir[3];
+  int gen, tag, state;
+  double Ef;
+} mt_ion;
+
+/* Per-collision record of mt_trim_flight, in trim.C order. */
+typedef struct {
+  double pka_E, pka_pos[3], pka_dir[3];  /* after the collision */
+  double ls, dee, den;
+  double rec_E, rec_pos[3], rec_dir[3];  /* recoil: E = den - Elbind, dir before normalisation */
+  int material, element;                 /* indices into the target */
+  int pka_state;                         /* IonBase::StateType after the collision */
+  int fate;                              /* 0 dissipated, 1 above threshold, 2 lost (dropped) */
+  int vacancy;                           /* 1: vacancyCreation(), 0: replacementCollision() (fate 1) */
+  int mat_tag;                           /* material tag at this collision (cluster index) */
+} mt_event;
+
+const char* mt_last_error(void);
+const char* mt_version(void);
+
+int mt_ctx_create(int device, mt_ctx** out);
+int mt_ctx_destroy(mt_ctx* ctx);
+/* 92 rows x 17 doubles: mm1 m1 mnat rho atrho vfermi heat pcoef[8] lfctr
+ * (SCOEF.95A + SLFCTR.dat columns).  Defaults to the tables built into the
+ * library. */
+int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols);
+int mt_ctx_set_target(mt_ctx* ctx, const mt_target* target);
+int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* tuning);
+
+int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts,
+                    mt_tallies* out);
+
+/* Single flight with the caller's uniform stream u[0..n_u): consumes draws in
+ * the reference's order (trim.C:71, 143, 147, 322-333).  Returns MT_E_LIMIT
+ * with *n_used = n_u when the stream ran out (call again with more).  The ion
+ * is updated in place; events[0..*n_events) hold every collision. */
+int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, long long* n_used,
+                   mt_event* events, int max_events, int* n_events);
+
+/* getrstop for ion (Z1, m1) at energies E[0..n) in material `mat` [eV/A]. */
+int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, long long n,
+                      double* se_out);
+/* scattering for (eps, b) pairs: s2 = sin^2(theta/2), ct = cos(theta),
+ * st = sin(theta), iters = Newton iterations (0: Rutherford branch). */
+int mt_scatter_batch(mt_ctx* ctx, int potential, const double* eps, const double* b, long long n,
+                     double* s2, double* ct, double* st, int* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MYTRIM_GPU_H */

@@ -1,0 +1,665 @@
+// C ABI of the cascade engine (include/mytrim_gpu.h): context, target
+// upload, batched cascades, per-function batches.  Host code only; the
+// kernels live in mt_cascade.hip, mt_batch.hip and mt_flight.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mt_cascade.h"
+#include "mt_host.h"
+#include "mytrim_gpu.h"
+
+namespace mt {
+hipError_t cascade_occupancy(int potential, int geo, bool perhist, int block, size_t lds, int* blocks_per_cu);
+hipError_t cascade_launch(const Params& P, int potential, bool perhist, int grid, int block, size_t lds,
+                          hipStream_t stream);
+int cascade_stack_bytes(int cap);
+hipError_t stopping_launch(const unsigned char* tables, int species, int mat, const double* E, long long n,
+                           double* out, hipStream_t stream);
+hipError_t scatter_launch(int potential, const double* eps, const double* b, long long n, double* s2,
+                          double* ct, double* st, int* iters, hipStream_t stream);
+struct FlightParams {
+  const unsigned char* tables;
+  Geometry geo;
+  double tau, cw, invscale;
+  mt_ion* ion;
+  const double* u;
+  long long n_u;
+  mt_event* ev;
+  int max_ev;
+  long long* n_used;
+};
+hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t stream);
+}  // namespace mt
+
+using namespace mt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHECK(x)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(MT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 16));
+    if (e == hipSuccess) bytes = std::max<size_t>(n, 16);
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+}  // namespace
+
+struct mt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cus = 0;
+  double clock_khz = 100000.0;
+  std::vector<CoefRow> coef;
+  HostTarget tgt;
+  bool has_target = false;
+  mt_tuning tune{};
+  DBuf tables, sh, cl, cx, cy, cz, cr;
+  DBuf pka_sp, pka_E, pka_pos, pka_dir;
+  DBuf ctrl, counters, wave_energy, energy;
+  DBuf hist[5];
+  DBuf per[8];
+  DBuf q[12];
+  uint64_t qcap = 0;
+  // single-flight path: device tables per ion species (Z, m), event buffers
+  std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
+  DBuf f_ion, f_u, f_ev, f_used;
+  DBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
+
+  ~mt_ctx() {
+    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &pka_sp, &pka_E, &pka_pos, &pka_dir, &ctrl,
+                   &counters, &wave_energy, &energy, &scratch_a, &scratch_b, &scratch_c, &scratch_d,
+                   &scratch_e};
+    for (DBuf* b : all) b->release();
+    for (auto& b : hist) b.release();
+    for (auto& b : per) b.release();
+    for (auto& b : q) b.release();
+    for (auto& kv : flight_tables) kv.second->release();
+    f_ion.release();
+    f_u.release();
+    f_ev.release();
+    f_used.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+const char* mt_last_error(void) { return g_err.c_str(); }
+const char* mt_version(void) { return "mytrim_amd 0.1 (gfx950)"; }
+
+int mt_ctx_create(int device, mt_ctx** out) {
+  if (!out) return fail(MT_E_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(MT_E_ARG, "no such device");
+  HIPCHECK(hipSetDevice(device));
+  mt_ctx* c = new mt_ctx;
+  c->device = device;
+  c->coef = default_coefs();
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
+  int khz = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(MT_E_HIP, std::string("context set-up: ") + hipGetErrorString(e));
+  }
+  if (khz > 0) c->clock_khz = khz;
+  *out = c;
+  return MT_OK;
+}
+
+int mt_ctx_destroy(mt_ctx* ctx) {
+  if (!ctx) return fail(MT_E_ARG, "ctx is NULL");
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return MT_OK;
+}
+
+int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols) {
+  if (!ctx || !rows) return fail(MT_E_ARG, "NULL argument");
+  if (n_rows != 92 || n_cols != 17) return fail(MT_E_ARG, "tables must be 92 x 17");
+  for (int i = 0; i < 92; ++i) {
+    const double* v = rows + 17 * i;
+    CoefRow& c = ctx->coef[i];
+    c.mm1 = v[0];
+    c.m1 = v[1];
+    c.mnat = v[2];
+    c.rho = v[3];
+    c.atrho = v[4];
+    c.vfermi = v[5];
+    c.heat = v[6];
+    for (int j = 0; j < 8; ++j) c.pcoef[j] = v[7 + j];
+    c.lfctr = v[15];
+  }
+  return MT_OK;
+}
+
+int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* t) {
+  if (!ctx || !t) return fail(MT_E_ARG, "NULL argument");
+  if (t->block && (t->block % 64 || t->block > 1024)) return fail(MT_E_ARG, "block must be a multiple of 64");
+  if (t->queue_cap && (t->queue_cap & (t->queue_cap - 1))) return fail(MT_E_ARG, "queue_cap must be a power of two");
+  if (t->stack_cap < 0 || t->stack_cap > 1024) return fail(MT_E_ARG, "stack_cap out of range");
+  ctx->tune = *t;
+  return MT_OK;
+}
+
+int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
+  if (!ctx || !t) return fail(MT_E_ARG, "NULL argument");
+  if (t->n_mat < 1 || t->n_mat > kMaxMaterials) return fail(MT_E_ARG, "n_mat out of range");
+  HostTarget tg;
+  int n_el = 0;
+  for (int k = 0; k < t->n_mat; ++k) {
+    HostMaterial m;
+    m.rho = t->mat_rho[k];
+    if (!(m.rho > 0.0)) return fail(MT_E_ARG, "material density must be positive");
+    const int b = t->mat_el_off[k], e = t->mat_el_off[k + 1];
+    if (e <= b) return fail(MT_E_ARG, "material without elements");
+    for (int i = b; i < e; ++i) {
+      if (t->el_Z[i] < 1 || t->el_Z[i] > 92) return fail(MT_E_ARG, "element Z out of range 1..92");
+      if (!(t->el_m[i] > 0.0)) return fail(MT_E_ARG, "element mass must be positive");
+      m.el.push_back(HostElement{t->el_Z[i], t->el_m[i], t->el_t[i], t->el_Edisp[i], t->el_Elbind[i]});
+    }
+    n_el += e - b;
+    m.prepare();
+    if (!(m.am > 0.0) || !(m.arho > 0.0)) return fail(MT_E_ARG, "stoichiometry invalid");
+    tg.mat.push_back(m);
+  }
+  if (n_el > kMaxElements) return fail(MT_E_LIMIT, "too many elements");
+  Geometry& g = tg.geo;
+  std::memset(&g, 0, sizeof(g));
+  g.kind = t->geo;
+  g.n_mat = t->n_mat;
+  for (int i = 0; i < 3; ++i) {
+    g.w[i] = t->w[i];
+    g.bc[i] = t->bc[i];
+    if (!(g.w[i] > 0.0)) return fail(MT_E_ARG, "sample size must be positive");
+    if (g.bc[i] < 0 || g.bc[i] > 2) return fail(MT_E_ARG, "bad boundary condition");
+  }
+  if (g.kind == MT_GEO_LAYERS) {
+    if (t->n_layers < 1 || t->n_layers > kMaxLayers) return fail(MT_E_ARG, "n_layers out of range");
+    g.n_layers = t->n_layers;
+    double d = 0.0;
+    for (int i = 0; i < t->n_layers; ++i) {  // sample_layers.C:32-37
+      d += t->layer_thick[i];
+      g.layer_end[i] = d;
+      tg.layers.push_back(t->layer_thick[i]);
+    }
+  } else if (g.kind == MT_GEO_CLUSTERS) {
+    if (t->n_mat < 2) return fail(MT_E_ARG, "clusters need two materials (matrix, inside)");
+    for (int i = 0; i < 3; ++i) {
+      if (t->kn[i] < 1) return fail(MT_E_ARG, "kn must be >= 1");
+      g.kn[i] = t->kn[i];
+      g.kd[i] = g.w[i] / double(g.kn[i]);
+    }
+    const size_t ncell = size_t(g.kn[0]) * g.kn[1] * g.kn[2];
+    if (ncell > (size_t(1) << 28)) return fail(MT_E_LIMIT, "spatial hash too large");
+    tg.sh.assign(ncell, -1);
+    tg.cl.assign(size_t(std::max(t->n_cl, 0)), -1);
+    g.cmr = 0.0;
+    for (int cn = 0; cn < t->n_cl; ++cn) {  // sampleClusters::addCluster (sample_clusters.C:187-224)
+      const double c[3] = {t->cl_x[cn], t->cl_y[cn], t->cl_z[cn]};
+      int k[3];
+      for (int i = 0; i < 3; ++i) {
+        k[i] = int(std::floor((c[i] * g.kn[i]) / g.w[i])) % g.kn[i];
+        k[i] = (k[i] < 0) ? k[i] + g.kn[i] : k[i];
+      }
+      int l = k[0] + g.kn[0] * (k[1] + g.kn[1] * k[2]);
+      if (tg.sh[l] < 0) {
+        tg.sh[l] = cn;
+      } else {
+        l = tg.sh[l];
+        while (tg.cl[l] >= 0) l = tg.cl[l];
+        tg.cl[l] = cn;
+      }
+      if (t->cl_r[cn] > g.cmr) g.cmr = t->cl_r[cn];
+      tg.clusters.push_back(HostCluster{c[0], c[1], c[2], t->cl_r[cn]});
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t n = std::max<size_t>(tg.clusters.size(), 1);
+    HIPCHECK(ctx->sh.ensure(ncell * sizeof(int)));
+    HIPCHECK(ctx->cl.ensure(n * sizeof(int)));
+    HIPCHECK(ctx->cx.ensure(n * sizeof(double)));
+    HIPCHECK(ctx->cy.ensure(n * sizeof(double)));
+    HIPCHECK(ctx->cz.ensure(n * sizeof(double)));
+    HIPCHECK(ctx->cr.ensure(n * sizeof(double)));
+    std::vector<double> v[4];
+    for (auto& c : tg.clusters) {
+      v[0].push_back(c.x);
+      v[1].push_back(c.y);
+      v[2].push_back(c.z);
+      v[3].push_back(c.r);
+    }
+    HIPCHECK(hipMemcpy(ctx->sh.p, tg.sh.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
+    if (!tg.clusters.empty()) {
+      HIPCHECK(hipMemcpy(ctx->cl.p, tg.cl.data(), tg.cl.size() * sizeof(int), hipMemcpyHostToDevice));
+      DBuf* dst[4] = {&ctx->cx, &ctx->cy, &ctx->cz, &ctx->cr};
+      for (int i = 0; i < 4; ++i)
+        HIPCHECK(hipMemcpy(dst[i]->p, v[i].data(), v[i].size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    g.sh = ctx->sh.as<int>();
+    g.cl = ctx->cl.as<int>();
+    g.cx = ctx->cx.as<double>();
+    g.cy = ctx->cy.as<double>();
+    g.cz = ctx->cz.as<double>();
+    g.cr = ctx->cr.as<double>();
+  } else if (g.kind != MT_GEO_SOLID) {
+    return fail(MT_E_ARG, "unknown geometry kind");
+  }
+  if (t->potential < 0 || t->potential > 2) return fail(MT_E_ARG, "unknown potential");
+  if (!(t->scale > 0.0)) return fail(MT_E_ARG, "length scale must be positive");
+  tg.tmin = t->tmin;
+  tg.tau = t->tau;
+  tg.cw = t->cw;
+  tg.scale = t->scale;
+  tg.potential = t->potential;
+  ctx->tgt = tg;
+  ctx->has_target = true;
+  for (auto& kv : ctx->flight_tables) kv.second->release();
+  ctx->flight_tables.clear();
+  return MT_OK;
+}
+
+int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_tallies* out) {
+  if (!ctx || !pk || !o || !out) return fail(MT_E_ARG, "NULL argument");
+  if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
+  if (pk->n_hist < 0 || pk->n_hist > (1ll << 31)) return fail(MT_E_LIMIT, "n_hist out of range");
+  if (pk->hist0 < 0 || pk->hist0 + pk->n_hist > (1ll << 32)) return fail(MT_E_LIMIT, "history index out of range");
+  if (o->arith != MT_FP64) return fail(MT_E_ARG, "arithmetic mode not available in this build");
+  if (o->range_nb < 1 || o->lat_nb < 1 || o->vac_nb < 1) return fail(MT_E_ARG, "histograms need >= 1 bin");
+  if (!(o->range_bw > 0.0) || !(o->lat_bw > 0.0) || !(o->vac_bw > 0.0)) return fail(MT_E_ARG, "bin widths must be positive");
+  HIPCHECK(hipSetDevice(ctx->device));
+  const long long n_hist = pk->n_hist;
+  const long long n_entries = pk->pka_uniform ? 1 : n_hist;
+  std::vector<Species> species;
+  std::vector<int> pka_species;
+  std::string msg = collect_species(ctx->tgt, pk->Z, pk->m, n_entries, species, pka_species);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  std::vector<unsigned char> blob;
+  msg = build_tables(ctx->coef, ctx->tgt, species, blob);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  double E0 = 0.0;
+  for (long long i = 0; i < n_entries; ++i) {
+    if (!(pk->E[i] > 0.0)) return fail(MT_E_ARG, "primary energy must be positive");
+    E0 += pk->E[i];
+  }
+  if (pk->pka_uniform) E0 = pk->E[0] * double(n_hist);
+
+  hipStream_t st = ctx->stream;
+  HIPCHECK(ctx->tables.ensure(blob.size()));
+  HIPCHECK(hipMemcpyAsync(ctx->tables.p, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+  const size_t ne = size_t(std::max<long long>(n_entries, 1));
+  HIPCHECK(ctx->pka_sp.ensure(ne * sizeof(int)));
+  HIPCHECK(ctx->pka_E.ensure(ne * sizeof(double)));
+  HIPCHECK(ctx->pka_pos.ensure(3 * ne * sizeof(double)));
+  HIPCHECK(ctx->pka_dir.ensure(3 * ne * sizeof(double)));
+  HIPCHECK(hipMemcpyAsync(ctx->pka_sp.p, pka_species.data(), n_entries * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->pka_E.p, pk->E, n_entries * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->pka_pos.p, pk->pos, 3 * n_entries * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->pka_dir.p, pk->dir, 3 * n_entries * sizeof(double), hipMemcpyHostToDevice, st));
+
+  // launch shape
+  const int block = ctx->tune.block ? ctx->tune.block : 256;
+  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 64;
+  const int nwb = block / 64;
+  const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
+  size_t lds = blob.size() + size_t(nwb) * cascade_stack_bytes(cap);
+  int lds_hist = ctx->tune.lds_hist >= 0 ? 1 : 0;
+  if (lds + lds_hist_bytes > 64 * 1024) lds_hist = 0;
+  if (lds_hist) lds += lds_hist_bytes;
+  if (lds > 64 * 1024) return fail(MT_E_LIMIT, "tables and stacks exceed 64 KiB of LDS");
+  const int pot = ctx->tgt.potential;
+  const bool perhist = o->per_history != 0;
+  int bpc = 0;
+  HIPCHECK(cascade_occupancy(pot, ctx->tgt.geo.kind, perhist, block, lds, &bpc));
+  if (ctx->tune.blocks_per_cu > 0) bpc = std::min(bpc, ctx->tune.blocks_per_cu);
+  if (bpc < 1) return fail(MT_E_LIMIT, "cascade kernel does not fit on a CU");
+  const int grid = bpc * ctx->cus;
+  const int waves = grid * nwb;
+  if (waves > 0xffff) return fail(MT_E_LIMIT, "too many waves for the idle counter");
+
+  // queue
+  uint64_t qcap = ctx->tune.queue_cap ? uint64_t(ctx->tune.queue_cap)
+                                      : std::min<uint64_t>(uint64_t(1) << 26,
+                                                           std::max<uint64_t>(uint64_t(1) << 16, next_pow2(uint64_t(n_hist) * 64)));
+  if (ctx->qcap != qcap) {
+    const size_t sz[12] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 8, 4};
+    for (int i = 0; i < 12; ++i) HIPCHECK(ctx->q[i].ensure(qcap * sz[i]));
+    ctx->qcap = qcap;
+  }
+  HIPCHECK(hipMemsetAsync(ctx->q[11].p, 0, qcap * sizeof(uint32_t), st));
+  int log2cap = 0;
+  while ((uint64_t(1) << log2cap) < qcap) ++log2cap;
+
+  // tallies
+  const int nb[5] = {o->range_nb + 2, o->lat_nb + 2, o->lat_nb + 2, o->vac_nb + 2, o->vac_nb + 2};
+  for (int i = 0; i < 5; ++i) {
+    HIPCHECK(ctx->hist[i].ensure(nb[i] * sizeof(unsigned long long)));
+    HIPCHECK(hipMemsetAsync(ctx->hist[i].p, 0, nb[i] * sizeof(unsigned long long), st));
+  }
+  HIPCHECK(ctx->ctrl.ensure(kNumCtrl * sizeof(unsigned long long)));
+  HIPCHECK(ctx->counters.ensure(kNumCounters * sizeof(unsigned long long)));
+  HIPCHECK(ctx->energy.ensure(kNumEnergy * sizeof(double)));
+  HIPCHECK(ctx->wave_energy.ensure(size_t(waves) * 2 * kNumEnergy * sizeof(double)));
+  HIPCHECK(hipMemsetAsync(ctx->ctrl.p, 0, kNumCtrl * sizeof(unsigned long long), st));
+  HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, kNumCounters * sizeof(unsigned long long), st));
+  HIPCHECK(hipMemsetAsync(ctx->energy.p, 0, kNumEnergy * sizeof(double), st));
+  HIPCHECK(hipMemsetAsync(ctx->wave_energy.p, 0, size_t(waves) * 2 * kNumEnergy * sizeof(double), st));
+  const size_t nh = size_t(std::max<long long>(n_hist, 1));
+  const size_t per_sz[8] = {4, 8, 4, 8, 8, 8, 4, 8};
+  if (perhist) {
+    for (int i = 0; i < 8; ++i) {
+      HIPCHECK(ctx->per[i].ensure(nh * per_sz[i]));
+      HIPCHECK(hipMemsetAsync(ctx->per[i].p, 0, nh * per_sz[i], st));
+    }
+  }
+
+  Params P;
+  std::memset(&P, 0, sizeof(P));
+  P.tables = ctx->tables.as<unsigned char>();
+  P.table_bytes = int(blob.size());
+  P.stack_cap = cap;
+  P.geo = ctx->tgt.geo;
+  P.tau = ctx->tgt.tau;
+  P.cw = ctx->tgt.cw;
+  P.invscale = 1.0 / ctx->tgt.scale;
+  P.Ef = pk->Ef;
+  P.follow_max_gen = o->follow_max_gen;
+  P.kp = o->kp;
+  P.n_hist = n_hist;
+  P.hist0 = pk->hist0;
+  P.pka_uniform = pk->pka_uniform ? 1 : 0;
+  P.pka_species = ctx->pka_sp.as<int>();
+  P.pka_E = ctx->pka_E.as<double>();
+  P.pka_pos = ctx->pka_pos.as<double>();
+  P.pka_dir = ctx->pka_dir.as<double>();
+  P.seed = pk->seed;
+  P.ctrl = ctx->ctrl.as<unsigned long long>();
+  Queue& Q = P.q;
+  Q.px = ctx->q[0].as<double>();
+  Q.py = ctx->q[1].as<double>();
+  Q.pz = ctx->q[2].as<double>();
+  Q.dx = ctx->q[3].as<double>();
+  Q.dy = ctx->q[4].as<double>();
+  Q.dz = ctx->q[5].as<double>();
+  Q.E = ctx->q[6].as<double>();
+  Q.meta = ctx->q[7].as<uint32_t>();
+  Q.tag = ctx->q[8].as<int>();
+  Q.hist = ctx->q[9].as<uint32_t>();
+  Q.key = ctx->q[10].as<uint64_t>();
+  Q.ready = ctx->q[11].as<uint32_t>();
+  Q.mask = qcap - 1;
+  Q.log2cap = log2cap;
+  P.admit_limit = (long long)(qcap / 4);
+  P.max_ic = 10000000;
+  P.max_wait_ticks = (unsigned long long)(60.0 * ctx->clock_khz * 1000.0);
+  P.counters = ctx->counters.as<unsigned long long>();
+  P.wave_energy = ctx->wave_energy.as<double>();
+  P.energy = ctx->energy.as<double>();
+  P.range_hist = ctx->hist[0].as<unsigned long long>();
+  P.lat_y_hist = ctx->hist[1].as<unsigned long long>();
+  P.lat_z_hist = ctx->hist[2].as<unsigned long long>();
+  P.vac_hist = ctx->hist[3].as<unsigned long long>();
+  P.repl_hist = ctx->hist[4].as<unsigned long long>();
+  P.range_x0 = o->range_x0;
+  P.range_bw = o->range_bw;
+  P.range_nb = o->range_nb;
+  P.lat_y0 = o->lat_y0;
+  P.lat_z0 = o->lat_z0;
+  P.lat_bw = o->lat_bw;
+  P.lat_nb = o->lat_nb;
+  P.vac_x0 = o->vac_x0;
+  P.vac_bw = o->vac_bw;
+  P.vac_nb = o->vac_nb;
+  P.lds_hist = lds_hist;
+  if (perhist) {
+    P.h_vac = ctx->per[0].as<int>();
+    P.h_coll = ctx->per[1].as<unsigned long long>();
+    P.h_flights = ctx->per[2].as<int>();
+    P.h_x = ctx->per[3].as<double>();
+    P.h_y = ctx->per[4].as<double>();
+    P.h_z = ctx->per[5].as<double>();
+    P.h_state = ctx->per[6].as<int>();
+    P.h_Eel = ctx->per[7].as<double>();
+  }
+
+  HIPCHECK(hipEventRecord(ctx->ev0, st));
+  HIPCHECK(cascade_launch(P, pot, perhist, grid, block, lds, st));
+  HIPCHECK(hipEventRecord(ctx->ev1, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+
+  unsigned long long ctrl[kNumCtrl], cnt[kNumCounters];
+  double en[kNumEnergy];
+  std::vector<double> we(size_t(waves) * 2 * kNumEnergy);
+  HIPCHECK(hipMemcpy(ctrl, ctx->ctrl.p, sizeof(ctrl), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(cnt, ctx->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(en, ctx->energy.p, sizeof(en), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(we.data(), ctx->wave_energy.p, we.size() * sizeof(double), hipMemcpyDeviceToHost));
+
+  out->n_coll = (long long)cnt[cColl];
+  out->n_flight = (long long)cnt[cFlight];
+  out->vac = (long long)cnt[cVac];
+  out->repl = (long long)cnt[cRepl];
+  out->subst = (long long)cnt[cSubst];
+  out->inter = (long long)cnt[cInter];
+  out->lost = (long long)cnt[cLost];
+  out->vacuum = (long long)cnt[cVacuum];
+  out->rutherford = (long long)cnt[cRuth];
+  out->newton = (long long)cnt[cNewton];
+  out->disp = (long long)cnt[cDisp];
+  out->range_n = (long long)cnt[cRangeN];
+  // compensated sum of the per-wave energy partials
+  double sums[2] = {0.0, 0.0}, errs[2] = {0.0, 0.0};
+  for (int w = 0; w < waves; ++w)
+    for (int q = 0; q < 2; ++q) {
+      const double* p = &we[size_t(w) * 2 * kNumEnergy + 2 * q];
+      for (int z = 0; z < 2; ++z) {
+        const double x = p[z];
+        const double t = sums[q] + x;
+        errs[q] += (std::fabs(sums[q]) >= std::fabs(x)) ? ((sums[q] - t) + x) : ((x - t) + sums[q]);
+        sums[q] = t;
+      }
+    }
+  out->E0 = E0;
+  out->Eel = sums[0] + errs[0];
+  out->Enuc = sums[1] + errs[1];
+  out->Eunf = en[eEunf];
+  out->Elost = en[eElost];
+  out->Edrop = en[eEdrop];
+  out->Evac = en[eEvac];
+  out->range_sum = en[eRangeSum];
+  out->range_sum2 = en[eRangeSum2];
+  long long* hdst[5] = {out->range_hist, out->lat_y_hist, out->lat_z_hist, out->vac_hist, out->repl_hist};
+  for (int i = 0; i < 5; ++i)
+    if (hdst[i]) HIPCHECK(hipMemcpy(hdst[i], ctx->hist[i].p, nb[i] * sizeof(long long), hipMemcpyDeviceToHost));
+  if (perhist) {
+    void* pdst[8] = {out->h_vac, out->h_coll, out->h_flights, out->h_x,
+                     out->h_y,   out->h_z,    out->h_state,   out->h_Eel};
+    for (int i = 0; i < 8; ++i)
+      if (pdst[i]) HIPCHECK(hipMemcpy(pdst[i], ctx->per[i].p, size_t(n_hist) * per_sz[i], hipMemcpyDeviceToHost));
+  }
+  out->kernel_ms = ms;
+  out->grid = grid;
+  out->block = block;
+  out->waves = waves;
+  out->queue_peak = (long long)ctrl[kQueuePeak];
+  out->queue_ops = (long long)ctrl[kQueueOps];
+  const unsigned errbits = unsigned(ctrl[kError]);
+  if (errbits) {
+    std::string m = "device error flags:";
+    if (errbits & kErrQueueFull) m += " queue-full";
+    if (errbits & kErrRunaway) m += " runaway-flight";
+    if (errbits & kErrBadEntry) m += " bad-queue-entry";
+    if (errbits & kErrTimeout) m += " wait-timeout";
+    if (errbits & kErrNewton) m += " newton-cap";
+    return fail(MT_E_DEVICE, m);
+  }
+  return MT_OK;
+}
+
+int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, long long* n_used, mt_event* events,
+                   int max_events, int* n_events) {
+  if (!ctx || !ion || !n_used || !n_events || (n_u > 0 && !u) || (max_events > 0 && !events))
+    return fail(MT_E_ARG, "NULL argument");
+  if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
+  if (ion->Z < 1 || ion->Z > 92 || !(ion->m > 0.0)) return fail(MT_E_ARG, "ion Z/m out of range");
+  if (n_u < 1 || max_events < 1) return fail(MT_E_ARG, "need at least one draw and one event slot");
+  HIPCHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const auto key = std::make_pair(ion->Z, ion->m);
+  auto it = ctx->flight_tables.find(key);
+  if (it == ctx->flight_tables.end()) {
+    std::vector<Species> species;
+    std::vector<int> ps;
+    std::string msg = collect_species(ctx->tgt, &ion->Z, &ion->m, 1, species, ps);
+    if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+    std::vector<unsigned char> blob;
+    msg = build_tables(ctx->coef, ctx->tgt, species, blob);
+    if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+    std::unique_ptr<DBuf> b(new DBuf);
+    HIPCHECK(b->ensure(blob.size()));
+    HIPCHECK(hipMemcpy(b->p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    it = ctx->flight_tables.emplace(key, std::move(b)).first;
+  }
+  HIPCHECK(ctx->f_ion.ensure(sizeof(mt_ion)));
+  HIPCHECK(ctx->f_u.ensure(size_t(n_u) * sizeof(double)));
+  HIPCHECK(ctx->f_ev.ensure(size_t(max_events) * sizeof(mt_event)));
+  HIPCHECK(ctx->f_used.ensure(3 * sizeof(long long)));
+  HIPCHECK(hipMemcpyAsync(ctx->f_ion.p, ion, sizeof(mt_ion), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->f_u.p, u, size_t(n_u) * sizeof(double), hipMemcpyHostToDevice, st));
+  FlightParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.tables = it->second->as<unsigned char>();
+  P.geo = ctx->tgt.geo;
+  P.tau = ctx->tgt.tau;
+  P.cw = ctx->tgt.cw;
+  P.invscale = 1.0 / ctx->tgt.scale;
+  P.ion = ctx->f_ion.as<mt_ion>();
+  P.u = ctx->f_u.as<double>();
+  P.n_u = n_u;
+  P.ev = ctx->f_ev.as<mt_event>();
+  P.max_ev = max_events;
+  P.n_used = ctx->f_used.as<long long>();
+  HIPCHECK(flight_launch(P, ctx->tgt.potential, st));
+  long long used[3];
+  HIPCHECK(hipMemcpyAsync(used, ctx->f_used.p, sizeof(used), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  *n_used = used[0];
+  *n_events = int(used[1]);
+  if (used[2] == 1) return fail(MT_E_LIMIT, "uniform stream exhausted");
+  if (used[2] == 2) return fail(MT_E_LIMIT, "event buffer full");
+  if (used[2] == 3) return fail(MT_E_DEVICE, "runaway flight");
+  HIPCHECK(hipMemcpyAsync(ion, ctx->f_ion.p, sizeof(mt_ion), hipMemcpyDeviceToHost, st));
+  if (used[1] > 0)
+    HIPCHECK(hipMemcpyAsync(events, ctx->f_ev.p, size_t(used[1]) * sizeof(mt_event), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, long long n, double* se_out) {
+  if (!ctx || (n > 0 && (!E || !se_out))) return fail(MT_E_ARG, "NULL argument");
+  if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
+  if (mat < 0 || mat >= int(ctx->tgt.mat.size())) return fail(MT_E_ARG, "material index out of range");
+  if (Z1 < 1 || Z1 > 92) return fail(MT_E_ARG, "Z1 out of range 1..92");
+  if (n <= 0) return MT_OK;
+  HIPCHECK(hipSetDevice(ctx->device));
+  std::vector<Species> species;
+  std::vector<int> ps;
+  std::string msg = collect_species(ctx->tgt, &Z1, &m1, 1, species, ps);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  std::vector<unsigned char> blob;
+  msg = build_tables(ctx->coef, ctx->tgt, species, blob);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  hipStream_t st = ctx->stream;
+  HIPCHECK(ctx->tables.ensure(blob.size()));
+  HIPCHECK(ctx->scratch_a.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_b.ensure(n * sizeof(double)));
+  HIPCHECK(hipMemcpyAsync(ctx->tables.p, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, E, n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(stopping_launch(ctx->tables.as<unsigned char>(), ps[0], mat, ctx->scratch_a.as<double>(), n,
+                           ctx->scratch_b.as<double>(), st));
+  HIPCHECK(hipMemcpyAsync(se_out, ctx->scratch_b.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+int mt_scatter_batch(mt_ctx* ctx, int potential, const double* eps, const double* b, long long n, double* s2,
+                     double* ct, double* st_out, int* iters) {
+  if (!ctx || (n > 0 && (!eps || !b || !s2 || !ct || !st_out || !iters))) return fail(MT_E_ARG, "NULL argument");
+  if (potential < 0 || potential > 2) return fail(MT_E_ARG, "unknown potential");
+  if (n <= 0) return MT_OK;
+  HIPCHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  HIPCHECK(ctx->scratch_a.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_b.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_c.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_d.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_e.ensure(n * 2 * sizeof(double)));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, eps, n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_b.p, b, n * sizeof(double), hipMemcpyHostToDevice, st));
+  double* d_s2 = ctx->scratch_c.as<double>();
+  double* d_ct = ctx->scratch_d.as<double>();
+  double* d_st = ctx->scratch_e.as<double>();
+  int* d_it = reinterpret_cast<int*>(d_st + n);
+  HIPCHECK(scatter_launch(potential, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n, d_s2, d_ct, d_st,
+                          d_it, st));
+  HIPCHECK(hipMemcpyAsync(s2, d_s2, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(ct, d_ct, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(st_out, d_st, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(iters, d_it, n * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+}  // extern "C"

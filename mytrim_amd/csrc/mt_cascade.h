@@ -1,0 +1,94 @@
+// Launch parameters of the persistent cascade kernel (shared by the host
+// launcher and the device code).
+#pragma once
+#include <stdint.h>
+
+#include "mt_tables.h"
+
+namespace mt {
+
+// HBM recoil queue: structure-of-arrays ring with a lap tag per slot.
+struct Queue {
+  double *px, *py, *pz, *dx, *dy, *dz, *E;
+  uint32_t* meta;   // species | (gen << 8)
+  int* tag;
+  uint32_t* hist;
+  uint64_t* key;
+  uint32_t* ready;  // lap number (position / capacity + 1) once the slot is written
+  uint64_t mask;    // capacity - 1
+  int log2cap;
+};
+
+// control words (one HBM allocation)
+enum Ctrl {
+  kState = 0,      // epoch << 32 | live waves << 16 | idle waves
+  kPkaNext = 1,    // next primary to start
+  kHead = 2,       // next queue position to pop
+  kTail = 3,       // next queue position to push
+  kError = 4,      // error bits
+  kQueueOps = 5,   // wave-level queue transfers
+  kQueuePeak = 6,  // largest tail - head seen at a push
+  kNumCtrl = 8
+};
+
+enum ErrBits {
+  kErrQueueFull = 1,
+  kErrRunaway = 2,    // flight exceeded max collisions
+  kErrBadEntry = 4,   // queue entry failed validation
+  kErrTimeout = 8,    // a wait exceeded the wall-clock limit
+  kErrNewton = 16     // Newton solve hit its iteration cap
+};
+
+enum Counter {
+  cColl = 0, cFlight, cVac, cRepl, cSubst, cInter, cLost, cVacuum, cRuth, cNewton, cDisp,
+  cRangeN, kNumCounters = 16
+};
+
+enum Energy { eEel = 0, eEnuc, eEunf, eElost, eEdrop, eEvac, eRangeSum, eRangeSum2, kNumEnergy = 8 };
+
+struct Params {
+  const unsigned char* tables;  // table blob (TableHeader + arrays)
+  int table_bytes;              // multiple of 16
+  int stack_cap;                // LDS recoil stack entries per wave
+  Geometry geo;
+  double tau, cw, invscale, Ef;
+  int follow_max_gen, kp;
+  // primaries
+  long long n_hist, hist0;
+  int pka_uniform;
+  const int* pka_species;
+  const double* pka_E;
+  const double* pka_pos;
+  const double* pka_dir;
+  uint32_t seed;
+  // work distribution
+  unsigned long long* ctrl;
+  Queue q;
+  long long admit_limit;        // no new primaries while the queue holds more
+  long long max_ic;             // collisions per flight before kErrRunaway
+  unsigned long long max_wait_ticks;  // wall-clock ticks before kErrTimeout
+  // tallies
+  unsigned long long* counters;   // kNumCounters
+  double* wave_energy;            // per wave: 2 * kNumEnergy (sum, compensation)
+  double* energy;                 // kNumEnergy, rare events (atomic)
+  unsigned long long *range_hist, *lat_y_hist, *lat_z_hist, *vac_hist, *repl_hist;
+  double range_x0, range_bw;
+  int range_nb;
+  double lat_y0, lat_z0, lat_bw;
+  int lat_nb;
+  double vac_x0, vac_bw;
+  int vac_nb;
+  int lds_hist;                   // bin vac/repl depths in LDS, flush at exit
+  // per history (index hist - hist0)
+  int* h_vac;
+  unsigned long long* h_coll;
+  int* h_flights;
+  double *h_x, *h_y, *h_z;
+  int* h_state;
+  double* h_Eel;
+};
+
+// LDS bytes of one wave's recoil stack entry
+constexpr int kStackEntryBytes = 7 * 8 + 8 + 4 + 4 + 4 + 4;  // doubles, key, meta, tag, hist, pad
+
+}  // namespace mt

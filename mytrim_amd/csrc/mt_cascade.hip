@@ -1,0 +1,690 @@
+// Persistent cascade kernel: every lane of every resident wave runs one ion
+// or recoil flight (TrimBase::trim, trim.C:35-425) one collision per loop
+// iteration; finished lanes refill from the wave's LDS recoil stack, then
+// from the HBM queue, then from the primaries (the FIFO drain of
+// apps/runmytrim.C:71-93, reordered: every flight's result depends only on
+// its own counter-based random stream, not on when it runs).
+//
+// Work distribution
+//   * recoils pushed by a wave go to its LDS stack (LIFO, depth-first); what
+//     does not fit goes to an HBM ring of structure-of-arrays slots with a
+//     lap tag each (one atomic per wave push/pop; release/acquire fences at
+//     agent scope so any XCD can take them)
+//   * primaries are claimed in wave batches from a counter, and only while
+//     the HBM queue holds fewer than admit_limit recoils
+//   * termination: a wave that finds no work registers idle in a packed
+//     (epoch | live | idle) word; every wave leaves once all live waves are
+//     idle, the queue is empty and the primaries are used up, observed twice
+//     with the same epoch.  Waits are bounded by a wall-clock limit.
+//
+// Tallies: wave-uniform counters in scalar registers, per-lane compensated
+// FP64 energy sums, histogram bins in LDS (or HBM atomics) flushed at exit.
+#include <hip/hip_runtime.h>
+
+#include "mt_cascade.h"
+#include "mt_geometry.h"
+#include "mt_philox.h"
+#include "mt_physics.h"
+
+namespace mt {
+
+namespace {
+
+__device__ __forceinline__ int lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {  // lane 0's value, in scalar registers
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// relaxed agent-scope load, made wave-uniform
+__device__ __forceinline__ uint64_t ld_relaxed(unsigned long long* p) {
+  return uni64(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Neumaier-compensated accumulation: value = s + e
+__device__ __forceinline__ void kadd(double& s, double& e, double x) {
+  const double t = s + x;
+  e += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+  s = t;
+}
+
+__device__ __forceinline__ int bin_of(double x, double x0, double bw, int nb) {
+  const double t = (x - x0) / bw;
+  if (!(t >= 0.0)) return 0;
+  if (t >= (double)nb) return nb + 1;
+  return 1 + (int)t;
+}
+
+// one flight's state, held in registers
+struct Flight {
+  double px, py, pz, dx, dy, dz, E;
+  int sp, gen, tag;
+  uint32_t hist;
+  uint64_t key;
+  int ic;
+  double eel;  // electronic loss of this flight (per-history tally)
+};
+
+struct StackView {  // one wave's LDS stack, structure of arrays
+  double *px, *py, *pz, *dx, *dy, *dz, *E;
+  uint64_t* key;
+  uint32_t* meta;
+  int* tag;
+  uint32_t* hist;
+  __device__ void bind(unsigned char* base, int cap) {
+    px = (double*)base;
+    py = px + cap;
+    pz = py + cap;
+    dx = pz + cap;
+    dy = dx + cap;
+    dz = dy + cap;
+    E = dz + cap;
+    key = (uint64_t*)(E + cap);
+    meta = (uint32_t*)(key + cap);
+    tag = (int*)(meta + cap);
+    hist = (uint32_t*)(tag + cap);
+  }
+};
+
+__host__ __device__ constexpr int stack_bytes(int cap) { return ((kStackEntryBytes - 4) * cap + 15) & ~15; }
+
+}  // namespace
+
+template <int POT, int GEO, bool PERHIST>
+__global__ void __launch_bounds__(256) cascade_fp64(Params P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  {
+    const uint4* src = (const uint4*)P.tables;
+    uint4* dst = (uint4*)lds;
+    for (int i = threadIdx.x; i < P.table_bytes / 16; i += blockDim.x) dst[i] = src[i];
+  }
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nwb = blockDim.x >> 6;
+  const int cap = P.stack_cap;
+  StackView S;
+  S.bind(lds + P.table_bytes + wid * stack_bytes(cap), cap);
+  uint32_t* lvac = (uint32_t*)(lds + P.table_bytes + nwb * stack_bytes(cap));
+  uint32_t* lrepl = lvac + (P.vac_nb + 2);
+  if (P.lds_hist)
+    for (int i = threadIdx.x; i < 2 * (P.vac_nb + 2); i += blockDim.x) lvac[i] = 0u;
+  __syncthreads();
+
+  const TableHeader* H = (const TableHeader*)lds;
+  TableView T;
+  T.n_species = H->n_species;
+  T.n_mat = H->n_mat;
+  T.n_el = H->n_el;
+  T.potential = H->potential;
+  T.mat = (const MatConst*)(lds + H->off_mat);
+  T.el = (const ElemConst*)(lds + H->off_el);
+  T.stop = (const StopConst*)(lds + H->off_stop);
+  const Geometry& G = P.geo;
+
+  unsigned long long* ctrl = P.ctrl;
+  if (lane == 0) atomicAdd(&ctrl[kState], 1ull << 16);  // live += 1
+
+  // wave-uniform state
+  int scount = 0;          // entries in this wave's LDS stack
+  bool wave_idle = false;  // registered idle
+  unsigned long long idle_since = 0;
+  unsigned long long c_coll = 0, c_flight = 0, c_vac = 0, c_repl = 0, c_subst = 0, c_inter = 0,
+                     c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0;
+  unsigned long long c_qops = 0;
+  // per-lane accumulators
+  long long l_newton = 0, l_kpvac = 0;
+  double eel_s = 0.0, eel_e = 0.0, enuc_s = 0.0, enuc_e = 0.0;
+  unsigned err = 0;
+
+  bool active = false;
+  Flight F;
+  F.ic = 0;
+
+  while (true) {
+    // ------------------------------------------------------------ refill
+    uint64_t idle_m = __ballot(!active);
+    if (idle_m) {
+      int n_idle = __popcll(idle_m);
+      const int take = min(n_idle, scount);
+      if (take > 0) {
+        const int r = lane_rank(idle_m);
+        if (!active && r < take) {
+          const int j = scount - 1 - r;
+          F.px = S.px[j];
+          F.py = S.py[j];
+          F.pz = S.pz[j];
+          F.dx = S.dx[j];
+          F.dy = S.dy[j];
+          F.dz = S.dz[j];
+          F.E = S.E[j];
+          F.key = S.key[j];
+          const uint32_t m = S.meta[j];
+          F.sp = (int)(m & 0xffu);
+          F.gen = (int)(m >> 8);
+          F.tag = S.tag[j];
+          F.hist = S.hist[j];
+          F.ic = 0;
+          F.eel = 0.0;
+          active = true;
+        }
+        scount -= take;
+        idle_m = __ballot(!active);
+        n_idle = __popcll(idle_m);
+      }
+      if (n_idle > 0) {
+        // HBM queue: claim up to n_idle positions with one CAS
+        const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
+        const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
+        const bool may_pop = t0 > h0;
+        const bool may_pka = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
+        if (wave_idle && (may_pop || may_pka)) {
+          if (lane == 0) atomicAdd(&ctrl[kState], (1ull << 32) - 1ull);  // epoch += 1, idle -= 1
+          wave_idle = false;
+        }
+        if (may_pop) {
+          uint64_t got = 0, base = 0;
+          if (lane == 0) {
+            for (int tries = 0; tries < 8; ++tries) {
+              const uint64_t h = ld_relaxed(&ctrl[kHead]);
+              const uint64_t t = ld_relaxed(&ctrl[kTail]);
+              if (t <= h) break;
+              const uint64_t k = min((uint64_t)n_idle, t - h);
+              unsigned long long expect = h;
+              if (__hip_atomic_compare_exchange_strong(&ctrl[kHead], &expect, h + k, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                got = k;
+                base = h;
+                break;
+              }
+            }
+          }
+          got = bcast64(got, 0);
+          base = bcast64(base, 0);
+          if (got > 0) {
+            ++c_qops;
+            const int r = lane_rank(idle_m);
+            bool mine = !active && r < (int)got;
+            uint64_t slot = 0;
+            if (mine) {
+              const uint64_t pos = base + (uint64_t)r;
+              slot = pos & P.q.mask;
+              const uint32_t lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
+              const unsigned long long t_start = (unsigned long long)wall_clock64();
+              while (__hip_atomic_load(&P.q.ready[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                     lap) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((unsigned long long)wall_clock64() - t_start > P.max_wait_ticks) {
+                  err |= kErrTimeout;
+                  mine = false;
+                  break;
+                }
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (mine) {
+              F.px = P.q.px[slot];
+              F.py = P.q.py[slot];
+              F.pz = P.q.pz[slot];
+              F.dx = P.q.dx[slot];
+              F.dy = P.q.dy[slot];
+              F.dz = P.q.dz[slot];
+              F.E = P.q.E[slot];
+              F.key = P.q.key[slot];
+              const uint32_t m = P.q.meta[slot];
+              F.sp = (int)(m & 0xffu);
+              F.gen = (int)(m >> 8);
+              F.tag = P.q.tag[slot];
+              F.hist = P.q.hist[slot];
+              F.ic = 0;
+              F.eel = 0.0;
+              if (F.sp >= T.n_species) {
+                err |= kErrBadEntry;
+              } else {
+                active = true;
+              }
+            }
+            idle_m = __ballot(!active);
+            n_idle = __popcll(idle_m);
+          }
+        }
+        if (n_idle > 0 && may_pka) {
+          const uint64_t h = ld_relaxed(&ctrl[kHead]);
+          const uint64_t t = ld_relaxed(&ctrl[kTail]);
+          if ((long long)(t - h) < P.admit_limit) {
+            uint64_t first = 0;
+            if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_idle);
+            first = bcast64(first, 0);
+            const int r = lane_rank(idle_m);
+            const long long j = (long long)first + r;
+            if (!active && r < n_idle && j < P.n_hist) {
+              const long long e = P.pka_uniform ? 0 : j;
+              F.px = P.pka_pos[3 * e + 0];
+              F.py = P.pka_pos[3 * e + 1];
+              F.pz = P.pka_pos[3 * e + 2];
+              F.dx = P.pka_dir[3 * e + 0];
+              F.dy = P.pka_dir[3 * e + 1];
+              F.dz = P.pka_dir[3 * e + 2];
+              F.E = P.pka_E[e];
+              F.sp = P.pka_species[e];
+              F.gen = 0;
+              F.tag = -1;
+              F.hist = (uint32_t)(P.hist0 + j);
+              F.key = 0;
+              F.ic = 0;
+              F.eel = 0.0;
+              active = true;
+            }
+          }
+        }
+      }
+      if (__ballot(active) == 0ull) {
+        // nothing to run: idle protocol and termination test
+        if (!wave_idle) {
+          if (lane == 0) atomicAdd(&ctrl[kState], 1ull);
+          wave_idle = true;
+          idle_since = (unsigned long long)wall_clock64();
+        }
+        const uint64_t s1 = ld_relaxed(&ctrl[kState]);
+        if ((s1 & 0xffffull) == ((s1 >> 16) & 0xffffull)) {
+          const uint64_t pn = ld_relaxed(&ctrl[kPkaNext]);
+          const uint64_t h = ld_relaxed(&ctrl[kHead]);
+          const uint64_t t = ld_relaxed(&ctrl[kTail]);
+          const uint64_t s2 = ld_relaxed(&ctrl[kState]);
+          if (s1 == s2 && h == t && pn >= (unsigned long long)P.n_hist) break;
+        }
+        if ((unsigned long long)wall_clock64() - idle_since > P.max_wait_ticks) {
+          err |= kErrTimeout;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+        continue;
+      }
+    }
+
+    // ------------------------------------------------------------ one collision
+    bool ended = false, vacuum = false, push = false, ruth = false;
+    int state = 0;
+    double rx = 0.0, ry = 0.0, rz = 0.0, rdx = 0.0, rdy = 0.0, rdz = 0.0, rE = 0.0;
+    int rsp = 0, rtag = -1;
+    if (active) {
+      ++F.ic;
+      int mtag;
+      const double pos[3] = {F.px, F.py, F.pz};
+      const int k = lookup_material<GEO>(G, pos, mtag);
+      if (k < 0) {
+        vacuum = true;
+        ended = true;
+      } else {
+        {  // v_norm (functions.h:66-70)
+          const double inv = 1.0 / sqrt(F.dx * F.dx + F.dy * F.dy + F.dz * F.dz);
+          F.dx *= inv;
+          F.dy *= inv;
+          F.dz *= inv;
+        }
+        const uint32_t klo = (uint32_t)F.key, khi = (uint32_t)(F.key >> 32);
+        const P4 blk = philox10((uint32_t)F.ic, klo, khi, 0u, P.seed, F.hist);
+        const MatConst& mc = T.mat[F.sp * T.n_mat + k];
+        // free path and maximum impact parameter (trim.C:88-95)
+        double eps = F.E * mc.f;
+        const double eeg = sqrt(eps * mc.epsdg);
+        const double pmax = mc.a / (eeg + sqrt(eeg) + 0.125 * pow(eeg, 0.1));
+        double ls = 1.0 / (M_PI * (pmax * pmax) * mc.arho);
+        if (F.ic == 1) {
+          const P4 b0 = philox10(0u, klo, khi, 0u, P.seed, F.hist);
+          ls = u53(b0.x, b0.y) * mn(ls, P.cw);
+        }
+        // impact parameter and target atom (trim.C:143-160)
+        const double p = pmax * sqrt(u53(blk.x, blk.y));
+        double hh = u32d(blk.z);
+        const int eb = F.sp * T.n_el + mc.el_begin;
+        int nn = 0;
+        for (; nn < mc.n_el; ++nn) {
+          hh -= T.el[eb + nn].t;
+          if (hh <= 0.0) break;
+        }
+        if (nn >= mc.n_el) nn = mc.n_el - 1;
+        const ElemConst& ec = T.el[eb + nn];
+        eps = ec.fi * F.E;
+        const double b = p / ec.ai;
+        // electronic loss along ls (trim.C:166-170)
+        double dee = ls * getrstop<double>(T, F.sp, k, F.E);
+        // scattering (trim.C:172-272)
+        double s2, ct, st;
+        int it;
+        scatter<double, POT>(eps, b, s2, ct, st, it);
+        ruth = (it == 0);
+        l_newton += it;
+        if (it >= kMaxNewton) err |= kErrNewton;
+        // energy transfer and momenta (trim.C:275-297)
+        double den = ec.ec * s2 * F.E;
+        if (dee > F.E) dee = F.E;
+        F.E -= dee;
+        kadd(eel_s, eel_e, dee);
+        if (PERHIST) F.eel += dee;
+        const double m1 = H->species[F.sp].m;
+        const double p1 = sqrt(2.0 * m1 * F.E);
+        if (den > F.E) den = F.E;
+        F.E -= den;
+        const double p2 = sqrt(2.0 * m1 * F.E);
+        // recoil at the collision site, ion moves on (trim.C:306-318)
+        rx = F.px;
+        ry = F.py;
+        rz = F.pz;
+        const double step = (ls - P.tau) * P.invscale;
+        F.px += F.dx * step;
+        F.py += F.dy * step;
+        F.pz += F.dz * step;
+        rdx = F.dx * p1;
+        rdy = F.dy * p1;
+        rdz = F.dz * p1;
+        rE = den - ec.Elbind;
+        // azimuth about the flight direction (replaces trim.C:320-333)
+        double ux, uy, uz;
+        {
+          const double ax = fabs(F.dx), ay = fabs(F.dy), az = fabs(F.dz);
+          if (ax <= ay && ax <= az) {
+            const double n = 1.0 / sqrt(F.dy * F.dy + F.dz * F.dz);
+            ux = 0.0;
+            uy = F.dz * n;
+            uz = -F.dy * n;
+          } else if (ay <= az) {
+            const double n = 1.0 / sqrt(F.dx * F.dx + F.dz * F.dz);
+            ux = -F.dz * n;
+            uy = 0.0;
+            uz = F.dx * n;
+          } else {
+            const double n = 1.0 / sqrt(F.dx * F.dx + F.dy * F.dy);
+            ux = F.dy * n;
+            uy = -F.dx * n;
+            uz = 0.0;
+          }
+        }
+        const double vx = F.dy * uz - F.dz * uy;
+        const double vy = F.dz * ux - F.dx * uz;
+        const double vz = F.dx * uy - F.dy * ux;
+        double sphi, cphi;
+        sincospi(2.0 * u32d(blk.w), &sphi, &cphi);
+        const double qx = cphi * ux + sphi * vx;
+        const double qy = cphi * uy + sphi * vy;
+        const double qz = cphi * uz + sphi * vz;
+        // new direction (trim.C:336-341): psi = atan2(st, ct + my), so
+        // cos psi and sin psi are the normalised arguments (st >= 0)
+        const double xpsi = ct + ec.my;
+        const double ipsi = 1.0 / sqrt(xpsi * xpsi + st * st);
+        const double cpsi = xpsi * ipsi;
+        const double spsi = st * ipsi;
+        F.dx = F.dx * cpsi + qx * spsi;
+        F.dy = F.dy * cpsi + qy * spsi;
+        F.dz = F.dz * cpsi + qz * spsi;
+        rdx -= F.dx * p2;
+        rdy -= F.dy * p2;
+        rdz -= F.dz * p2;
+        // boundaries (trim.C:344-352)
+        bool lost = false;
+        if (G.bc[0] == 2 && (F.px > G.w[0] || F.px < 0.0)) lost = true;
+        if (G.bc[1] == 2 && (F.py > G.w[1] || F.py < 0.0)) lost = true;
+        if (G.bc[2] == 2 && (F.pz > G.w[2] || F.pz < 0.0)) lost = true;
+        // fate of recoil and ion (trim.C:357-411)
+        if (lost) {
+          state = 4;
+          atomicAdd(&P.energy[eElost], F.E);
+          atomicAdd(&P.energy[eEdrop], den);
+        } else if (rE > ec.Edisp - ec.Elbind) {
+          const bool follow = P.follow_max_gen < 0 || F.gen + 1 < P.follow_max_gen;
+          if (follow) {
+            push = true;
+            kadd(enuc_s, enuc_e, ec.Elbind);
+            const double inv = 1.0 / sqrt(rdx * rdx + rdy * rdy + rdz * rdz);
+            rdx *= inv;
+            rdy *= inv;
+            rdz *= inv;
+            rsp = ec.recoil_species;
+            rtag = mtag;
+          } else {
+            atomicAdd(&P.energy[eEunf], den);
+          }
+          if (F.E > ec.Edisp) {
+            // vacancyCreation (trim.C:439-443, TrimVacCount.C:31-42, KP trim.C:445-464)
+            const int bn = bin_of(rx, P.vac_x0, P.vac_bw, P.vac_nb);
+            if (P.lds_hist)
+              atomicAdd(&lvac[bn], 1u);
+            else
+              atomicAdd(&P.vac_hist[bn], 1ull);
+            int extra = 0;
+            if (P.kp && F.gen + 1 == P.follow_max_gen) {
+              const double ed = 0.0115 * pow(mc.az, -7.0 / 3.0) * rE;
+              const double g = 3.4008 * pow(ed, 1.0 / 6.0) + 0.40244 * pow(ed, 3.0 / 4.0) + ed;
+              const double kd = 0.1337 * pow(mc.az, 2.0 / 3.0) / sqrt(mc.am);
+              const double Ev = rE / (1.0 + kd * g);
+              extra = (int)(0.8 * Ev / (2.0 * ec.Edisp));
+              l_kpvac += extra;
+            }
+            if (PERHIST) atomicAdd(&P.h_vac[F.hist - (uint32_t)P.hist0], 1 + extra);
+            state = -1;  // vacancy marker, ion keeps moving
+          } else {
+            const int bn = bin_of(rx, P.vac_x0, P.vac_bw, P.vac_nb);
+            if (P.lds_hist)
+              atomicAdd(&lrepl[bn], 1u);
+            else
+              atomicAdd(&P.repl_hist[bn], 1ull);
+            state = (H->species[F.sp].Z == ec.Z) ? 1 : 2;
+            kadd(enuc_s, enuc_e, F.E);
+          }
+        } else {
+          kadd(enuc_s, enuc_e, rE + ec.Elbind);
+          if (F.E < P.Ef) {
+            state = 3;
+            kadd(enuc_s, enuc_e, F.E);
+          }
+        }
+        ended = state > 0;
+        if (F.ic >= P.max_ic && !ended) {
+          err |= kErrRunaway;
+          ended = true;
+        }
+      }
+    }
+    // wave-uniform event counts
+    c_coll += __popcll(__ballot(active && !vacuum));
+    c_ruth += __popcll(__ballot(active && !vacuum && ruth));
+    c_vac += __popcll(__ballot(active && state == -1));
+    c_repl += __popcll(__ballot(active && state == 1));
+    c_subst += __popcll(__ballot(active && state == 2));
+    c_inter += __popcll(__ballot(active && state == 3));
+    c_lost += __popcll(__ballot(active && state == 4));
+    c_disp += __popcll(__ballot(active && (state == -1 || state == 1 || state == 2)));
+
+    // ------------------------------------------------------------ push recoils
+    const uint64_t push_m = __ballot(push);
+    if (push_m) {
+      const int n_push = __popcll(push_m);
+      const int room = cap - scount;
+      const int r = lane_rank(push_m);
+      const uint32_t meta = (uint32_t)rsp | ((uint32_t)(F.gen + 1) << 8);
+      const uint64_t rkey = child_key(F.key, (uint32_t)F.ic);
+      if (push && r < room) {
+        const int j = scount + r;
+        S.px[j] = rx;
+        S.py[j] = ry;
+        S.pz[j] = rz;
+        S.dx[j] = rdx;
+        S.dy[j] = rdy;
+        S.dz[j] = rdz;
+        S.E[j] = rE;
+        S.key[j] = rkey;
+        S.meta[j] = meta;
+        S.tag[j] = rtag;
+        S.hist[j] = F.hist;
+      }
+      const int n_over = n_push - min(n_push, room);
+      scount += min(n_push, room);
+      if (n_over > 0) {
+        ++c_qops;
+        uint64_t base = 0, head = 0;
+        if (lane == 0) {
+          base = atomicAdd(&ctrl[kTail], (unsigned long long)n_over);
+          head = ld_relaxed(&ctrl[kHead]);
+        }
+        base = bcast64(base, 0);
+        head = bcast64(head, 0);
+        // consumers may already have claimed reserved slots, so head can be
+        // past base: the occupancy is a signed difference
+        const long long fill = (long long)(base + (uint64_t)n_over) - (long long)head;
+        if (lane == 0 && fill > 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
+        if (fill > (long long)P.q.mask) err |= kErrQueueFull;  // caller discards the run
+        const bool over = push && r >= room;
+        uint64_t slot = 0;
+        uint32_t lap = 0;
+        if (over) {
+          const uint64_t pos = base + (uint64_t)(r - room);
+          slot = pos & P.q.mask;
+          lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
+          P.q.px[slot] = rx;
+          P.q.py[slot] = ry;
+          P.q.pz[slot] = rz;
+          P.q.dx[slot] = rdx;
+          P.q.dy[slot] = rdy;
+          P.q.dz[slot] = rdz;
+          P.q.E[slot] = rE;
+          P.q.key[slot] = rkey;
+          P.q.meta[slot] = meta;
+          P.q.tag[slot] = rtag;
+          P.q.hist[slot] = F.hist;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (over) __hip_atomic_store(&P.q.ready[slot], lap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+
+    // ------------------------------------------------------------ flight ends
+    const bool fin = active && (ended || vacuum);
+    const uint64_t fin_m = __ballot(fin);
+    if (fin_m) {
+      c_flight += __popcll(fin_m);
+      c_vacuum += __popcll(__ballot(fin && vacuum));
+      if (fin) {
+        if (vacuum) atomicAdd(&P.energy[eEvac], F.E);
+        const uint32_t hl = F.hist - (uint32_t)P.hist0;
+        if (PERHIST) {
+          atomicAdd(&P.h_coll[hl], (unsigned long long)(vacuum ? F.ic - 1 : F.ic));
+          atomicAdd(&P.h_flights[hl], 1);
+          atomicAdd(&P.h_Eel[hl], F.eel);
+        }
+        if (F.gen == 0) {
+          if (PERHIST) {
+            P.h_x[hl] = F.px;
+            P.h_y[hl] = F.py;
+            P.h_z[hl] = F.pz;
+            P.h_state[hl] = vacuum ? -1 : (state > 0 ? state : 0);
+          }
+          if (!vacuum && state != 4) {
+            atomicAdd(&P.range_hist[bin_of(F.px, P.range_x0, P.range_bw, P.range_nb)], 1ull);
+            const double lo = -0.5 * P.lat_nb * P.lat_bw;
+            atomicAdd(&P.lat_y_hist[bin_of(F.py - P.lat_y0, lo, P.lat_bw, P.lat_nb)], 1ull);
+            atomicAdd(&P.lat_z_hist[bin_of(F.pz - P.lat_z0, lo, P.lat_bw, P.lat_nb)], 1ull);
+            atomicAdd(&P.energy[eRangeSum], F.px);
+            atomicAdd(&P.energy[eRangeSum2], F.px * F.px);
+          }
+        }
+        active = false;
+      }
+      c_rangen += __popcll(__ballot(fin && F.gen == 0 && !vacuum && state != 4));
+    }
+  }
+
+  // ------------------------------------------------------------ flush
+  for (int off = 32; off > 0; off >>= 1) {
+    l_newton += __shfl_xor(l_newton, off);
+    l_kpvac += __shfl_xor(l_kpvac, off);
+    const double s_a = __shfl_xor(eel_s, off), e_a = __shfl_xor(eel_e, off);
+    kadd(eel_s, eel_e, s_a);
+    eel_e += e_a;
+    const double s_b = __shfl_xor(enuc_s, off), e_b = __shfl_xor(enuc_e, off);
+    kadd(enuc_s, enuc_e, s_b);
+    enuc_e += e_b;
+  }
+  if (lane == 0) {
+    unsigned long long* C = P.counters;
+    atomicAdd(&C[cColl], c_coll);
+    atomicAdd(&C[cFlight], c_flight);
+    atomicAdd(&C[cVac], c_vac + (unsigned long long)l_kpvac);
+    atomicAdd(&C[cRepl], c_repl);
+    atomicAdd(&C[cSubst], c_subst);
+    atomicAdd(&C[cInter], c_inter);
+    atomicAdd(&C[cLost], c_lost);
+    atomicAdd(&C[cVacuum], c_vacuum);
+    atomicAdd(&C[cRuth], c_ruth);
+    atomicAdd(&C[cNewton], (unsigned long long)l_newton);
+    atomicAdd(&C[cDisp], c_disp);
+    atomicAdd(&C[cRangeN], c_rangen);
+    atomicAdd(&ctrl[kQueueOps], c_qops);
+    const int gw = blockIdx.x * nwb + wid;
+    double* we = P.wave_energy + (size_t)gw * (2 * kNumEnergy);
+    we[2 * eEel] = eel_s;
+    we[2 * eEel + 1] = eel_e;
+    we[2 * eEnuc] = enuc_s;
+    we[2 * eEnuc + 1] = enuc_e;
+  }
+  if (err) atomicOr((unsigned int*)&ctrl[kError], err);
+  if (P.lds_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < P.vac_nb + 2; i += blockDim.x) {
+      if (lvac[i]) atomicAdd(&P.vac_hist[i], (unsigned long long)lvac[i]);
+      if (lrepl[i]) atomicAdd(&P.repl_hist[i], (unsigned long long)lrepl[i]);
+    }
+  }
+}
+
+int cascade_stack_bytes(int cap) { return stack_bytes(cap); }
+
+}  // namespace mt
+
+// ---------------------------------------------------------------- launchers
+namespace mt {
+
+typedef void (*CascadeKernel)(Params);
+
+template <int POT, int GEO>
+static CascadeKernel pick2(bool perhist) {
+  return perhist ? cascade_fp64<POT, GEO, true> : cascade_fp64<POT, GEO, false>;
+}
+
+template <int POT>
+static CascadeKernel pick1(int geo, bool perhist) {
+  if (geo == 1) return pick2<POT, 1>(perhist);
+  if (geo == 2) return pick2<POT, 2>(perhist);
+  return pick2<POT, 0>(perhist);
+}
+
+static CascadeKernel pick(int potential, int geo, bool perhist) {
+  if (potential == 1) return pick1<1>(geo, perhist);
+  if (potential == 2) return pick1<2>(geo, perhist);
+  return pick1<0>(geo, perhist);
+}
+
+hipError_t cascade_occupancy(int potential, int geo, bool perhist, int block, size_t lds, int* blocks_per_cu) {
+  CascadeKernel k = pick(potential, geo, perhist);
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)k, block, lds);
+}
+
+hipError_t cascade_launch(const Params& P, int potential, bool perhist, int grid, int block, size_t lds,
+                          hipStream_t stream) {
+  CascadeKernel k = pick(potential, P.geo.kind, perhist);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, stream, P);
+  return hipGetLastError();
+}
+
+}  // namespace mt

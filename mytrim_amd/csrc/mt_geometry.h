@@ -1,0 +1,79 @@
+// Position -> material lookups of the three sample kinds, on the device.
+//   SampleSolid::lookupMaterial     sample_solid.C:25-29
+//   SampleLayers::lookupLayer       sample_layers.C:26-43 (x < 0 -> first
+//                                   layer, beyond the last -> last material)
+//   sampleClusters::lookupCluster   sample_clusters.C:43-133 (cell hash with a
+//                                   ks-cell neighbourhood and minimum-image
+//                                   distances along periodic axes)
+// GEO is the sample kind (0 solid, 1 layers, 2 clusters).  Returns the material index, or -1 for vacuum (a CUT axis left by a
+// clusters sample); `tag` receives the cluster index when inside one.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mt_tables.h"
+
+namespace mt {
+
+__device__ __forceinline__ int lookup_cluster(const Geometry& g, const double pos[3]) {
+  int k[3], k1[3], k2[3];
+  for (int i = 0; i < 3; ++i) {
+    k[i] = (int)floor((pos[i] * g.kn[i]) / g.w[i]);
+    if (pos[i] < 0.0 || pos[i] >= g.w[i]) {
+      if (g.bc[i] == 2) return -2;
+      if (g.bc[i] == 1) return -1;
+      k[i] = k[i] % g.kn[i];
+      if (k[i] < 0) k[i] += g.kn[i];
+    }
+    const int ks = (int)(g.cmr / g.kd[i]) + 1;
+    k1[i] = k[i] - ks;
+    k2[i] = k[i] + ks;
+    if (k1[i] < 0 && g.bc[i] != 0) k1[i] = 0;
+    if (k2[i] >= g.kn[i] && g.bc[i] != 0) k2[i] = g.kn[i] - 1;
+  }
+  for (int j0 = k1[0]; j0 <= k2[0]; ++j0) {
+    int c0 = j0 % g.kn[0];
+    if (c0 < 0) c0 += g.kn[0];
+    for (int j1 = k1[1]; j1 <= k2[1]; ++j1) {
+      int c1 = j1 % g.kn[1];
+      if (c1 < 0) c1 += g.kn[1];
+      for (int j2 = k1[2]; j2 <= k2[2]; ++j2) {
+        int c2 = j2 % g.kn[2];
+        if (c2 < 0) c2 += g.kn[2];
+        int l = g.sh[c0 + g.kn[0] * (c1 + g.kn[1] * c2)];
+        while (l >= 0) {
+          double d0 = pos[0] - g.cx[l];
+          double d1 = pos[1] - g.cy[l];
+          double d2 = pos[2] - g.cz[l];
+          if (g.bc[0] == 0) d0 -= round(d0 / g.w[0]) * g.w[0];
+          if (g.bc[1] == 0) d1 -= round(d1 / g.w[1]) * g.w[1];
+          if (g.bc[2] == 0) d2 -= round(d2 / g.w[2]) * g.w[2];
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          const double rc = g.cr[l];
+          if (r2 < rc * rc) return l;
+          l = g.cl[l];
+        }
+      }
+    }
+  }
+  return -1;
+}
+
+template <int GEO>
+__device__ __forceinline__ int lookup_material(const Geometry& g, const double pos[3], int& tag) {
+  tag = -1;
+  if (GEO == 0) return 0;
+  if (GEO == 1) {
+    // first layer with x < its running end (the ends are non-decreasing, so
+    // this counts a prefix); uniform loop bound keeps layer_end in scalars
+    int i = 0;
+    for (int j = 0; j < g.n_layers; ++j) i += !(pos[0] < g.layer_end[j]);
+    return i >= g.n_mat ? g.n_mat - 1 : i;
+  }
+  const int l = lookup_cluster(g, pos);
+  if (l == -2) return -1;  // vacuum beyond a CUT face
+  if (l == -1) return 0;
+  tag = l;
+  return 1;
+}
+
+}  // namespace mt

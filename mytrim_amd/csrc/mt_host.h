@@ -1,0 +1,61 @@
+// Host-side preparation of the device tables.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "mt_tables.h"
+
+namespace mt {
+
+// SCOEF.95A + SLFCTR.dat columns per element (simconf.h:71-90)
+struct CoefRow {
+  double mm1, m1, mnat, rho, atrho, vfermi, heat;
+  double pcoef[8];
+  double lfctr;
+};
+
+struct HostElement {
+  int Z;
+  double m, t, Edisp, Elbind;
+};
+
+struct HostMaterial {   // MaterialBase after prepare() (material.C:37-74)
+  double rho = 0.0, am = 0.0, az = 0.0, arho = 0.0;
+  std::vector<HostElement> el;
+  void prepare();
+};
+
+struct HostCluster {
+  double x, y, z, r;
+};
+
+struct HostTarget {
+  std::vector<HostMaterial> mat;
+  Geometry geo{};  // cluster pointers filled at upload
+  std::vector<double> layers;
+  std::vector<HostCluster> clusters;
+  std::vector<int> sh, cl;  // spatial hash of sampleClusters (sample_clusters.C:135-224)
+  double tmin = 0.2, tau = 0.0, cw = 0.001, scale = 1.0;
+  int potential = 0;
+};
+
+// the tables built into the library (mytrim_amd/data/scoef.dat)
+const std::vector<CoefRow>& default_coefs();
+
+// Table blob for the given species (index = position in `species`): every
+// (species, material) and (species, element) constant of
+// MaterialBase::average (material.C:78-110) and the (ion, target)-only parts
+// of rstop (material.C:161-282).  Returns "" or an error message.
+std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt,
+                         const std::vector<Species>& species, std::vector<unsigned char>& blob);
+
+// species index of (Z, m), appended when new
+int species_index(std::vector<Species>& species, int Z, double m);
+
+// the species list for primaries `pZ/pm` (n entries) plus every target element
+std::string collect_species(const HostTarget& tgt, const int* pZ, const double* pm, long long n,
+                            std::vector<Species>& species, std::vector<int>& pka_species);
+
+double host_rpstop(const std::vector<CoefRow>& coef, int z2p, double e);
+
+}  // namespace mt

@@ -1,0 +1,249 @@
+"""ctypes binding of the C ABI in include/mytrim_gpu.h (libmytrim_gpu.so).
+
+The library must have been built (``python -c "import __graft_entry__ as g;
+g.build()"`` or ``make -C mytrim_amd``); nothing here falls back to a CPU
+path: every call goes to the HIP kernels and raises on failure.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .problem import Problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmytrim_gpu.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+lp = C.POINTER(C.c_longlong)
+
+MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
+
+# every symbol include/mytrim_gpu.h declares
+EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt_ctx_set_tables",
+           "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_trim_flight",
+           "mt_stopping_batch", "mt_scatter_batch"]
+
+
+class Target(C.Structure):
+    _fields_ = [("n_mat", C.c_int), ("mat_rho", dp), ("mat_el_off", ip), ("el_Z", ip), ("el_m", dp),
+                ("el_t", dp), ("el_Edisp", dp), ("el_Elbind", dp), ("geo", C.c_int),
+                ("w", C.c_double * 3), ("bc", C.c_int * 3), ("n_layers", C.c_int),
+                ("layer_thick", dp), ("kn", C.c_int * 3), ("n_cl", C.c_int), ("cl_x", dp),
+                ("cl_y", dp), ("cl_z", dp), ("cl_r", dp), ("tmin", C.c_double), ("tau", C.c_double),
+                ("cw", C.c_double), ("scale", C.c_double), ("potential", C.c_int)]
+
+
+class Primaries(C.Structure):
+    _fields_ = [("n_hist", C.c_longlong), ("hist0", C.c_longlong), ("pka_uniform", C.c_int),
+                ("Z", ip), ("m", dp), ("E", dp), ("pos", dp), ("dir", dp), ("Ef", C.c_double),
+                ("seed", C.c_uint32)]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("follow_max_gen", C.c_int), ("kp", C.c_int), ("arith", C.c_int),
+                ("range_x0", C.c_double), ("range_bw", C.c_double), ("range_nb", C.c_int),
+                ("lat_y0", C.c_double), ("lat_z0", C.c_double), ("lat_bw", C.c_double),
+                ("lat_nb", C.c_int), ("vac_x0", C.c_double), ("vac_bw", C.c_double),
+                ("vac_nb", C.c_int), ("per_history", C.c_int)]
+
+
+COUNTERS = ["n_coll", "n_flight", "vac", "repl", "subst", "inter", "lost", "vacuum",
+            "rutherford", "newton", "disp"]
+ENERGIES = ["E0", "Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac"]
+
+
+class Tallies(C.Structure):
+    _fields_ = ([(k, C.c_longlong) for k in COUNTERS] + [(k, C.c_double) for k in ENERGIES] + [
+        ("range_n", C.c_longlong), ("range_sum", C.c_double), ("range_sum2", C.c_double),
+        ("range_hist", lp), ("lat_y_hist", lp), ("lat_z_hist", lp), ("vac_hist", lp),
+        ("repl_hist", lp), ("h_vac", ip), ("h_coll", lp), ("h_flights", ip), ("h_x", dp),
+        ("h_y", dp), ("h_z", dp), ("h_state", ip), ("h_Eel", dp), ("kernel_ms", C.c_double),
+        ("grid", C.c_int), ("block", C.c_int), ("waves", C.c_int), ("queue_peak", C.c_longlong),
+        ("queue_ops", C.c_longlong)])
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("stack_cap", C.c_int),
+                ("queue_cap", C.c_longlong), ("lds_hist", C.c_int)]
+
+
+class Ion(C.Structure):
+    _fields_ = [("Z", C.c_int), ("m", C.c_double), ("E", C.c_double), ("pos", C.c_double * 3),
+                ("dir", C.c_double * 3), ("gen", C.c_int), ("tag", C.c_int), ("state", C.c_int),
+                ("Ef", C.c_double)]
+
+
+class Event(C.Structure):
+    _fields_ = [("pka_E", C.c_double), ("pka_pos", C.c_double * 3), ("pka_dir", C.c_double * 3),
+                ("ls", C.c_double), ("dee", C.c_double), ("den", C.c_double), ("rec_E", C.c_double),
+                ("rec_pos", C.c_double * 3), ("rec_dir", C.c_double * 3), ("material", C.c_int),
+                ("element", C.c_int), ("pka_state", C.c_int), ("fate", C.c_int),
+                ("vacancy", C.c_int), ("mat_tag", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmytrim_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmytrim_gpu.so not built: run __graft_entry__.build() or "
+                               "make -C mytrim_amd")
+        L = C.CDLL(LIB_PATH)
+        L.mt_last_error.restype = C.c_char_p
+        L.mt_version.restype = C.c_char_p
+        L.mt_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.mt_ctx_destroy.argtypes = [C.c_void_p]
+        L.mt_ctx_set_tables.argtypes = [C.c_void_p, dp, C.c_int, C.c_int]
+        L.mt_ctx_set_target.argtypes = [C.c_void_p, C.POINTER(Target)]
+        L.mt_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.mt_run_cascades.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
+                                      C.POINTER(Tallies)]
+        L.mt_trim_flight.argtypes = [C.c_void_p, C.POINTER(Ion), dp, C.c_longlong, lp,
+                                     C.POINTER(Event), C.c_int, ip]
+        L.mt_stopping_batch.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, dp,
+                                        C.c_longlong, dp]
+        L.mt_scatter_batch.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_longlong, dp, dp, dp, ip]
+        _lib = L
+    return _lib
+
+
+class MtError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise MtError("%s failed (%d): %s" % (what, rc, lib().mt_last_error().decode()))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Engine:
+    """One device context (mt_ctx): target tables on the GPU plus its stream."""
+
+    def __init__(self, device: int = 0):
+        self._keep = []
+        h = C.c_void_p()
+        _check(lib().mt_ctx_create(device, C.byref(h)), "mt_ctx_create")
+        self.h = h
+        self.problem: Optional[Problem] = None
+
+    def close(self):
+        if self.h:
+            lib().mt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, block=0, blocks_per_cu=0, stack_cap=0, queue_cap=0, lds_hist=0):
+        t = Tuning(block, blocks_per_cu, stack_cap, queue_cap, lds_hist)
+        _check(lib().mt_ctx_set_tuning(self.h, C.byref(t)), "mt_ctx_set_tuning")
+
+    def set_target(self, prob: Problem):
+        rho, off, Z, m, t, ed, eb = prob.flat_materials()
+        g = prob.geometry
+        layers = np.ascontiguousarray(g.layers, np.float64) if len(g.layers) else np.zeros(1)
+        cl = np.ascontiguousarray(g.clusters if g.clusters is not None else np.zeros((1, 4)))
+        cols = [np.ascontiguousarray(cl[:, i]) for i in range(4)]
+        keep = [rho, off, Z, m, t, ed, eb, layers] + cols
+        tg = Target()
+        tg.n_mat = len(rho)
+        tg.mat_rho, tg.mat_el_off, tg.el_Z = _p(rho, dp), _p(off, ip), _p(Z, ip)
+        tg.el_m, tg.el_t, tg.el_Edisp, tg.el_Elbind = _p(m, dp), _p(t, dp), _p(ed, dp), _p(eb, dp)
+        tg.geo = g.kind
+        tg.w[:] = [float(x) for x in g.w]
+        tg.bc[:] = [int(x) for x in g.bc]
+        tg.n_layers = len(g.layers)
+        tg.layer_thick = _p(layers, dp)
+        tg.kn[:] = [int(x) for x in g.kn]
+        tg.n_cl = 0 if g.clusters is None else len(g.clusters)
+        tg.cl_x, tg.cl_y, tg.cl_z, tg.cl_r = (_p(c, dp) for c in cols)
+        tg.tmin, tg.tau, tg.cw, tg.scale = prob.tmin, prob.tau, prob.cw, prob.scale
+        tg.potential = prob.potential
+        _check(lib().mt_ctx_set_target(self.h, C.byref(tg)), "mt_ctx_set_target")
+        del keep
+        self.problem = prob
+
+    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64) -> dict:
+        """Full cascades for every history of `prob` (sets the target if needed)."""
+        if self.problem is not prob:
+            self.set_target(prob)
+        p = prob.primaries
+        pZ = np.ascontiguousarray(p.Z, np.int32)
+        pm = np.ascontiguousarray(p.m, np.float64)
+        pE = np.ascontiguousarray(p.E, np.float64)
+        ppos = np.ascontiguousarray(p.pos, np.float64)
+        pdir = np.ascontiguousarray(p.dir, np.float64)
+        pk = Primaries(prob.n_hist, prob.hist0, 1 if p.uniform else 0, _p(pZ, ip), _p(pm, dp),
+                       _p(pE, dp), _p(ppos, dp), _p(pdir, dp), prob.Ef, prob.seed & 0xFFFFFFFF)
+        o = RunOpts(prob.follow_max_gen, int(prob.kp), arith, prob.range_bins.x0,
+                    prob.range_bins.bw, prob.range_bins.nb, prob.lat_y0, prob.lat_z0, prob.lat_bw,
+                    prob.lat_nb, prob.vac_bins.x0, prob.vac_bins.bw, prob.vac_bins.nb,
+                    int(per_history))
+        out = Tallies()
+        hists = {"range_hist": np.zeros(prob.range_bins.nb + 2, np.int64),
+                 "lat_y_hist": np.zeros(prob.lat_nb + 2, np.int64),
+                 "lat_z_hist": np.zeros(prob.lat_nb + 2, np.int64),
+                 "vac_hist": np.zeros(prob.vac_bins.nb + 2, np.int64),
+                 "repl_hist": np.zeros(prob.vac_bins.nb + 2, np.int64)}
+        for k, a in hists.items():
+            setattr(out, k, _p(a, lp))
+        per = {}
+        if per_history:
+            n = prob.n_hist
+            per = {"h_vac": np.zeros(n, np.int32), "h_coll": np.zeros(n, np.int64),
+                   "h_flights": np.zeros(n, np.int32), "h_x": np.zeros(n), "h_y": np.zeros(n),
+                   "h_z": np.zeros(n), "h_state": np.zeros(n, np.int32), "h_Eel": np.zeros(n)}
+            for k, a in per.items():
+                setattr(out, k, _p(a, {np.int32: ip, np.int64: lp, np.float64: dp}[a.dtype.type]))
+        _check(lib().mt_run_cascades(self.h, C.byref(pk), C.byref(o), C.byref(out)),
+               "mt_run_cascades")
+        res = {k: getattr(out, k) for k in COUNTERS + ENERGIES}
+        res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
+                   kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
+                   queue_peak=out.queue_peak, queue_ops=out.queue_ops)
+        res.update(hists)
+        res.update(per)
+        return res
+
+    def stopping(self, Z1: int, m1: float, mat: int, E: np.ndarray) -> np.ndarray:
+        E = np.ascontiguousarray(E, np.float64)
+        out = np.zeros_like(E)
+        _check(lib().mt_stopping_batch(self.h, Z1, m1, mat, _p(E, dp), len(E), _p(out, dp)),
+               "mt_stopping_batch")
+        return out
+
+    def scatter(self, eps: np.ndarray, b: np.ndarray, potential: int = 0):
+        eps = np.ascontiguousarray(eps, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        n = len(eps)
+        s2, ct, st = np.zeros(n), np.zeros(n), np.zeros(n)
+        it = np.zeros(n, np.int32)
+        _check(lib().mt_scatter_batch(self.h, potential, _p(eps, dp), _p(b, dp), n, _p(s2, dp),
+                                      _p(ct, dp), _p(st, dp), _p(it, ip)), "mt_scatter_batch")
+        return s2, ct, st, it
+
+    def trim_flight(self, ion: Ion, u: np.ndarray, max_events: int = 1 << 16):
+        """One TrimBase::trim() flight driven by the uniforms `u`.  Returns
+        (draws used, events) or raises MtError when `u` ran out."""
+        u = np.ascontiguousarray(u, np.float64)
+        used = C.c_longlong()
+        nev = C.c_int()
+        ev = (Event * max_events)()
+        _check(lib().mt_trim_flight(self.h, C.byref(ion), _p(u, dp), len(u), C.byref(used), ev,
+                                    max_events, C.byref(nev)), "mt_trim_flight")
+        return used.value, ev[:nev.value]

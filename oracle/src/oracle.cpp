@@ -4,6 +4,8 @@
 // that g++ -O3 (no FMA contraction on x86-64) produces the same doubles.
 #include "oracle.h"
 
+#include "counted.h"
+
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -74,74 +76,92 @@ void Material::average(int Z, double m, double tmin) {  // material.C:78-110
   }
 }
 
-double Material::getrstop(int Z, double m, double E) const {  // material.C:114-122
-  double se = 0.0;
-  for (const auto& e : el) se += rstop(Z, m, E, e.Z) * e.t;
-  return se * arho;
+// The per-collision physics is written once over the scalar type R: double
+// for the restatement itself, Counted (counted.h) to count its operations.
+// Values that depend only on the (ion, target) pair stay double.
+template <class R>
+static inline R xmax(R a, R b) {  // std::max
+  return (a < b) ? b : a;
+}
+template <class R>
+static inline R xmin(R a, R b) {  // std::min
+  return (b < a) ? b : a;
+}
+template <class R>
+static inline R sqr(R x) {
+  return x * x;
 }
 
-double rpstop(int z2p, double e) {  // material.C:134-158
+template <class R>
+R rpstop_t(int z2p, R e) {  // material.C:134-158
+  using std::log;
+  using std::pow;
   const int z2 = z2p - 1;
-  const double pe0 = 25.0;
-  const double pe = std::max(pe0, e);
+  const R pe0 = 25.0;
+  const R pe = xmax(pe0, e);
   const double* c = g_coef[z2].pcoef;
-  const double sl = (c[0] * std::pow(pe, c[1])) + (c[2] * std::pow(pe, c[3]));
-  const double sh = c[4] / std::pow(pe, c[5]) * std::log(c[6] / pe + c[7] * pe);
-  double sp = sl * sh / (sl + sh);
+  const R sl = (c[0] * pow(pe, R(c[1]))) + (c[2] * pow(pe, R(c[3])));
+  const R sh = c[4] / pow(pe, R(c[5])) * log(c[6] / pe + c[7] * pe);
+  R sp = sl * sh / (sl + sh);
   if (e <= pe0) {
     const double velpwr = (z2p <= 6) ? 0.25 : 0.45;
-    sp *= std::pow(e / pe0, velpwr);
+    sp *= pow(e / pe0, R(velpwr));
   }
   return sp;
 }
 
-double rstop(int z1, double m_ion, double E, int z2) {  // material.C:161-282
+template <class R>
+R rstop_t(int z1, double m_ion, R E, int z2) {  // material.C:161-282
+  using std::exp;
+  using std::log;
+  using std::pow;
+  using std::sqrt;
   const double fz1 = double(z1);
   const double fz2 = double(z2);
   const double lfctr = g_coef[z1 - 1].lfctr;
   const double mm1 = g_coef[z1 - 1].mm1;
   const double vfermi = g_coef[z2 - 1].vfermi;
   const double m1 = (m_ion == 0.0) ? mm1 : m_ion;
-  const double ee = 0.001 * E;
-  const double e = ee / m1;
-  double se;
+  const R ee = 0.001 * E;
+  const R e = ee / m1;
+  R se;
   if (z1 == 1) {
-    se = rpstop(z2, e);
+    se = rpstop_t<R>(z2, e);
   } else if (z1 == 2) {
     const double he0 = 1.0;
-    double he = std::max(he0, e);
-    const double b = std::log(he);
-    const double b2 = b * b;
-    const double b4 = b2 * b2;
-    double a = 0.2865 + 0.1266 * b - 0.001429 * b2 + 0.02402 * b * b2 - 0.01135 * b4 +
-               0.001475 * b4 * b;
-    double heh = 1.0 - std::exp(-std::min(30.0, a));
-    he = std::max(he, 1.0);
-    a = 1.0 + (0.007 + 0.00005 * fz2) * std::exp(-sq(7.6 - std::log(he)));
+    R he = xmax(R(he0), e);
+    const R b = log(he);
+    const R b2 = b * b;
+    const R b4 = b2 * b2;
+    R a = 0.2865 + 0.1266 * b - 0.001429 * b2 + 0.02402 * b * b2 - 0.01135 * b4 +
+          0.001475 * b4 * b;
+    R heh = 1.0 - exp(-xmin(R(30.0), a));
+    he = xmax(he, R(1.0));
+    a = 1.0 + (0.007 + 0.00005 * fz2) * exp(-sqr(7.6 - log(he)));
     heh *= a * a;
-    const double sp = rpstop(z2, he);
+    const R sp = rpstop_t<R>(z2, he);
     se = sp * heh * 4.0;
-    if (e <= he0) se *= std::sqrt(e / he0);
+    if (e <= he0) se *= sqrt(e / he0);
   } else {
     const double yrmin = 0.13;
     double vrmin = 1.0;
-    const double v = std::sqrt(e / 25.0) / vfermi;
-    const double v2 = v * v;
-    double vr;
+    const R v = sqrt(e / 25.0) / vfermi;
+    const R v2 = v * v;
+    R vr;
     if (v >= 1.0)
       vr = v * vfermi * (1.0 + 1.0 / (5.0 * v2));
     else
       vr = (3.0 * vfermi / 4.0) * (1.0 + (2.0 * v2 / 3.0) - v2 * v2 / 15.0);
     const double cbrt_fz1 = std::cbrt(fz1);
     const double cbrt2_fz1 = cbrt_fz1 * cbrt_fz1;
-    double yr = std::max(yrmin, vr / cbrt2_fz1);
-    yr = std::max(yr, vrmin / cbrt2_fz1);
-    const double yr03 = std::pow(yr, 0.3);
-    double a = -0.803 * yr03 + 1.3167 * yr03 * yr03 + 0.38157 * yr + 0.008983 * yr * yr;
-    const double q = std::min(1.0, std::max(0.0, 1.0 - std::exp(-std::min(a, 50.0))));
+    R yr = xmax(R(yrmin), vr / cbrt2_fz1);
+    yr = xmax(yr, R(vrmin / cbrt2_fz1));
+    const R yr03 = pow(yr, R(0.3));
+    R a = -0.803 * yr03 + 1.3167 * yr03 * yr03 + 0.38157 * yr + 0.008983 * yr * yr;
+    const R q = xmin(R(1.0), xmax(R(0.0), 1.0 - exp(-xmin(a, R(50.0)))));
     const double b = (std::min(0.43, std::max(0.32, 0.12 + 0.025 * fz1))) / cbrt_fz1;
-    const double l0 = (0.8 - q * std::min(1.2, 0.6 + fz1 / 30.0)) / cbrt_fz1;
-    double l1;
+    const R l0 = (0.8 - q * std::min(1.2, 0.6 + fz1 / 30.0)) / cbrt_fz1;
+    R l1;
     if (q < 0.2)
       l1 = 0.0;
     else if (q < std::max(0.0, 0.9 - 0.025 * fz1))
@@ -150,26 +170,37 @@ double rstop(int z1, double m_ion, double E, int z2) {  // material.C:161-282
       l1 = b;
     else
       l1 = b * (1.0 - q) / (0.025 * std::min(16.0, fz1));
-    const double l = std::max(l1, l0 * lfctr);
-    double zeta = q + (1.0 / (2.0 * vfermi * vfermi)) * (1.0 - q) *
-                          std::log(1.0 + sq(4.0 * l * vfermi / 1.919));
-    a = -sq(7.6 - std::max(0.0, std::log(e)));
-    zeta *= 1.0 + (1.0 / (fz1 * fz1)) * (0.18 + 0.0015 * fz2) * std::exp(a);
+    const R l = xmax(l1, l0 * lfctr);
+    R zeta = q + (1.0 / (2.0 * vfermi * vfermi)) * (1.0 - q) *
+                     log(1.0 + sqr(4.0 * l * vfermi / 1.919));
+    a = -sqr(7.6 - xmax(R(0.0), log(e)));
+    zeta *= 1.0 + (1.0 / (fz1 * fz1)) * (0.18 + 0.0015 * fz2) * exp(a);
     if (yr <= std::max(yrmin, vrmin / cbrt2_fz1)) {
       vrmin = std::max(vrmin, yrmin * cbrt2_fz1);
       const double vmin =
           0.5 * (vrmin + std::sqrt(std::max(0.0, vrmin * vrmin - 0.8 * vfermi * vfermi)));
       const double eee = 25.0 * vmin * vmin;
-      const double sp = rpstop(z2, eee);
+      const double sp = rpstop_t<double>(z2, eee);
       const double power = (z2 == 6 || ((z2 == 14 || z2 == 32) && z1 <= 19)) ? 0.375 : 0.5;
-      se = sp * sq(zeta * fz1) * std::pow(e / eee, power);
+      se = sp * sqr(zeta * fz1) * pow(e / eee, R(power));
     } else {
-      const double sp = rpstop(z2, e);
-      se = sp * sq(zeta * fz1);
+      const R sp = rpstop_t<R>(z2, e);
+      se = sp * sqr(zeta * fz1);
     }
   }
   return se * 10.0;
 }
+
+template <class R>
+R getrstop_t(const Material& mat, int Z, double m, R E) {  // material.C:114-122
+  R se = 0.0;
+  for (const auto& e : mat.el) se += rstop_t<R>(Z, m, E, e.Z) * e.t;
+  return se * mat.arho;
+}
+
+double Material::getrstop(int Z, double m, double E) const { return getrstop_t<double>(*this, Z, m, E); }
+double rpstop(int z2p, double e) { return rpstop_t<double>(z2p, e); }
+double rstop(int z1, double m_ion, double E, int z2) { return rstop_t<double>(z1, m_ion, E, z2); }
 
 // ------------------------------------------------------------------ samples
 Sample::Sample(double x, double y, double z) {  // sample.C:25-32
@@ -288,83 +319,95 @@ void SampleClusters::addCluster(double x, double y, double z, double r) {  // sa
 }
 
 // ------------------------------------------------------------------ scattering
-Scatter scatter(double eps, double b, int potential) {  // trim.C:172-272
-  Scatter s;
+template <class R>
+void scatter_t(R eps, R b, int potential, R& s2, R& ct, R& st, int& iters) {  // trim.C:172-272
+  using std::abs;
+  using std::exp;
+  using std::log;
+  using std::pow;
+  using std::sqrt;
   if (eps > 10.0) {
-    s.s2 = 1.0 / (1.0 + (1.0 + b * (1.0 + b)) * sq(2.0 * eps * b));
-    const double c2 = 1.0 - s.s2;
-    s.ct = 2.0 * c2 - 1.0;
-    s.st = std::sqrt(1.0 - s.ct * s.ct);
-    s.iters = 0;
-    return s;
+    s2 = 1.0 / (1.0 + (1.0 + b * (1.0 + b)) * sqr(2.0 * eps * b));
+    const R c2 = 1.0 - s2;
+    ct = 2.0 * c2 - 1.0;
+    st = sqrt(1.0 - ct * ct);
+    iters = 0;
+    return;
   }
-  double r = b;
-  double rr = -2.7 * std::log(eps * b);
+  R r = b;
+  R rr = -2.7 * log(eps * b);
   if (rr >= b) {
-    rr = -2.7 * std::log(eps * rr);
+    rr = -2.7 * log(eps * rr);
     if (rr >= b) r = rr;
   }
-  double v = 0.0, v1 = 0.0, q;
+  R v = 0.0, v1 = 0.0, q;
   int it = 0;
   do {
-    double ex1, ex2, ex3, ex4;
+    R ex1, ex2, ex3, ex4;
     if (potential == 1) {  // Moliere
-      ex1 = std::exp(-0.3 * r);
-      ex2 = sq(ex1) * sq(ex1);
-      ex3 = ex2 * (sq(ex2) * sq(ex2));
+      ex1 = exp(-0.3 * r);
+      ex2 = sqr(ex1) * sqr(ex1);
+      ex3 = ex2 * (sqr(ex2) * sqr(ex2));
       v = (0.35 * ex1 + 0.55 * ex2 + 0.1 * ex3) / r;
       v1 = -(v + 0.105 * ex1 + 0.66 * ex2 + 0.6 * ex3) / r;
     } else if (potential == 2) {  // C-Kr
-      ex1 = std::exp(-0.279 * r);
-      ex2 = std::exp(-0.637 * r);
-      ex3 = std::exp(-1.1919 * r);
+      ex1 = exp(-0.279 * r);
+      ex2 = exp(-0.637 * r);
+      ex3 = exp(-1.1919 * r);
       v = (0.191 * ex1 + 0.474 * ex2 + 0.335 * ex3) / r;
       v1 = -(v + 0.531865 * ex1 + 0.30181 * ex2 + 0.6437 * ex3) / r;
     } else {  // universal
-      ex1 = 0.18175 * std::exp(-3.1998 * r);
-      ex2 = 0.50986 * std::exp(-0.94229 * r);
-      ex3 = 0.28022 * std::exp(-0.4029 * r);
-      ex4 = 0.028171 * std::exp(-0.20162 * r);
+      ex1 = 0.18175 * exp(-3.1998 * r);
+      ex2 = 0.50986 * exp(-0.94229 * r);
+      ex3 = 0.28022 * exp(-0.4029 * r);
+      ex4 = 0.028171 * exp(-0.20162 * r);
       v = (ex1 + ex2 + ex3 + ex4) / r;
       v1 = -(v + 3.1998 * ex1 + 0.94229 * ex2 + 0.4029 * ex3 + 0.20162 * ex4) / r;
     }
-    const double fr = b * b / r + v * r / eps - r;
-    const double fr1 = -b * b / (r * r) + (v + v1 * r) / eps - 1.0;
+    const R fr = b * b / r + v * r / eps - r;
+    const R fr1 = -b * b / (r * r) + (v + v1 * r) / eps - 1.0;
     q = fr / fr1;
     r -= q;
     ++it;
-  } while (std::abs(q / r) > 0.001);
-  const double roc = -2.0 * (eps - v) / v1;
-  const double sqe = std::sqrt(eps);
-  double cc, aa, ff;
+  } while (abs(q / r) > 0.001);
+  const R roc = -2.0 * (eps - v) / v1;
+  const R sqe = sqrt(eps);
+  R cc, aa, ff;
   if (potential == 1) {
     cc = (0.009611 + sqe) / (0.005175 + sqe);
-    aa = 2.0 * eps * (1.0 + (0.6743 / sqe)) * std::pow(b, cc);
-    ff = (std::sqrt(aa * aa + 1.0) - aa) * ((6.314 + eps) / (10.0 + eps));
+    aa = 2.0 * eps * (1.0 + (0.6743 / sqe)) * pow(b, cc);
+    ff = (sqrt(aa * aa + 1.0) - aa) * ((6.314 + eps) / (10.0 + eps));
   } else if (potential == 2) {
     cc = (0.235809 + sqe) / (0.126000 + sqe);
-    aa = 2.0 * eps * (1.0 + (1.0144 / sqe)) * std::pow(b, cc);
-    ff = (std::sqrt(aa * aa + 1.0) - aa) * ((6935.0 + eps) / (83550.0 + eps));
+    aa = 2.0 * eps * (1.0 + (1.0144 / sqe)) * pow(b, cc);
+    ff = (sqrt(aa * aa + 1.0) - aa) * ((6935.0 + eps) / (83550.0 + eps));
   } else {
     cc = (0.011615 + sqe) / (0.0071222 + sqe);
-    aa = 2.0 * eps * (1.0 + (0.99229 / sqe)) * std::pow(b, cc);
-    ff = (std::sqrt(aa * aa + 1.0) - aa) * ((9.3066 + eps) / (14.813 + eps));
+    aa = 2.0 * eps * (1.0 + (0.99229 / sqe)) * pow(b, cc);
+    ff = (sqrt(aa * aa + 1.0) - aa) * ((9.3066 + eps) / (14.813 + eps));
   }
-  const double delta = (r - b) * aa * ff / (ff + 1.0);
-  const double co = (b + delta + roc) / (r + roc);
-  const double c2 = co * co;
-  s.s2 = 1.0 - c2;
-  s.ct = 2.0 * c2 - 1.0;
-  s.st = std::sqrt(1.0 - s.ct * s.ct);
-  s.iters = it;
+  const R delta = (r - b) * aa * ff / (ff + 1.0);
+  const R co = (b + delta + roc) / (r + roc);
+  const R c2 = co * co;
+  s2 = 1.0 - c2;
+  ct = 2.0 * c2 - 1.0;
+  st = sqrt(1.0 - ct * ct);
+  iters = it;
+}
+
+Scatter scatter(double eps, double b, int potential) {
+  Scatter s;
+  scatter_t<double>(eps, b, potential, s.s2, s.ct, s.st, s.iters);
   return s;
 }
 
 // ------------------------------------------------------------------ draws
 void MtDraws::perp(const double d[3], double out[3]) {  // trim.C:322-333
   double rdir[3], norm;
+  last_tries = 0;
   do {
     do {
+      ++last_tries;
       for (int i = 0; i < 3; ++i) rdir[i] = 2.0 * rng.drand() - 1.0;
     } while (rdir[0] * rdir[0] + rdir[1] * rdir[1] + rdir[2] * rdir[2] > 1.0);
     for (int i = 0; i < 3; ++i)  // v_cross, functions.h:37-42
@@ -458,6 +501,9 @@ void trim(Ion* pka, std::deque<Ion*>& recoils, Sample& sample, const Config& cfg
     h.element = &e;
     eps = e.fi * pka->E;
     const double b = p / e.ai;
+    h.E_in = pka->E;
+    h.eps_in = eps;
+    h.b_in = b;
     const double see = mat->getrstop(pka->Z, pka->m, pka->E);
     double dee = ls * see;
     const Scatter sc = scatter(eps, b, cfg.potential);
@@ -547,6 +593,19 @@ void trim(Ion* pka, std::deque<Ion*>& recoils, Sample& sample, const Config& cfg
 template void trim<MtDraws>(Ion*, std::deque<Ion*>&, Sample&, const Config&, MtDraws&, Hooks&);
 template void trim<PhiloxDraws>(Ion*, std::deque<Ion*>&, Sample&, const Config&, PhiloxDraws&,
                                 Hooks&);
+
+// ------------------------------------------------------------------ counting
+thread_local OpCount* g_ops = nullptr;
+
+void count_physics(const Material& mat, int Z, double m, double E, double eps, double b,
+                   int potential, OpCount& ops) {
+  g_ops = &ops;
+  Counted s2, ct, st;
+  int it;
+  (void)getrstop_t<Counted>(mat, Z, m, Counted(E));
+  scatter_t<Counted>(Counted(eps), Counted(b), potential, s2, ct, st, it);
+  g_ops = nullptr;
+}
 
 // ------------------------------------------------------------------ uo2 driver
 namespace {

@@ -168,6 +168,9 @@ struct Hooks {
   const Element* element = nullptr;
   double ls = 0.0, dee = 0.0, den = 0.0;
   int newton_iters = 0;  // of the last collision (0 = Rutherford branch)
+  // inputs of the last collision's physics (ion energy before it, reduced
+  // energy and impact parameter), recorded for operation counting
+  double E_in = 0.0, eps_in = 0.0, b_in = 0.0;
   long long vacancies_created = 0;
   double EelTotal = 0.0, EnucTotal = 0.0;
   int next_id = 0;
@@ -187,6 +190,7 @@ struct Hooks {
 // the unit-ball rejection loop for the azimuth (trim.C:71, 143, 147, 322-333)
 struct MtDraws {
   MtRng& rng;
+  int last_tries = 0;  // unit-ball draws of the last azimuth (operation counting)
   explicit MtDraws(MtRng& r) : rng(r) {}
   double first(const Ion&) { return rng.drand(); }
   void collision(const Ion&, int) {}
@@ -216,6 +220,12 @@ void trim(Ion* pka, std::deque<Ion*>& recoils, Sample& sample, const Config& cfg
           Hooks& h);
 
 // ---------------------------------------------------------------- drivers
+// Operations of one collision's stopping and scattering evaluated on the
+// same inputs with the counting scalar (oracle/src/counted.h).
+struct OpCount;
+void count_physics(const Material& mat, int Z, double m, double E, double eps, double b,
+                   int potential, OpCount& ops);
+
 // apps/mytrim_uo2.C:48-358 in PLAIN mode; returns the three output files
 struct Uo2Out {
   std::string erec, clcoor, dist;

@@ -89,9 +89,13 @@ void orc_average(int Z1, double m1, double tmin, double rho, int n_el, const int
 int orc_scatter(double eps, double b, int potential, double* out3);
 void orc_philox(const unsigned* ctr4, const unsigned* key2, unsigned* out4);
 unsigned long long orc_flight_key_child(unsigned long long parent, unsigned ic);
+void orc_mt_stream(unsigned seed, long long skip, long long n, double* out);
+unsigned orc_mt_history_seed(unsigned master, long long h);
 int orc_uo2(int seed, double r, double cbf, int nev, char* erec, int erec_len, char* clcoor,
             int clcoor_len, char* dist, int dist_len);
 int orc_run(const orc_run_spec* spec, orc_run_out* out);
+/* algorithmic work per collision: 16 doubles (see runs.cpp) */
+int orc_workcount(const orc_run_spec* spec, double* out);
 /* per-collision trace of history `h` (rows of 21 doubles, see runs.cpp) */
 int orc_trace(const orc_run_spec* spec, long long h, double* rows, int max_rows);
 

@@ -18,6 +18,7 @@
 #include <memory>
 #include <thread>
 
+#include "counted.h"
 #include "oracle.h"
 #include "oracle_capi.h"
 
@@ -182,6 +183,75 @@ Ion* make_primary(const orc_run_spec& s, long long h) {
   return ion;
 }
 
+// Operation count of the reference's collision loop (trim.C:74-424).
+// Stopping and scattering are counted by evaluating them with the counting
+// scalar on the collision's own inputs; the straight-line code of the loop is
+// counted here statement by statement (add/sub, mul, div, cmp, sqrt, ...):
+//   v_norm(dir)                    mul 6, add 2, sqrt 1, div 1      (trim.C:85)
+//   eps, eeg, pmax, ls             mul 6, add 2, div 2, sqrt 2, pow 1 (trim.C:88-92)
+//   first step r1*min(ls, cw)      cmp 1, mul 1                     (trim.C:93-94)
+//   p = pmax sqrt(r2)              mul 1, sqrt 1                    (trim.C:144)
+//   element pick                   add 1, cmp 1 per element tried   (trim.C:148-154)
+//   eps, b                         mul 1, div 1                     (trim.C:159-160)
+//   dee, den, clamps, E, p1, p2    mul 7, add 3, cmp 2, sqrt 2      (trim.C:170, 275-296)
+//   move + recoil dir and E        mul 9, add 5                     (trim.C:310-316)
+//   azimuth, per unit-ball try     mul 6, add 5, cmp 1              (trim.C:324-328)
+//   azimuth, once                  mul 9, add 5, sqrt 1, cmp 1, div 3 (trim.C:330-333)
+//   psi, new dir, recoil dir       add 7, mul 9, atan2, cos, sin    (trim.C:336-341)
+//   CUT test                       cmp 2 per CUT axis               (trim.C:344-352)
+//   fate                           add 1, cmp 2                     (trim.C:359-406)
+//   queued recoil v_norm           mul 6, add 2, sqrt 1, div 1      (trim.C:364)
+//   layer lookup                   add 1, cmp 1 per layer tried     (sample_layers.C:32-37)
+// Uniform draws are counted apart (they are integer work on the device).
+struct WorkHooks : Hooks {
+  const orc_run_spec& s;
+  OpCount ops;
+  long long coll = 0, uniforms = 0, flights = 0;
+  MtDraws* draws = nullptr;
+  Sample* sample = nullptr;
+  WorkHooks(const orc_run_spec& spec) : s(spec) {}
+  bool followRecoil() override {
+    ops.mul += 6;
+    ops.add += 2;
+    ops.sqrt_ += 1;
+    ops.div += 1;
+    return true;
+  }
+  void checkPKAState() override {
+    count_physics(*material, pka->Z, pka->m, E_in, eps_in, b_in, s.potential, ops);
+    ++coll;
+    ops.mul += 6 + 6 + 1 + 1 + 7 + 9 + 9 + 9;
+    ops.add += 2 + 2 + 3 + 5 + 5 + 7 + 1;
+    ops.div += 1 + 2 + 1 + 3;
+    ops.sqrt_ += 1 + 2 + 1 + 2 + 1;
+    ops.pow_ += 1;
+    ops.cmp += 2 + 1 + 2;
+    ops.atan2_ += 1;
+    ops.cos_ += 1;
+    ops.sin_ += 1;
+    const long long tried = (element - &material->el[0]) + 1;
+    ops.add += tried;
+    ops.cmp += tried;
+    const int t = draws ? draws->last_tries : 1;
+    ops.mul += 6 * t;
+    ops.add += 5 * t;
+    ops.cmp += t;
+    for (int i = 0; i < 3; ++i)
+      if (s.bc[i] == CUT) ops.cmp += 2;
+    if (s.geo == 1) {
+      int k = 0;
+      while (k < int(sample->material.size()) && sample->material[k] != material) ++k;
+      ops.add += k + 1;
+      ops.cmp += k + 1;
+    }
+    uniforms += 2 + 3 * t;
+  }
+  void flightEnd(bool) override {
+    ++flights;
+    ++uniforms;  // r1
+  }
+};
+
 }  // namespace
 
 // per-history std::mt19937 seed in the MT scheme (unique per history, unlike
@@ -268,6 +338,16 @@ void orc_philox(const unsigned* c, const unsigned* k, unsigned* out4) {
   const Philox4 r = philox4x32_10(c[0], c[1], c[2], c[3], k[0], k[1]);
   for (int i = 0; i < 4; ++i) out4[i] = r.v[i];
 }
+
+// n uniforms of SimconfType::drand() after seed(seed), skipping `skip` draws
+void orc_mt_stream(unsigned seed, long long skip, long long n, double* out) {
+  MtRng rng;
+  rng.seed(seed);
+  for (long long i = 0; i < skip; ++i) rng.drand();
+  for (long long i = 0; i < n; ++i) out[i] = rng.drand();
+}
+
+unsigned orc_mt_history_seed(unsigned master, long long h) { return mt_history_seed(master, h); }
 
 unsigned long long orc_flight_key_child(unsigned long long parent, unsigned ic) {
   return flight_key_child(parent, ic);
@@ -471,6 +551,42 @@ int orc_trace(const orc_run_spec* sp, long long h, double* rows, int max_rows) {
   }
   for (Ion* ion : q) delete ion;
   return hk.n < max_rows ? hk.n : max_rows;
+}
+
+// Algorithmic work per collision of the reference's loop on `spec`
+// (std::mt19937 draws): out = flop_eq, collisions, flights, uniforms, then
+// add mul div cmp sqrt exp log pow cbrt sin cos atan2 counts per collision.
+int orc_workcount(const orc_run_spec* sp, double* out) {
+  const orc_run_spec& s = *sp;
+  std::vector<std::unique_ptr<Material>> mats;
+  std::unique_ptr<Sample> smp = build_sample(s, mats);
+  WorkHooks hk(s);
+  MtRng rng;
+  MtDraws mtd(rng);
+  hk.draws = &mtd;
+  hk.sample = smp.get();
+  const Config cfg{s.tmin, s.tau, s.cw, s.scale, s.potential};
+  for (long long h = s.hist0; h < s.hist0 + s.n_hist; ++h) {
+    rng.seed(mt_history_seed(s.seed, h));
+    std::deque<Ion*> q;
+    q.push_back(make_primary(s, h));
+    while (!q.empty()) {
+      Ion* ion = q.front();
+      q.pop_front();
+      smp->averages(*ion, cfg.tmin);
+      trim(ion, q, *smp, cfg, mtd, hk);
+      delete ion;
+    }
+  }
+  const double n = double(hk.coll > 0 ? hk.coll : 1);
+  const OpCount& o = hk.ops;
+  out[0] = o.flop_eq() / n;
+  out[1] = double(hk.coll);
+  out[2] = double(hk.flights);
+  out[3] = double(hk.uniforms) / n;
+  const long long c[12] = {o.add, o.mul, o.div, o.cmp, o.sqrt_, o.exp_, o.log_, o.pow_, o.cbrt_, o.sin_, o.cos_, o.atan2_};
+  for (int i = 0; i < 12; ++i) out[4 + i] = double(c[i]) / n;
+  return 0;
 }
 
 }  // extern "C"

@@ -78,7 +78,11 @@ def lib():
         L.orc_flight_key_child.restype = C.c_ulonglong
         L.orc_flight_key_child.argtypes = [C.c_ulonglong, C.c_uint]
         L.orc_uo2.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int] + [C.c_char_p, C.c_int] * 3
+        L.orc_mt_stream.argtypes = [C.c_uint, C.c_longlong, C.c_longlong, dp]
+        L.orc_mt_history_seed.restype = C.c_uint
+        L.orc_mt_history_seed.argtypes = [C.c_uint, C.c_longlong]
         L.orc_run.argtypes = [C.POINTER(RunSpec), C.POINTER(RunOut)]
+        L.orc_workcount.argtypes = [C.POINTER(RunSpec), dp]
         L.orc_trace.argtypes = [C.POINTER(RunSpec), C.c_longlong, dp, C.c_int]
         _lib = L
     return _lib
@@ -165,6 +169,18 @@ def run(prob, rng: int = 1, nthreads: int = 1, per_history: bool = False) -> dic
     return res
 
 
+WORK_FIELDS = ["flop_eq", "collisions", "flights", "uniforms", "add", "mul", "div", "cmp", "sqrt",
+               "exp", "log", "pow", "cbrt", "sin", "cos", "atan2"]
+
+
+def workcount(prob) -> dict:
+    """FP64 flop-equivalents per collision of the reference loop on `prob`."""
+    s, keep = make_spec(prob, 0)
+    out = np.zeros(16)
+    lib().orc_workcount(C.byref(s), _p(out, dp))
+    return dict(zip(WORK_FIELDS, out.tolist()))
+
+
 def trace(prob, h: int, max_rows: int, rng: int = 0) -> np.ndarray:
     L = lib()
     s, keep = make_spec(prob, rng)
@@ -201,6 +217,18 @@ def philox(ctr, key):
     o = (C.c_uint * 4)()
     lib().orc_philox(c, k, o)
     return list(o)
+
+
+def mt_stream(seed: int, n: int, skip: int = 0) -> np.ndarray:
+    """SimconfType::drand() values after seed(seed) (std::mt19937 +
+    uniform_real_distribution<double>)."""
+    out = np.zeros(n)
+    lib().orc_mt_stream(seed, skip, n, _p(out, dp))
+    return out
+
+
+def mt_history_seed(master: int, h: int) -> int:
+    return lib().orc_mt_history_seed(master, h)
 
 
 def uo2(seed=39172, r=10.0, cbf=0.1, nev=1):

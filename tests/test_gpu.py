@@ -1,0 +1,252 @@
+"""GPU parity tests: the HIP library (through its C ABI) against the oracle,
+the compiled reference's fixtures and the reference's statistics.
+
+Tolerances (FP64 path):
+* stopping (getrstop): relative 1e-12 against the reference's own values.
+* scattering: |d cos(theta)| <= 1e-12 and |d s2| <= 1e-12 s2 + 8 ulp(1); s2 =
+  1 - co^2 is formed by a subtraction from 1, so glancing collisions carry an
+  absolute rounding floor whatever the libm.
+* cascades: the device draw scheme is restated by the oracle (rng=1), so
+  histories are compared one by one; device and host libm differ in the last
+  ulp, which can flip a rare threshold decision deep in a long cascade, so
+  >= 97 % of histories must agree exactly in every integer tally and totals
+  within 1 %.
+* against the reference (std::mt19937 streams): statistics within 2.5 sigma.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fixture_cases import ENERGIES, IONS, TARGETS, c4_uniform, trace_cases
+from mytrim_amd import configs, gpu
+from mytrim_amd.problem import (CUT, GEO_SOLID, PBC, Bins, Element, Geometry, Material,
+                                Primaries, Problem)
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = gpu.Engine(0)
+    yield e
+    e.close()
+
+
+def load_tally(name):
+    with open(os.path.join(GOLD, "tally_%s.json" % name)) as f:
+        return json.load(f)
+
+
+def target_problem(rho, els):
+    return Problem(materials=[Material(rho, [Element(Z, m, t) for Z, m, t in els])],
+                   geometry=Geometry(), primaries=Primaries.single(1, 1.0, 1.0, (0, 0, 0)),
+                   n_hist=0)
+
+
+# ---------------------------------------------------------------- per-function twins
+def test_stopping_matches_reference_grid(eng):
+    g = np.load(os.path.join(GOLD, "stopping_grid.npz"))
+    worst = 0.0
+    for j, (_, rho, els) in enumerate(TARGETS):
+        eng.set_target(target_problem(rho, els))
+        for i, (Z1, m1) in enumerate(IONS):
+            se = eng.stopping(Z1, m1, 0, ENERGIES)
+            ref = g["se"][i, j]
+            worst = max(worst, float(np.max(np.abs(se - ref) / np.abs(ref))))
+    print("stopping worst relative error %.3g" % worst)
+    assert worst <= 1e-12
+
+
+@pytest.mark.parametrize("potential", [0, 1, 2])
+def test_scatter_matches_oracle(eng, potential):
+    eps = np.repeat(np.logspace(-5, 3, 81), 41)
+    b = np.tile(np.concatenate([[1e-6, 1e-3], np.linspace(0.01, 20.0, 39)]), 81)
+    s2, ct, st, it = eng.scatter(eps, b, potential)
+    ref = np.array([O.scatter(e, bb, potential) for e, bb in zip(eps, b)])
+    d_ct = np.abs(ct - ref[:, 1])
+    d_s2 = np.abs(s2 - ref[:, 0])
+    print("max |d ct| %.3g, max |d s2| - tol %.3g" % (d_ct.max(), (d_s2 - 1e-12 * ref[:, 0]).max()))
+    assert np.all(d_ct <= 1e-12)
+    assert np.all(d_s2 <= 1e-12 * ref[:, 0] + 8 * 2.0 ** -52)
+    assert np.all(np.abs(st - ref[:, 2]) <= 1e-12 + 1e-12 * ref[:, 2])
+    assert np.mean(it == ref[:, 3]) >= 0.999
+
+
+# ---------------------------------------------------------------- cascades vs oracle
+CASES = {
+    "c1": lambda: configs.c1_he_fe(400, seed=5),
+    "c2": lambda: configs.c2_cu_cu(6, seed=5),
+    "c3": lambda: configs.c3_ag_layers(8, seed=5),
+    "c4": lambda: c4_uniform(2000.0, 12).with_(seed=5),
+    "c4mix": lambda: configs.c4_mixed_uo2(64, seed=5, w=2000.0),
+    "c5": lambda: configs.c5_c_w(24, seed=5),
+    "kp": lambda: configs.c2_cu_cu(200, seed=5).with_(follow_max_gen=1, kp=True),
+    "moliere": lambda: configs.c1_he_fe(100, seed=5).with_(potential=1),
+    "ckr": lambda: configs.c5_c_w(10, seed=5).with_(potential=2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_cascade_histories_match_oracle(eng, case):
+    prob = CASES[case]()
+    g = eng.run(prob, per_history=True)
+    o = O.run(prob, rng=1, nthreads=8, per_history=True)
+    keys = ["h_vac", "h_coll", "h_flights", "h_state"]
+    same = np.ones(prob.n_hist, bool)
+    for k in keys:
+        same &= g[k] == o[k]
+    pos_ok = np.isclose(g["h_x"], o["h_x"], rtol=1e-9, atol=1e-6)
+    frac = float(np.mean(same & pos_ok))
+    print("%s: %d histories, %d collisions, %.1f ms, identical fraction %.4f"
+          % (case, prob.n_hist, g["n_coll"], g["kernel_ms"], frac))
+    assert frac >= 0.97
+    for k in ["n_coll", "n_flight", "vac", "disp", "range_n"]:
+        assert g[k] == pytest.approx(o[k], rel=0.01), k
+    for k in ["Eel", "Enuc"]:
+        assert g[k] == pytest.approx(o[k], rel=0.01), k
+    tot = g["Eel"] + g["Enuc"] + g["Eunf"] + g["Elost"] + g["Edrop"] + g["Evac"]
+    assert tot == pytest.approx(g["E0"], rel=1e-11)
+
+
+def test_cascade_bitwise_totals_when_histories_agree(eng):
+    """Where every history agrees, the tallies must agree exactly."""
+    prob = configs.c1_he_fe(300, seed=9)
+    g = eng.run(prob, per_history=True)
+    o = O.run(prob, rng=1, nthreads=8, per_history=True)
+    if not np.array_equal(g["h_coll"], o["h_coll"]) or not np.array_equal(g["h_vac"], o["h_vac"]):
+        pytest.skip("a history diverged at the last ulp; covered by the fraction test")
+    for k in ["n_coll", "n_flight", "vac", "repl", "subst", "inter", "lost", "disp", "range_n"]:
+        assert g[k] == o[k], k
+    for k in ["range_hist", "lat_y_hist", "vac_hist", "repl_hist"]:
+        assert np.array_equal(g[k], o[k]), k
+    assert g["Eel"] == pytest.approx(o["Eel"], rel=1e-12)
+
+
+def test_history_offsets_select_the_same_streams(eng):
+    """History h's result does not depend on the batch it is run in."""
+    full = configs.c1_he_fe(200, seed=21)
+    a = eng.run(full, per_history=True)
+    b = eng.run(full.with_(hist0=100, n_hist=100), per_history=True)
+    assert np.array_equal(a["h_vac"][100:], b["h_vac"])
+    assert np.array_equal(a["h_coll"][100:], b["h_coll"])
+
+
+# ---------------------------------------------------------------- vs the reference
+@pytest.mark.parametrize("name,prob,n", [
+    ("c1", lambda n: configs.c1_he_fe(n, seed=31), 20000),
+    ("vac_cu_10keV", lambda n: configs.vacancy_case("cu", 1e4, n, seed=31), 20000),
+    ("c5", lambda n: configs.c5_c_w(n, seed=31), 2000),
+    ("c2", lambda n: configs.c2_cu_cu(n, seed=31), 800),
+])
+def test_cascade_statistics_vs_reference(eng, name, prob, n):
+    ref = load_tally(name)
+    g = eng.run(prob(n), per_history=True)
+    rv = np.array(ref["h_vac"], float)
+    sig = math.sqrt(np.var(rv) / len(rv) + np.var(g["h_vac"]) / n)
+    print("%s vac/ion gpu %.3f ref %.3f (sigma %.3f), %.1f ms" %
+          (name, g["vac"] / n, rv.mean(), sig, g["kernel_ms"]))
+    assert abs(g["vac"] / n - rv.mean()) < 2.5 * sig
+    m_ref = ref["range_sum"] / ref["range_n"]
+    v_ref = ref["range_sum2"] / ref["range_n"] - m_ref ** 2
+    m = g["range_sum"] / g["range_n"]
+    v = g["range_sum2"] / g["range_n"] - m ** 2
+    assert abs(m - m_ref) < 2.5 * math.sqrt(v_ref / ref["range_n"] + v / g["range_n"])
+    # collisions per history (electronic loss ledger share) within 2.5 sigma of the mean
+    cpi = ref["n_coll"] / ref["n_hist"]
+    assert g["n_coll"] / n == pytest.approx(cpi, rel=0.05)
+
+
+def test_vacancy_table_on_gpu(eng):
+    rows = np.loadtxt(os.path.join(GOLD, "validation", "vacancy_count_comparison.dat"),
+                      skiprows=2, delimiter=",")
+    for (ion, e, val) in [("cu", 1e5, rows[4, 10]), ("si", 1e5, rows[4, 2]), ("xe", 1e5, rows[4, 6])]:
+        n = 2000
+        g = eng.run(configs.vacancy_case(ion, e, n, seed=3), per_history=True)
+        mean = g["vac"] / n
+        sem = np.std(g["h_vac"]) / math.sqrt(n)
+        assert abs(mean - val) < 3 * sem + 0.015 * val, (ion, e, mean, val)
+
+
+# ---------------------------------------------------------------- drop-in flight path
+def replay_history(eng, prob, h, max_rows):
+    """TrimBase::trim one flight at a time on the GPU, FIFO drain, with the
+    reference's std::mt19937 stream; returns trace rows like orc_trace."""
+    seed = O.mt_history_seed(prob.seed, h)
+    stream = O.mt_stream(seed, 400000)
+    pos = 0
+    p = prob.primaries
+    q = [dict(Z=int(p.Z[0]), m=float(p.m[0]), E=float(p.E[0]), pos=list(p.pos[0]),
+              dir=list(p.dir[0]), gen=0, tag=-1)]
+    rows = []
+    flight = 0
+    els = [e for mat in prob.materials for e in mat.elements]
+    offs = np.cumsum([0] + [len(m.elements) for m in prob.materials])
+    while q and len(rows) < max_rows:
+        d = q.pop(0)
+        ion = gpu.Ion(d["Z"], d["m"], d["E"], tuple(d["pos"]), tuple(d["dir"]), d["gen"], d["tag"],
+                      0, prob.Ef)
+        used, evs = eng.trim_flight(ion, stream[pos:pos + 200000])
+        pos += used
+        for ic, e in enumerate(evs, 1):
+            el = els[offs[e.material] + e.element]
+            rows.append([flight, ic, d["Z"], e.pka_E, *e.pka_pos, *e.pka_dir, e.pka_state, e.ls,
+                         e.dee, e.den, el.Z, e.rec_E, *e.rec_dir, d["gen"], 0.0])
+            if e.fate == 1:
+                rd = np.array(e.rec_dir)
+                inv = 1.0 / math.sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2])
+                q.append(dict(Z=el.Z, m=el.m, E=e.rec_E, pos=list(e.rec_pos),
+                              dir=list(rd * inv), gen=d["gen"] + 1, tag=e.mat_tag))
+        flight += 1
+    return np.array(rows[:max_rows])
+
+
+@pytest.mark.parametrize("case", ["c1", "c2", "c3", "c4", "c5", "moliere", "ckr"])
+def test_flight_path_reproduces_reference_trace(eng, case):
+    ref = np.load(os.path.join(GOLD, "trace_%s_h0.npz" % case))["rows"][:600]
+    prob = trace_cases()[case]
+    eng.set_target(prob)
+    mine = replay_history(eng, prob, 0, len(ref))
+    assert mine.shape == ref.shape
+    # integer columns exact: flight, ic, Z, state, element Z, gen
+    for c in (0, 1, 2, 10, 14, 19):
+        assert np.array_equal(mine[:, c], ref[:, c]), c
+    scale = np.maximum(np.abs(ref), 1.0)
+    err = np.max(np.abs(mine - ref) / scale, axis=0)
+    print("%s: max scaled deviation per column %s" % (case, np.array2string(err, precision=1)))
+    assert np.all(err <= 1e-9)
+
+
+# ---------------------------------------------------------------- edge cases
+def test_zero_histories(eng):
+    g = eng.run(configs.c1_he_fe(0))
+    assert g["n_coll"] == 0 and g["n_flight"] == 0
+
+
+def test_ions_leaving_through_the_cut_face(eng):
+    prob = configs.c1_he_fe(64)
+    prob = prob.with_(primaries=Primaries.single(2, 4.0026, 1e5, (0.0, 5000.0, 5000.0), (-1, 0, 0)))
+    g = eng.run(prob, per_history=True)
+    o = O.run(prob, rng=1, per_history=True)
+    assert g["lost"] == o["lost"] == 64
+    assert np.array_equal(g["h_coll"], o["h_coll"])
+    assert g["range_n"] == 0
+
+
+def test_sub_threshold_primaries(eng):
+    prob = configs.c2_cu_cu(128).with_(
+        primaries=Primaries.single(29, 63.546, 2.0, (100.0, 1e4, 1e4)))
+    g = eng.run(prob, per_history=True)
+    o = O.run(prob, rng=1, per_history=True)
+    assert np.array_equal(g["h_coll"], o["h_coll"])
+    assert g["inter"] == o["inter"]
+
+
+def test_invalid_input_raises(eng):
+    bad = configs.c1_he_fe(4).with_(materials=[Material(7.87, [Element(120, 55.0)])])
+    with pytest.raises(gpu.MtError):
+        eng.set_target(bad)
