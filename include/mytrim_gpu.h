@@ -7,7 +7,7 @@
  *
  * What each entry point replaces in the reference (MyTRIM):
  *   mt_ctx_create / mt_ctx_set_tables   SimconfType construction and
- *                                       readDataFiles (simconf.C:36-137)
+ *   mt_default_tables                   readDataFiles (simconf.C:36-137)
  *   mt_ctx_set_target                   MaterialBase + Element set-up and
  *                                       prepare() (material.C:37-74), the
  *                                       Sample* geometry (sample*.h) and the
@@ -169,10 +169,12 @@ const char* mt_version(void);
 
 int mt_ctx_create(int device, mt_ctx** out);
 int mt_ctx_destroy(mt_ctx* ctx);
-/* 92 rows x 17 doubles: mm1 m1 mnat rho atrho vfermi heat pcoef[8] lfctr
+/* 92 rows x 16 doubles: mm1 m1 mnat rho atrho vfermi heat pcoef[8] lfctr
  * (SCOEF.95A + SLFCTR.dat columns).  Defaults to the tables built into the
  * library. */
 int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols);
+/* copy the tables built into the library (same 92 x 16 layout) */
+int mt_default_tables(double* rows, int n_rows, int n_cols);
 int mt_ctx_set_target(mt_ctx* ctx, const mt_target* target);
 int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* tuning);
 

@@ -167,9 +167,9 @@ int mt_ctx_destroy(mt_ctx* ctx) {
 
 int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols) {
   if (!ctx || !rows) return fail(MT_E_ARG, "NULL argument");
-  if (n_rows != 92 || n_cols != 17) return fail(MT_E_ARG, "tables must be 92 x 17");
+  if (n_rows != 92 || n_cols != 16) return fail(MT_E_ARG, "tables must be 92 x 16");
   for (int i = 0; i < 92; ++i) {
-    const double* v = rows + 17 * i;
+    const double* v = rows + 16 * i;
     CoefRow& c = ctx->coef[i];
     c.mm1 = v[0];
     c.m1 = v[1];
@@ -180,6 +180,25 @@ int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols) {
     c.heat = v[6];
     for (int j = 0; j < 8; ++j) c.pcoef[j] = v[7 + j];
     c.lfctr = v[15];
+  }
+  return MT_OK;
+}
+
+int mt_default_tables(double* rows, int n_rows, int n_cols) {
+  if (!rows) return fail(MT_E_ARG, "NULL argument");
+  if (n_rows != 92 || n_cols != 16) return fail(MT_E_ARG, "tables are 92 x 16");
+  const std::vector<CoefRow>& c = default_coefs();
+  for (int i = 0; i < 92; ++i) {
+    double* v = rows + 16 * i;
+    v[0] = c[i].mm1;
+    v[1] = c[i].m1;
+    v[2] = c[i].mnat;
+    v[3] = c[i].rho;
+    v[4] = c[i].atrho;
+    v[5] = c[i].vfermi;
+    v[6] = c[i].heat;
+    for (int j = 0; j < 8; ++j) v[7 + j] = c[i].pcoef[j];
+    v[15] = c[i].lfctr;
   }
   return MT_OK;
 }

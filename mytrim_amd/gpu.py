@@ -25,6 +25,7 @@ MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
 
 # every symbol include/mytrim_gpu.h declares
 EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt_ctx_set_tables",
+           "mt_default_tables",
            "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_trim_flight",
            "mt_stopping_batch", "mt_scatter_batch"]
 

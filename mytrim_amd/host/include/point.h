@@ -1,0 +1,64 @@
+// Point and Real of the MyTRIM API (shim/point.h, simconf.h:39): a 3-vector
+// with component access and component-wise arithmetic.
+#ifndef MYTRIM_AMD_POINT_H
+#define MYTRIM_AMD_POINT_H
+
+#include <cmath>
+
+typedef double Real;
+
+class Point {
+ public:
+  Point() : data{0.0, 0.0, 0.0} {}
+  Point(Real x, Real y, Real z) : data{x, y, z} {}
+  Real& operator()(unsigned int i) { return data[i]; }
+  const Real& operator()(unsigned int i) const { return data[i]; }
+  Real norm_sq() const { return data[0] * data[0] + data[1] * data[1] + data[2] * data[2]; }
+  Real norm() const { return std::sqrt(norm_sq()); }
+  Point operator+(const Point& r) const { return Point(data[0] + r.data[0], data[1] + r.data[1], data[2] + r.data[2]); }
+  Point operator-(const Point& r) const { return Point(data[0] - r.data[0], data[1] - r.data[1], data[2] - r.data[2]); }
+  Point operator*(Real r) const { return Point(data[0] * r, data[1] * r, data[2] * r); }
+  Point operator/(Real r) const { return Point(data[0] / r, data[1] / r, data[2] / r); }
+  Point operator-() const { return Point(-data[0], -data[1], -data[2]); }
+  Point& operator+=(const Point& r) {
+    for (int i = 0; i < 3; ++i) data[i] += r.data[i];
+    return *this;
+  }
+  Point& operator-=(const Point& r) {
+    for (int i = 0; i < 3; ++i) data[i] -= r.data[i];
+    return *this;
+  }
+  Point& operator*=(Real r) {
+    for (int i = 0; i < 3; ++i) data[i] *= r;
+    return *this;
+  }
+  Point& operator/=(Real r) {
+    for (int i = 0; i < 3; ++i) data[i] /= r;
+    return *this;
+  }
+
+ private:
+  Real data[3];
+};
+
+namespace Utility {
+// integer powers by repeated squaring (shim/pow.h)
+template <int N>
+inline Real pow(const Real& x) {
+  if constexpr (N == 0) {
+    return 1.0;
+  } else if constexpr (N == 1) {
+    return x;
+  } else if constexpr (N == 6) {
+    const Real x2 = x * x, x4 = x2 * x2;
+    return x4 * x2;
+  } else if constexpr (N % 2) {
+    return x * pow<N - 1>(x);
+  } else {
+    const Real h = pow<N / 2>(x);
+    return h * h;
+  }
+}
+}  // namespace Utility
+
+#endif

@@ -1,0 +1,59 @@
+// TrimGPU: the batched path.  Runs the full cascades of many primaries in one
+// persistent-kernel launch (mt_run_cascades) with the tally hooks built in:
+//   follow every recoil (TrimBase) or recoils with gen < maxGen
+//   (TrimPrimaries / TrimRecoils, optionally with modified Kinchin-Pease),
+//   vacancy and replacement depth bins (TrimVacCount), the phonon energy
+//   ledger (TrimPhononOut), range and lateral histograms of the primaries.
+// Each history draws from its own counter-based stream (seed, history), so
+// results do not depend on scheduling; they are statistically equivalent to
+// the host-stream path of TrimBase::trim.
+#ifndef MYTRIM_AMD_TRIM_GPU_H
+#define MYTRIM_AMD_TRIM_GPU_H
+
+#include <cstdint>
+#include <vector>
+
+#include "trim.h"
+
+namespace MyTRIM_NS {
+
+class TrimGPU : public TrimBase {
+ public:
+  TrimGPU(SimconfType* simconf, SampleBase* sample) : TrimBase(simconf, sample) {}
+
+  struct Options {
+    int follow_max_gen = -1;  // < 0: every recoil
+    bool kinchin_pease = false;
+    Real range_x0 = 0.0, range_bin = 10.0;
+    int range_bins = 1000;
+    Real lat_y0 = 0.0, lat_z0 = 0.0, lat_bin = 10.0;
+    int lat_bins = 400;
+    Real vac_x0 = 0.0, vac_bin = 1.0;
+    int vac_bins = 10000;
+    bool per_history = false;
+  };
+
+  struct Tallies {
+    long long collisions = 0, flights = 0, vacancies = 0, replacements = 0, substitutions = 0,
+              interstitials = 0, lost = 0, vacuum = 0, displacements = 0;
+    Real E0 = 0, Eel = 0, Enuc = 0, Eunfollowed = 0, Elost = 0, Edropped = 0, Evacuum = 0;
+    long long range_n = 0;
+    Real range_sum = 0, range_sum2 = 0;
+    std::vector<long long> range_hist, lat_y_hist, lat_z_hist, vac_hist, repl_hist;
+    std::vector<int> h_vac, h_flights, h_state;
+    std::vector<long long> h_coll;
+    std::vector<Real> h_x, h_y, h_z, h_Eel;
+    double kernel_ms = 0;
+  };
+
+  // n copies of `pka` (histories hist0 .. hist0+n-1 of stream `seed`)
+  Tallies runCascades(const IonBase& pka, long long n, uint32_t seed, const Options& opt,
+                      long long hist0 = 0);
+  // one primary per history
+  Tallies runCascades(const std::vector<IonBase*>& pkas, uint32_t seed, const Options& opt,
+                      long long hist0 = 0);
+};
+
+}  // namespace MyTRIM_NS
+
+#endif
