@@ -16,8 +16,9 @@
 #include "mytrim_gpu.h"
 
 namespace mt {
-hipError_t cascade_occupancy(int potential, int geo, bool perhist, int block, size_t lds, int* blocks_per_cu);
-hipError_t cascade_launch(const Params& P, int potential, bool perhist, int grid, int block, size_t lds,
+hipError_t cascade_occupancy(int arith, int potential, int geo, bool perhist, int block, size_t lds,
+                             int* blocks_per_cu);
+hipError_t cascade_launch(const Params& P, int arith, int potential, bool perhist, int grid, int block, size_t lds,
                           hipStream_t stream);
 int cascade_stack_bytes(int cap);
 hipError_t stopping_launch(const unsigned char* tables, int species, int mat, const double* E, long long n,
@@ -333,7 +334,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
   if (pk->n_hist < 0 || pk->n_hist > (1ll << 31)) return fail(MT_E_LIMIT, "n_hist out of range");
   if (pk->hist0 < 0 || pk->hist0 + pk->n_hist > (1ll << 32)) return fail(MT_E_LIMIT, "history index out of range");
-  if (o->arith != MT_FP64) return fail(MT_E_ARG, "arithmetic mode not available in this build");
+  if (o->arith != MT_FP64 && o->arith != MT_FP32) return fail(MT_E_ARG, "arithmetic mode not available in this build");
   if (o->range_nb < 1 || o->lat_nb < 1 || o->vac_nb < 1) return fail(MT_E_ARG, "histograms need >= 1 bin");
   if (!(o->range_bw > 0.0) || !(o->lat_bw > 0.0) || !(o->vac_bw > 0.0)) return fail(MT_E_ARG, "bin widths must be positive");
   HIPCHECK(hipSetDevice(ctx->device));
@@ -379,7 +380,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   const int pot = ctx->tgt.potential;
   const bool perhist = o->per_history != 0;
   int bpc = 0;
-  HIPCHECK(cascade_occupancy(pot, ctx->tgt.geo.kind, perhist, block, lds, &bpc));
+  HIPCHECK(cascade_occupancy(o->arith, pot, ctx->tgt.geo.kind, perhist, block, lds, &bpc));
   if (ctx->tune.blocks_per_cu > 0) bpc = std::min(bpc, ctx->tune.blocks_per_cu);
   if (bpc < 1) return fail(MT_E_LIMIT, "cascade kernel does not fit on a CU");
   const int grid = bpc * ctx->cus;
@@ -492,7 +493,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   }
 
   HIPCHECK(hipEventRecord(ctx->ev0, st));
-  HIPCHECK(cascade_launch(P, pot, perhist, grid, block, lds, st));
+  HIPCHECK(cascade_launch(P, o->arith, pot, perhist, grid, block, lds, st));
   HIPCHECK(hipEventRecord(ctx->ev1, st));
   HIPCHECK(hipStreamSynchronize(st));
   float ms = 0.0f;

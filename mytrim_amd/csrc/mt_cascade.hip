@@ -99,10 +99,68 @@ struct StackView {  // one wave's LDS stack, structure of arrays
 
 __host__ __device__ constexpr int stack_bytes(int cap) { return ((kStackEntryBytes - 4) * cap + 15) & ~15; }
 
+// Free path, impact parameter, target atom, electronic loss and scattering
+// of one collision (trim.C:88-272) in precision R.  FP64 follows the
+// reference's expressions; FP32 evaluates the same quantities in single
+// precision (scatter_f32 for the angle) and hands back doubles.
+template <typename R, int POT>
+__device__ __forceinline__ void physics(const TableView& T, int sp, int k, double E, int ic, uint32_t klo,
+                                        uint32_t khi, uint32_t seed, uint32_t hist, const P4& blk, double cw,
+                                        int& nn, double& ls, double& dee, double& s2, double& ct, double& st,
+                                        int& it) {
+  const MatConst& mc = T.mat[sp * T.n_mat + k];
+  const int eb = sp * T.n_el + mc.el_begin;
+  double hh = u32d(blk.z);
+  for (nn = 0; nn < mc.n_el; ++nn) {
+    hh -= T.el[eb + nn].t;
+    if (hh <= 0.0) break;
+  }
+  if (nn >= mc.n_el) nn = mc.n_el - 1;
+  const ElemConst& ec = T.el[eb + nn];
+  if constexpr (sizeof(R) == 8) {
+    // free path and maximum impact parameter (trim.C:88-95)
+    double eps = E * mc.f;
+    const double eeg = sqrt(eps * mc.epsdg);
+    const double pmax = mc.a / (eeg + sqrt(eeg) + 0.125 * pow(eeg, 0.1));
+    ls = 1.0 / (M_PI * (pmax * pmax) * mc.arho);
+    if (ic == 1) {
+      const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
+      ls = u53(b0.x, b0.y) * mn(ls, cw);
+    }
+    // impact parameter and target atom (trim.C:143-160)
+    const double p = pmax * sqrt(u53(blk.x, blk.y));
+    eps = ec.fi * E;
+    const double b = p / ec.ai;
+    // electronic loss along ls (trim.C:166-170) and scattering (trim.C:172-272)
+    dee = ls * getrstop<double>(T, sp, k, E);
+    scatter<double, POT>(eps, b, s2, ct, st, it);
+  } else {
+    const float Ef = (float)E;
+    float eps = Ef * (float)mc.f;
+    const float eeg = m_sqrt(eps * (float)mc.epsdg);
+    const float pmax = (float)mc.a / (eeg + m_sqrt(eeg) + 0.125f * m_pow(eeg, 0.1f));
+    float lsf = 1.0f / (3.14159265358979f * (pmax * pmax) * (float)mc.arho);
+    if (ic == 1) {
+      const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
+      lsf = u24f(b0.y) * mn(lsf, (float)cw);
+    }
+    const float p = pmax * m_sqrt(u24f(blk.x));
+    eps = (float)ec.fi * Ef;
+    const float b = p / (float)ec.ai;
+    ls = (double)lsf;
+    dee = ls * (double)getrstop<float>(T, sp, k, Ef);
+    float s2f, ctf, stf;
+    scatter_f32<POT>(eps, b, s2f, ctf, stf, it);
+    s2 = s2f;
+    ct = ctf;
+    st = stf;
+  }
+}
+
 }  // namespace
 
-template <int POT, int GEO, bool PERHIST>
-__global__ void __launch_bounds__(256) cascade_fp64(Params P) {
+template <typename R, int POT, int GEO, bool PERHIST>
+__global__ void __launch_bounds__(256) cascade_kernel(Params P) {
   extern __shared__ __align__(16) unsigned char lds[];
   {
     const uint4* src = (const uint4*)P.tables;
@@ -335,34 +393,10 @@ __global__ void __launch_bounds__(256) cascade_fp64(Params P) {
         const uint32_t klo = (uint32_t)F.key, khi = (uint32_t)(F.key >> 32);
         const P4 blk = philox10((uint32_t)F.ic, klo, khi, 0u, P.seed, F.hist);
         const MatConst& mc = T.mat[F.sp * T.n_mat + k];
-        // free path and maximum impact parameter (trim.C:88-95)
-        double eps = F.E * mc.f;
-        const double eeg = sqrt(eps * mc.epsdg);
-        const double pmax = mc.a / (eeg + sqrt(eeg) + 0.125 * pow(eeg, 0.1));
-        double ls = 1.0 / (M_PI * (pmax * pmax) * mc.arho);
-        if (F.ic == 1) {
-          const P4 b0 = philox10(0u, klo, khi, 0u, P.seed, F.hist);
-          ls = u53(b0.x, b0.y) * mn(ls, P.cw);
-        }
-        // impact parameter and target atom (trim.C:143-160)
-        const double p = pmax * sqrt(u53(blk.x, blk.y));
-        double hh = u32d(blk.z);
-        const int eb = F.sp * T.n_el + mc.el_begin;
-        int nn = 0;
-        for (; nn < mc.n_el; ++nn) {
-          hh -= T.el[eb + nn].t;
-          if (hh <= 0.0) break;
-        }
-        if (nn >= mc.n_el) nn = mc.n_el - 1;
-        const ElemConst& ec = T.el[eb + nn];
-        eps = ec.fi * F.E;
-        const double b = p / ec.ai;
-        // electronic loss along ls (trim.C:166-170)
-        double dee = ls * getrstop<double>(T, F.sp, k, F.E);
-        // scattering (trim.C:172-272)
-        double s2, ct, st;
-        int it;
-        scatter<double, POT>(eps, b, s2, ct, st, it);
+        double ls, dee, s2, ct, st;
+        int it, nn;
+        physics<R, POT>(T, F.sp, k, F.E, F.ic, klo, khi, P.seed, F.hist, blk, P.cw, nn, ls, dee, s2, ct, st, it);
+        const ElemConst& ec = T.el[F.sp * T.n_el + mc.el_begin + nn];
         ruth = (it == 0);
         l_newton += it;
         if (it >= kMaxNewton) err |= kErrNewton;
@@ -655,34 +689,40 @@ namespace mt {
 
 typedef void (*CascadeKernel)(Params);
 
-template <int POT, int GEO>
-static CascadeKernel pick2(bool perhist) {
-  return perhist ? cascade_fp64<POT, GEO, true> : cascade_fp64<POT, GEO, false>;
+template <typename R, int POT, int GEO>
+static CascadeKernel pick3(bool perhist) {
+  return perhist ? cascade_kernel<R, POT, GEO, true> : cascade_kernel<R, POT, GEO, false>;
 }
 
-template <int POT>
-static CascadeKernel pick1(int geo, bool perhist) {
-  if (geo == 1) return pick2<POT, 1>(perhist);
-  if (geo == 2) return pick2<POT, 2>(perhist);
-  return pick2<POT, 0>(perhist);
+template <typename R, int POT>
+static CascadeKernel pick2(int geo, bool perhist) {
+  if (geo == 1) return pick3<R, POT, 1>(perhist);
+  if (geo == 2) return pick3<R, POT, 2>(perhist);
+  return pick3<R, POT, 0>(perhist);
 }
 
-static CascadeKernel pick(int potential, int geo, bool perhist) {
-  if (potential == 1) return pick1<1>(geo, perhist);
-  if (potential == 2) return pick1<2>(geo, perhist);
-  return pick1<0>(geo, perhist);
+template <typename R>
+static CascadeKernel pick1(int potential, int geo, bool perhist) {
+  if (potential == 1) return pick2<R, 1>(geo, perhist);
+  if (potential == 2) return pick2<R, 2>(geo, perhist);
+  return pick2<R, 0>(geo, perhist);
 }
 
-hipError_t cascade_occupancy(int potential, int geo, bool perhist, int block, size_t lds, int* blocks_per_cu) {
-  CascadeKernel k = pick(potential, geo, perhist);
+static CascadeKernel pick(int arith, int potential, int geo, bool perhist) {
+  return arith == 1 ? pick1<float>(potential, geo, perhist) : pick1<double>(potential, geo, perhist);
+}
+
+hipError_t cascade_occupancy(int arith, int potential, int geo, bool perhist, int block, size_t lds,
+                             int* blocks_per_cu) {
+  CascadeKernel k = pick(arith, potential, geo, perhist);
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)k, block, lds);
 }
 
-hipError_t cascade_launch(const Params& P, int potential, bool perhist, int grid, int block, size_t lds,
+hipError_t cascade_launch(const Params& P, int arith, int potential, bool perhist, int grid, int block, size_t lds,
                           hipStream_t stream) {
-  CascadeKernel k = pick(potential, P.geo.kind, perhist);
+  CascadeKernel k = pick(arith, potential, P.geo.kind, perhist);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, stream, P);
   return hipGetLastError();
 }

@@ -188,6 +188,90 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
   iters = it;
 }
 
+// Single-precision scattering for the FP32 path.  Same Newton solve and
+// MAGIC constants as scatter(), but the small-angle end is evaluated without
+// cancellation: at the apsis b^2/r + v r/eps - r = 0, so
+//   r - b = v r^2 / (eps (r + b)),
+//   1 - co = (r - b)(1 - A) / (r + roc),   A = aa ff / (ff + 1),
+//   s2 = (1 - co)(2 - (1 - co)),  ct = 1 - 2 s2,  st = 2 sqrt(s2 (1 - s2)),
+// which is algebraically the reference's co = (b + delta + roc)/(r + roc),
+// s2 = 1 - co^2 but keeps full relative precision when s2 << 1 (glancing
+// collisions transfer s2 ~ tmin / (ec E), below single-precision epsilon).
+template <int POT>
+__device__ __forceinline__ void scatter_f32(float eps, float b, float& s2, float& ct, float& st, int& iters) {
+  typedef float R;
+  if (eps > K(10.0)) {
+    s2 = K(1.0) / (K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
+    ct = K(1.0) - K(2.0) * s2;
+    st = K(2.0) * m_sqrt(s2 * (K(1.0) - s2));
+    iters = 0;
+    return;
+  }
+  R r = b;
+  R rr = K(-2.7) * m_log(eps * b);
+  if (rr >= b) {
+    rr = K(-2.7) * m_log(eps * rr);
+    if (rr >= b) r = rr;
+  }
+  R v, v1, q;
+  int it = 0;
+  auto pot = [](R x, R& vv, R& vv1) {
+    if (POT == 1) {
+      const R ex1 = m_exp(K(-0.3) * x);
+      const R ex2 = sq(ex1) * sq(ex1);
+      const R ex3 = ex2 * (sq(ex2) * sq(ex2));
+      vv = (K(0.35) * ex1 + K(0.55) * ex2 + K(0.1) * ex3) / x;
+      vv1 = -(vv + K(0.105) * ex1 + K(0.66) * ex2 + K(0.6) * ex3) / x;
+    } else if (POT == 2) {
+      const R ex1 = m_exp(K(-0.279) * x);
+      const R ex2 = m_exp(K(-0.637) * x);
+      const R ex3 = m_exp(K(-1.1919) * x);
+      vv = (K(0.191) * ex1 + K(0.474) * ex2 + K(0.335) * ex3) / x;
+      vv1 = -(vv + K(0.531865) * ex1 + K(0.30181) * ex2 + K(0.6437) * ex3) / x;
+    } else {
+      const R ex1 = K(0.18175) * m_exp(K(-3.1998) * x);
+      const R ex2 = K(0.50986) * m_exp(K(-0.94229) * x);
+      const R ex3 = K(0.28022) * m_exp(K(-0.4029) * x);
+      const R ex4 = K(0.028171) * m_exp(K(-0.20162) * x);
+      vv = (ex1 + ex2 + ex3 + ex4) / x;
+      vv1 = -(vv + K(3.1998) * ex1 + K(0.94229) * ex2 + K(0.4029) * ex3 + K(0.20162) * ex4) / x;
+    }
+  };
+  do {
+    pot(r, v, v1);
+    const R fr = b * b / r + v * r / eps - r;
+    const R fr1 = -b * b / (r * r) + (v + v1 * r) / eps - K(1.0);
+    q = fr / fr1;
+    r -= q;
+    ++it;
+  } while (m_abs(q / r) > K(0.001) && it < kMaxNewton);
+  const R roc = K(-2.0) * (eps - v) / v1;
+  const R sqe = m_sqrt(eps);
+  R cc, aa, ff;
+  if (POT == 1) {
+    cc = (K(0.009611) + sqe) / (K(0.005175) + sqe);
+    aa = K(2.0) * eps * (K(1.0) + (K(0.6743) / sqe)) * m_pow(b, cc);
+    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6.314) + eps) / (K(10.0) + eps));
+  } else if (POT == 2) {
+    cc = (K(0.235809) + sqe) / (K(0.126000) + sqe);
+    aa = K(2.0) * eps * (K(1.0) + (K(1.0144) / sqe)) * m_pow(b, cc);
+    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6935.0) + eps) / (K(83550.0) + eps));
+  } else {
+    cc = (K(0.011615) + sqe) / (K(0.0071222) + sqe);
+    aa = K(2.0) * eps * (K(1.0) + (K(0.99229) / sqe)) * m_pow(b, cc);
+    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(9.3066) + eps) / (K(14.813) + eps));
+  }
+  R vr, v1r;
+  pot(r, vr, v1r);
+  const R rmb = vr * r * r / (eps * (r + b));
+  const R A = aa * ff / (ff + K(1.0));
+  const R omc = rmb * (K(1.0) - A) / (r + roc);
+  s2 = omc * (K(2.0) - omc);
+  ct = K(1.0) - K(2.0) * s2;
+  st = K(2.0) * m_sqrt(mx(K(0.0), s2 * (K(1.0) - s2)));
+  iters = it;
+}
+
 #undef K
 
 }  // namespace mt

@@ -250,3 +250,31 @@ def test_invalid_input_raises(eng):
     bad = configs.c1_he_fe(4).with_(materials=[Material(7.87, [Element(120, 55.0)])])
     with pytest.raises(gpu.MtError):
         eng.set_target(bad)
+
+
+# ---------------------------------------------------------------- FP32 physics path
+@pytest.mark.parametrize("name,prob", [
+    ("c1", lambda: configs.c1_he_fe(20000, seed=41)),
+    ("c3", lambda: configs.c3_ag_layers(300, seed=41)),
+    ("c5", lambda: configs.c5_c_w(1000, seed=41)),
+])
+def test_fp32_tallies_within_one_percent_of_fp64(eng, name, prob):
+    """FP32 stopping and scattering, FP64 energies, positions and ledger:
+    vacancies/ion, mean range and the electronic share of the energy within
+    1 % and within 2 sigma of the FP64 path on the same histories."""
+    p = prob()
+    a = eng.run(p, per_history=True, arith=gpu.MT_FP64)
+    b = eng.run(p, per_history=True, arith=gpu.MT_FP32)
+    n = p.n_hist
+    va, vb = a["vac"] / n, b["vac"] / n
+    sig = math.sqrt((np.var(a["h_vac"]) + np.var(b["h_vac"])) / n)
+    ra = a["range_sum"] / a["range_n"]
+    rb = b["range_sum"] / b["range_n"]
+    rs = math.sqrt((a["range_sum2"] / a["range_n"] - ra ** 2) / a["range_n"] * 2)
+    print("%s FP64 %.1f ms FP32 %.1f ms; vac/ion %.3f vs %.3f; range %.2f vs %.2f; Eel %.5g vs %.5g"
+          % (name, a["kernel_ms"], b["kernel_ms"], va, vb, ra, rb, a["Eel"], b["Eel"]))
+    assert abs(vb - va) <= 0.01 * va and abs(vb - va) <= 2 * sig
+    assert abs(rb - ra) <= 0.01 * abs(ra) and abs(rb - ra) <= 2 * rs
+    assert b["Eel"] / b["E0"] == pytest.approx(a["Eel"] / a["E0"], rel=0.01)
+    tot = b["Eel"] + b["Enuc"] + b["Eunf"] + b["Elost"] + b["Edrop"] + b["Evac"]
+    assert tot == pytest.approx(b["E0"], rel=1e-11)
