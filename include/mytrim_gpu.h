@@ -132,6 +132,7 @@ typedef struct {
   int grid, block, waves;
   long long queue_peak;  /* most recoils held in the HBM queue */
   long long queue_ops;   /* wave-level HBM queue transfers */
+  long long surrogate;   /* collisions whose angle came from the surrogate */
 } mt_tallies;
 
 /* Tuning knobs of the persistent kernel (0 = default). */
@@ -164,6 +165,21 @@ typedef struct {
   int mat_tag;                           /* material tag at this collision (cluster index) */
 } mt_event;
 
+/* Scattering surrogate (no counterpart in the reference): an MLP
+ * 2 -> 64 -> 64 -> 1 (tanh) from (log10 eps, b) to ln sin^2(theta/2), trained
+ * on the device on the engine's own universal-potential scattering; used by
+ * MT_FP32_SURROGATE runs inside its domain, the MAGIC solve outside it. */
+#define MT_SURROGATE_PARAMS 4417
+typedef struct {
+  double eps_min, eps_max; /* reduced-energy domain, sampled log-uniform */
+  double b_max;            /* reduced impact parameters in [0, b_max] */
+  long long n_train;       /* training pairs generated on the device */
+  int epochs, batch;
+  double lr;               /* Adam step size (halved at 1/3 and 2/3 of the run) */
+  uint32_t seed;
+  int bf16;                /* inference with BF16 dot products (else FP32 FMA) */
+} mt_surrogate_opts;
+
 const char* mt_last_error(void);
 const char* mt_version(void);
 
@@ -187,6 +203,17 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* op
  * is updated in place; events[0..*n_events) hold every collision. */
 int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, long long* n_used,
                    mt_event* events, int max_events, int* n_events);
+
+/* Train the surrogate from `params` (initial values in, trained values out);
+ * epoch_loss[e] = mean squared error (normalised units) of epoch e. */
+int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* opts, float* params, double* epoch_loss);
+/* Use `params` for MT_FP32_SURROGATE runs on this context. */
+int mt_surrogate_install(mt_ctx* ctx, const mt_surrogate_opts* opts, const float* params);
+/* Installed surrogate at (eps, b): s2 = sin^2(theta/2), -1 outside its domain. */
+int mt_surrogate_eval(mt_ctx* ctx, const double* eps, const double* b, long long n, double* s2);
+/* Loss and gradient of one batch (targets ln s2), FP32 back-propagation. */
+int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* opts, const float* params, const double* eps,
+                      const double* b, const double* ln_s2, int n, float* grad, double* loss);
 
 /* getrstop for ion (Z1, m1) at energies E[0..n) in material `mat` [eV/A]. */
 int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, long long n,

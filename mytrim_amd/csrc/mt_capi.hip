@@ -37,6 +37,16 @@ struct FlightParams {
   long long* n_used;
 };
 hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t stream);
+hipError_t mlp_make_data(uint32_t seed, long long n, double lmin, double lmax, double bmax, const MlpNorm& nm,
+                         float* X, float* T, hipStream_t st);
+hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
+                     float* rec, float* g, float* m, float* v, int* t, float* batch_loss, double* losses,
+                     hipStream_t st);
+hipError_t mlp_batch_grad(const float* p, const float* X, const float* T, int B, float* rec, float* g,
+                          float* loss, hipStream_t st);
+hipError_t mlp_pack(const float* p, uint32_t* w2bf, hipStream_t st);
+hipError_t mlp_infer_launch(const MlpDev& S, const double* eps, const double* b, long long n, double* s2,
+                            hipStream_t st);
 }  // namespace mt
 
 using namespace mt;
@@ -107,6 +117,11 @@ struct mt_ctx {
   // single-flight path: device tables per ion species (Z, m), event buffers
   std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
   DBuf f_ion, f_u, f_ev, f_used;
+  // scattering surrogate
+  DBuf mlp_p, mlp_w2bf, mlp_X, mlp_T, mlp_rec, mlp_g, mlp_m, mlp_v, mlp_t, mlp_bl;
+  MlpNorm mlp_norm{};
+  bool mlp_ready = false;
+  int mlp_bf16 = 0;
   DBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
 
   ~mt_ctx() {
@@ -122,6 +137,8 @@ struct mt_ctx {
     f_u.release();
     f_ev.release();
     f_used.release();
+    DBuf* ml[] = {&mlp_p, &mlp_w2bf, &mlp_X, &mlp_T, &mlp_rec, &mlp_g, &mlp_m, &mlp_v, &mlp_t, &mlp_bl};
+    for (DBuf* b : ml) b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -334,7 +351,10 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
   if (pk->n_hist < 0 || pk->n_hist > (1ll << 31)) return fail(MT_E_LIMIT, "n_hist out of range");
   if (pk->hist0 < 0 || pk->hist0 + pk->n_hist > (1ll << 32)) return fail(MT_E_LIMIT, "history index out of range");
-  if (o->arith != MT_FP64 && o->arith != MT_FP32) return fail(MT_E_ARG, "arithmetic mode not available in this build");
+  if (o->arith < MT_FP64 || o->arith > MT_FP32_SURROGATE) return fail(MT_E_ARG, "unknown arithmetic mode");
+  if (o->arith == MT_FP32_SURROGATE && !ctx->mlp_ready) return fail(MT_E_STATE, "no surrogate installed");
+  if (o->arith == MT_FP32_SURROGATE && ctx->tgt.potential != MT_UNIVERSAL)
+    return fail(MT_E_ARG, "the surrogate is trained on the universal potential");
   if (o->range_nb < 1 || o->lat_nb < 1 || o->vac_nb < 1) return fail(MT_E_ARG, "histograms need >= 1 bin");
   if (!(o->range_bw > 0.0) || !(o->lat_bw > 0.0) || !(o->vac_bw > 0.0)) return fail(MT_E_ARG, "bin widths must be positive");
   HIPCHECK(hipSetDevice(ctx->device));
@@ -491,6 +511,12 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     P.h_state = ctx->per[6].as<int>();
     P.h_Eel = ctx->per[7].as<double>();
   }
+  if (o->arith == MT_FP32_SURROGATE) {
+    P.mlp.p = ctx->mlp_p.as<float>();
+    P.mlp.w2bf = ctx->mlp_w2bf.as<uint32_t>();
+    P.mlp.n = ctx->mlp_norm;
+    P.mlp.bf16 = ctx->mlp_bf16;
+  }
 
   HIPCHECK(hipEventRecord(ctx->ev0, st));
   HIPCHECK(cascade_launch(P, o->arith, pot, perhist, grid, block, lds, st));
@@ -519,6 +545,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   out->newton = (long long)cnt[cNewton];
   out->disp = (long long)cnt[cDisp];
   out->range_n = (long long)cnt[cRangeN];
+  out->surrogate = (long long)cnt[cSurr];
   // compensated sum of the per-wave energy partials
   double sums[2] = {0.0, 0.0}, errs[2] = {0.0, 0.0};
   for (int w = 0; w < waves; ++w)
@@ -624,6 +651,131 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
   if (used[1] > 0)
     HIPCHECK(hipMemcpyAsync(events, ctx->f_ev.p, size_t(used[1]) * sizeof(mt_event), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+static std::string surrogate_norm(const mt_surrogate_opts* o, MlpNorm& nm) {
+  if (!(o->eps_min > 0.0) || !(o->eps_max > o->eps_min)) return "need 0 < eps_min < eps_max";
+  if (!(o->b_max > 0.0)) return "need b_max > 0";
+  const double l0 = std::log10(o->eps_min), l1 = std::log10(o->eps_max);
+  nm.e_mid = float(0.5 * (l0 + l1));
+  nm.e_inv_half = float(2.0 / (l1 - l0));
+  nm.b_mid = float(0.5 * o->b_max);
+  nm.b_inv_half = float(2.0 / o->b_max);
+  nm.y_off = -12.0f;
+  nm.y_scale = 12.0f;
+  nm.eps_lo = float(o->eps_min);
+  nm.eps_hi = float(o->eps_max);
+  nm.b_hi = float(o->b_max);
+  return "";
+}
+
+int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* o, float* params, double* epoch_loss) {
+  if (!ctx || !o || !params || !epoch_loss) return fail(MT_E_ARG, "NULL argument");
+  MlpNorm nm;
+  std::string msg = surrogate_norm(o, nm);
+  if (!msg.empty()) return fail(MT_E_ARG, msg);
+  if (o->batch < 64 || o->batch > 65536 || o->epochs < 1 || o->n_train < o->batch || o->n_train > (1ll << 31))
+    return fail(MT_E_ARG, "bad batch / epochs / n_train");
+  HIPCHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const long long n = o->n_train;
+  const int B = o->batch;
+  const int steps = int(n / B);
+  HIPCHECK(ctx->mlp_p.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->mlp_X.ensure(size_t(n) * 2 * sizeof(float)));
+  HIPCHECK(ctx->mlp_T.ensure(size_t(n) * sizeof(float)));
+  HIPCHECK(ctx->mlp_rec.ensure(size_t(B) * 260 * sizeof(float)));
+  HIPCHECK(ctx->mlp_g.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->mlp_m.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->mlp_v.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->mlp_t.ensure(sizeof(int)));
+  HIPCHECK(ctx->mlp_bl.ensure(size_t(steps) * sizeof(float)));
+  HIPCHECK(hipMemcpyAsync(ctx->mlp_p.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemsetAsync(ctx->mlp_m.p, 0, kMlpParams * sizeof(float), st));
+  HIPCHECK(hipMemsetAsync(ctx->mlp_v.p, 0, kMlpParams * sizeof(float), st));
+  HIPCHECK(hipMemsetAsync(ctx->mlp_t.p, 0, sizeof(int), st));
+  HIPCHECK(mlp_make_data(o->seed, n, std::log10(o->eps_min), std::log10(o->eps_max), o->b_max, nm,
+                         ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), st));
+  HIPCHECK(mlp_train(ctx->mlp_p.as<float>(), ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), n, B, o->epochs,
+                     float(o->lr), ctx->mlp_rec.as<float>(), ctx->mlp_g.as<float>(), ctx->mlp_m.as<float>(),
+                     ctx->mlp_v.as<float>(), ctx->mlp_t.as<int>(), ctx->mlp_bl.as<float>(), epoch_loss, st));
+  HIPCHECK(hipMemcpyAsync(params, ctx->mlp_p.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+int mt_surrogate_install(mt_ctx* ctx, const mt_surrogate_opts* o, const float* params) {
+  if (!ctx || !o || !params) return fail(MT_E_ARG, "NULL argument");
+  MlpNorm nm;
+  std::string msg = surrogate_norm(o, nm);
+  if (!msg.empty()) return fail(MT_E_ARG, msg);
+  HIPCHECK(hipSetDevice(ctx->device));
+  HIPCHECK(ctx->mlp_p.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->mlp_w2bf.ensure(64 * 32 * sizeof(uint32_t)));
+  HIPCHECK(hipMemcpyAsync(ctx->mlp_p.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(mlp_pack(ctx->mlp_p.as<float>(), ctx->mlp_w2bf.as<uint32_t>(), ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_norm = nm;
+  ctx->mlp_bf16 = o->bf16 ? 1 : 0;
+  ctx->mlp_ready = true;
+  return MT_OK;
+}
+
+int mt_surrogate_eval(mt_ctx* ctx, const double* eps, const double* b, long long n, double* s2) {
+  if (!ctx || (n > 0 && (!eps || !b || !s2))) return fail(MT_E_ARG, "NULL argument");
+  if (!ctx->mlp_ready) return fail(MT_E_STATE, "no surrogate installed");
+  if (n <= 0) return MT_OK;
+  HIPCHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  HIPCHECK(ctx->scratch_a.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_b.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_c.ensure(n * sizeof(double)));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, eps, n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_b.p, b, n * sizeof(double), hipMemcpyHostToDevice, st));
+  MlpDev S;
+  S.p = ctx->mlp_p.as<float>();
+  S.w2bf = ctx->mlp_w2bf.as<uint32_t>();
+  S.n = ctx->mlp_norm;
+  S.bf16 = ctx->mlp_bf16;
+  HIPCHECK(mlp_infer_launch(S, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n,
+                            ctx->scratch_c.as<double>(), st));
+  HIPCHECK(hipMemcpyAsync(s2, ctx->scratch_c.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* o, const float* params, const double* eps,
+                      const double* b, const double* ln_s2, int n, float* grad, double* loss) {
+  if (!ctx || !o || !params || !eps || !b || !ln_s2 || !grad || !loss) return fail(MT_E_ARG, "NULL argument");
+  if (n < 1 || n > 65536) return fail(MT_E_ARG, "batch out of range");
+  MlpNorm nm;
+  std::string msg = surrogate_norm(o, nm);
+  if (!msg.empty()) return fail(MT_E_ARG, msg);
+  std::vector<float> X(2 * size_t(n)), T(n);
+  for (int i = 0; i < n; ++i) {
+    X[2 * i] = float((std::log10(eps[i]) - nm.e_mid) * nm.e_inv_half);
+    X[2 * i + 1] = float((b[i] - nm.b_mid) * nm.b_inv_half);
+    T[i] = float((ln_s2[i] - nm.y_off) / nm.y_scale);
+  }
+  HIPCHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  HIPCHECK(ctx->scratch_a.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->scratch_b.ensure(X.size() * sizeof(float)));
+  HIPCHECK(ctx->scratch_c.ensure(T.size() * sizeof(float) + 16));
+  HIPCHECK(ctx->scratch_d.ensure(size_t(n) * 260 * sizeof(float)));
+  HIPCHECK(ctx->scratch_e.ensure((kMlpParams + 1) * sizeof(float)));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_b.p, X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_c.p, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  float* g = ctx->scratch_e.as<float>();
+  HIPCHECK(mlp_batch_grad(ctx->scratch_a.as<float>(), ctx->scratch_b.as<float>(), ctx->scratch_c.as<float>(), n,
+                          ctx->scratch_d.as<float>(), g, g + kMlpParams, st));
+  float lf = 0.0f;
+  HIPCHECK(hipMemcpyAsync(grad, g, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&lf, g + kMlpParams, sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  *loss = lf;
   return MT_OK;
 }
 

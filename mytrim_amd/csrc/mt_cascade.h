@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 
+#include "mt_mlp.h"
 #include "mt_tables.h"
 
 namespace mt {
@@ -41,7 +42,7 @@ enum ErrBits {
 
 enum Counter {
   cColl = 0, cFlight, cVac, cRepl, cSubst, cInter, cLost, cVacuum, cRuth, cNewton, cDisp,
-  cRangeN, kNumCounters = 16
+  cRangeN, cSurr, kNumCounters = 16
 };
 
 enum Energy { eEel = 0, eEnuc, eEunf, eElost, eEdrop, eEvac, eRangeSum, eRangeSum2, kNumEnergy = 8 };
@@ -86,6 +87,8 @@ struct Params {
   double *h_x, *h_y, *h_z;
   int* h_state;
   double* h_Eel;
+  // scattering surrogate (FP32 + surrogate mode)
+  MlpDev mlp;
 };
 
 // LDS bytes of one wave's recoil stack entry

@@ -23,6 +23,7 @@
 
 #include "mt_cascade.h"
 #include "mt_geometry.h"
+#include "mt_mlp.h"
 #include "mt_philox.h"
 #include "mt_physics.h"
 
@@ -103,11 +104,11 @@ __host__ __device__ constexpr int stack_bytes(int cap) { return ((kStackEntryByt
 // of one collision (trim.C:88-272) in precision R.  FP64 follows the
 // reference's expressions; FP32 evaluates the same quantities in single
 // precision (scatter_f32 for the angle) and hands back doubles.
-template <typename R, int POT>
+template <int ARITH, int POT>
 __device__ __forceinline__ void physics(const TableView& T, int sp, int k, double E, int ic, uint32_t klo,
                                         uint32_t khi, uint32_t seed, uint32_t hist, const P4& blk, double cw,
-                                        int& nn, double& ls, double& dee, double& s2, double& ct, double& st,
-                                        int& it) {
+                                        const MlpDev& mlp, int& nn, double& ls, double& dee, double& s2,
+                                        double& ct, double& st, int& it) {
   const MatConst& mc = T.mat[sp * T.n_mat + k];
   const int eb = sp * T.n_el + mc.el_begin;
   double hh = u32d(blk.z);
@@ -117,7 +118,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
   }
   if (nn >= mc.n_el) nn = mc.n_el - 1;
   const ElemConst& ec = T.el[eb + nn];
-  if constexpr (sizeof(R) == 8) {
+  if constexpr (ARITH == 0) {
     // free path and maximum impact parameter (trim.C:88-95)
     double eps = E * mc.f;
     const double eeg = sqrt(eps * mc.epsdg);
@@ -149,8 +150,17 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     const float b = p / (float)ec.ai;
     ls = (double)lsf;
     dee = ls * (double)getrstop<float>(T, sp, k, Ef);
-    float s2f, ctf, stf;
-    scatter_f32<POT>(eps, b, s2f, ctf, stf, it);
+    float s2f = -1.0f, ctf, stf;
+    if constexpr (ARITH == 2) {
+      if (eps <= 10.0f) s2f = mlp_s2(mlp, eps, b);  // the closed form is cheaper above 10
+    }
+    if (s2f >= 0.0f) {
+      ctf = 1.0f - 2.0f * s2f;
+      stf = 2.0f * m_sqrt(fmaxf(0.0f, s2f * (1.0f - s2f)));
+      it = -1;
+    } else {
+      scatter_f32<POT>(eps, b, s2f, ctf, stf, it);
+    }
     s2 = s2f;
     ct = ctf;
     st = stf;
@@ -159,7 +169,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
 
 }  // namespace
 
-template <typename R, int POT, int GEO, bool PERHIST>
+template <int ARITH, int POT, int GEO, bool PERHIST>
 __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
   extern __shared__ __align__(16) unsigned char lds[];
   {
@@ -198,7 +208,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
   bool wave_idle = false;  // registered idle
   unsigned long long idle_since = 0;
   unsigned long long c_coll = 0, c_flight = 0, c_vac = 0, c_repl = 0, c_subst = 0, c_inter = 0,
-                     c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0;
+                     c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0, c_surr = 0;
   unsigned long long c_qops = 0;
   // per-lane accumulators
   long long l_newton = 0, l_kpvac = 0;
@@ -371,7 +381,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
     }
 
     // ------------------------------------------------------------ one collision
-    bool ended = false, vacuum = false, push = false, ruth = false;
+    bool ended = false, vacuum = false, push = false, ruth = false, surr = false;
     int state = 0;
     double rx = 0.0, ry = 0.0, rz = 0.0, rdx = 0.0, rdy = 0.0, rdz = 0.0, rE = 0.0;
     int rsp = 0, rtag = -1;
@@ -395,10 +405,12 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
         const MatConst& mc = T.mat[F.sp * T.n_mat + k];
         double ls, dee, s2, ct, st;
         int it, nn;
-        physics<R, POT>(T, F.sp, k, F.E, F.ic, klo, khi, P.seed, F.hist, blk, P.cw, nn, ls, dee, s2, ct, st, it);
+        physics<ARITH, POT>(T, F.sp, k, F.E, F.ic, klo, khi, P.seed, F.hist, blk, P.cw, P.mlp, nn, ls, dee, s2, ct,
+                            st, it);
         const ElemConst& ec = T.el[F.sp * T.n_el + mc.el_begin + nn];
         ruth = (it == 0);
-        l_newton += it;
+        surr = (it < 0);
+        l_newton += it > 0 ? it : 0;
         if (it >= kMaxNewton) err |= kErrNewton;
         // energy transfer and momenta (trim.C:275-297)
         double den = ec.ec * s2 * F.E;
@@ -532,6 +544,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
     // wave-uniform event counts
     c_coll += __popcll(__ballot(active && !vacuum));
     c_ruth += __popcll(__ballot(active && !vacuum && ruth));
+    c_surr += __popcll(__ballot(active && !vacuum && surr));
     c_vac += __popcll(__ballot(active && state == -1));
     c_repl += __popcll(__ballot(active && state == 1));
     c_subst += __popcll(__ballot(active && state == 2));
@@ -662,6 +675,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
     atomicAdd(&C[cNewton], (unsigned long long)l_newton);
     atomicAdd(&C[cDisp], c_disp);
     atomicAdd(&C[cRangeN], c_rangen);
+    atomicAdd(&C[cSurr], c_surr);
     atomicAdd(&ctrl[kQueueOps], c_qops);
     const int gw = blockIdx.x * nwb + wid;
     double* we = P.wave_energy + (size_t)gw * (2 * kNumEnergy);
@@ -689,27 +703,29 @@ namespace mt {
 
 typedef void (*CascadeKernel)(Params);
 
-template <typename R, int POT, int GEO>
+template <int ARITH, int POT, int GEO>
 static CascadeKernel pick3(bool perhist) {
-  return perhist ? cascade_kernel<R, POT, GEO, true> : cascade_kernel<R, POT, GEO, false>;
+  return perhist ? cascade_kernel<ARITH, POT, GEO, true> : cascade_kernel<ARITH, POT, GEO, false>;
 }
 
-template <typename R, int POT>
+template <int ARITH, int POT>
 static CascadeKernel pick2(int geo, bool perhist) {
-  if (geo == 1) return pick3<R, POT, 1>(perhist);
-  if (geo == 2) return pick3<R, POT, 2>(perhist);
-  return pick3<R, POT, 0>(perhist);
+  if (geo == 1) return pick3<ARITH, POT, 1>(perhist);
+  if (geo == 2) return pick3<ARITH, POT, 2>(perhist);
+  return pick3<ARITH, POT, 0>(perhist);
 }
 
-template <typename R>
+template <int ARITH>
 static CascadeKernel pick1(int potential, int geo, bool perhist) {
-  if (potential == 1) return pick2<R, 1>(geo, perhist);
-  if (potential == 2) return pick2<R, 2>(geo, perhist);
-  return pick2<R, 0>(geo, perhist);
+  if (potential == 1) return pick2<ARITH, 1>(geo, perhist);
+  if (potential == 2) return pick2<ARITH, 2>(geo, perhist);
+  return pick2<ARITH, 0>(geo, perhist);
 }
 
+// the surrogate is trained on the universal potential only
 static CascadeKernel pick(int arith, int potential, int geo, bool perhist) {
-  return arith == 1 ? pick1<float>(potential, geo, perhist) : pick1<double>(potential, geo, perhist);
+  if (arith == 2) return pick2<2, 0>(geo, perhist);
+  return arith == 1 ? pick1<1>(potential, geo, perhist) : pick1<0>(potential, geo, perhist);
 }
 
 hipError_t cascade_occupancy(int arith, int potential, int geo, bool perhist, int block, size_t lds,

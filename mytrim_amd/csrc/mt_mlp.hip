@@ -1,0 +1,226 @@
+// Training and evaluation kernels of the scattering surrogate (mt_mlp.h).
+//
+//   mlp_data      training pairs: eps log-uniform, b uniform, target
+//                 ln s2 from scatter_stable<double> (universal potential),
+//                 one Philox stream per sample
+//   mlp_fwdbwd    one lane per sample: forward pass and the per-sample
+//                 back-propagated deltas (stored for the gradient kernel)
+//   mlp_grad      one lane per parameter: batch sum of delta x activation
+//                 (deterministic, no atomics); the last lane sums the loss
+//   mlp_adam      Adam update, step counter and step-size schedule on device
+// An epoch (every batch: fwdbwd, grad, adam) is captured once in a hipGraph
+// and replayed.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mt_mlp.h"
+#include "mt_philox.h"
+#include "mt_physics.h"
+
+namespace mt {
+
+// per-sample record written by mlp_fwdbwd: x0 x1 dy h1[64] dh1[64] h2[64] dh2[64] e2
+constexpr int kRec = 3 + 4 * 64 + 1;
+
+__global__ void __launch_bounds__(256) mlp_data(uint32_t seed, long long n, double lmin, double lmax, double bmax,
+                                                MlpNorm nm, float* X, float* T) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const P4 r = philox10((uint32_t)i, (uint32_t)(i >> 32), 0x5a17u, 0u, seed, 0x6d6c70u);
+    const double le = lmin + (lmax - lmin) * u53(r.x, r.y);
+    const double eps = pow(10.0, le);
+    const double b = bmax * u32d(r.z);
+    double s2, ct, st;
+    int it;
+    scatter_stable<double, 0>(eps, b, s2, ct, st, it);
+    const double ls2 = log(fmax(s2, 1e-300));
+    X[2 * i] = (float)((le - nm.e_mid) * nm.e_inv_half);
+    X[2 * i + 1] = (float)((b - nm.b_mid) * nm.b_inv_half);
+    T[i] = (float)((ls2 - nm.y_off) / nm.y_scale);
+  }
+}
+
+__global__ void __launch_bounds__(256) mlp_fwdbwd(const float* __restrict__ p, const float* X, const float* T,
+                                                  long long base, int B, float* rec) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const long long i = base + s;
+  const float x0 = X[2 * i], x1 = X[2 * i + 1];
+  float h1[64], h2[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) h1[j] = tanhf(p[kOffW1 + 2 * j] * x0 + p[kOffW1 + 2 * j + 1] * x1 + p[kOffB1 + j]);
+  float y = p[kOffB3];
+  for (int k = 0; k < 64; ++k) {
+    float a = p[kOffB2 + k];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) a += p[kOffW2 + 64 * k + j] * h1[j];
+    h2[k] = tanhf(a);
+    y += p[kOffW3 + k] * h2[k];
+  }
+  const float e = y - T[i];
+  const float dy = 2.0f * e / (float)B;  // d(mean squared error)/dy
+  float* r = rec + (size_t)s * kRec;
+  r[0] = x0;
+  r[1] = x1;
+  r[2] = dy;
+  float dh1[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) dh1[j] = 0.0f;
+  for (int k = 0; k < 64; ++k) {
+    const float d2 = dy * p[kOffW3 + k] * (1.0f - h2[k] * h2[k]);
+    r[3 + 128 + k] = h2[k];
+    r[3 + 192 + k] = d2;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) dh1[j] += p[kOffW2 + 64 * k + j] * d2;
+  }
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    r[3 + j] = h1[j];
+    r[3 + 64 + j] = dh1[j] * (1.0f - h1[j] * h1[j]);
+  }
+  r[kRec - 1] = e * e;
+}
+
+__global__ void __launch_bounds__(256) mlp_grad(const float* rec, int B, float* g, float* loss_slot) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > kMlpParams) return;
+  float acc = 0.0f;
+  if (q == kMlpParams) {  // mean squared error of the batch
+    for (int s = 0; s < B; ++s) acc += rec[(size_t)s * kRec + kRec - 1];
+    *loss_slot = acc / (float)B;
+    return;
+  }
+  // dL/dparam = sum over samples of (delta of the unit it feeds) x (its input)
+  int a_off, b_off;  // record offsets: delta index, input index (-1 = bias)
+  if (q < kOffB1) {
+    a_off = 3 + 64 + q / 2;
+    b_off = q % 2;
+  } else if (q < kOffW2) {
+    a_off = 3 + 64 + (q - kOffB1);
+    b_off = -1;
+  } else if (q < kOffB2) {
+    const int k = (q - kOffW2) / 64, j = (q - kOffW2) % 64;
+    a_off = 3 + 192 + k;
+    b_off = 3 + j;
+  } else if (q < kOffW3) {
+    a_off = 3 + 192 + (q - kOffB2);
+    b_off = -1;
+  } else if (q < kOffB3) {
+    a_off = 2;
+    b_off = 3 + 128 + (q - kOffW3);
+  } else {
+    a_off = 2;
+    b_off = -1;
+  }
+  for (int s = 0; s < B; ++s) {
+    const float* r = rec + (size_t)s * kRec;
+    acc += r[a_off] * (b_off < 0 ? 1.0f : r[b_off]);
+  }
+  g[q] = acc;
+}
+
+// Adam (Kingma & Ba 2015) with beta1 0.9, beta2 0.999; the step size halves
+// at 1/3 and 2/3 of the run; t counts steps on the device
+__global__ void __launch_bounds__(256) mlp_adam(float* p, const float* g, float* m, float* v, int* t, float lr0,
+                                                int total_steps) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int step = *t + 1;
+  if (q < kMlpParams) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float lr = lr0 * ((3 * step > 2 * total_steps) ? 0.25f : (3 * step > total_steps) ? 0.5f : 1.0f);
+    m[q] = b1 * m[q] + (1.0f - b1) * g[q];
+    v[q] = b2 * v[q] + (1.0f - b2) * g[q] * g[q];
+    const float mh = m[q] / (1.0f - powf(b1, (float)step));
+    const float vh = v[q] / (1.0f - powf(b2, (float)step));
+    p[q] -= lr * mh / (sqrtf(vh) + 1e-8f);
+  }
+}
+
+__global__ void mlp_tick(int* t) { *t += 1; }
+
+__global__ void __launch_bounds__(256) mlp_infer(MlpDev S, const double* eps, const double* b, long long n,
+                                                 double* s2) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = mlp_s2(S, (float)eps[i], (float)b[i]);
+    s2[i] = v < 0.0f ? -1.0 : (double)v;
+  }
+}
+
+__global__ void mlp_pack_bf16(const float* p, uint32_t* w2bf) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < 64 * 32) {
+    const int k = q / 32, jp = q % 32;
+    w2bf[q] = bf16_bits(p[kOffW2 + 64 * k + 2 * jp]) | (bf16_bits(p[kOffW2 + 64 * k + 2 * jp + 1]) << 16);
+  }
+}
+
+// ------------------------------------------------------------------ host side
+static int grid_of(long long n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
+
+hipError_t mlp_make_data(uint32_t seed, long long n, double lmin, double lmax, double bmax, const MlpNorm& nm,
+                         float* X, float* T, hipStream_t st) {
+  hipLaunchKernelGGL(mlp_data, dim3(grid_of(n)), dim3(256), 0, st, seed, n, lmin, lmax, bmax, nm, X, T);
+  return hipGetLastError();
+}
+
+// one optimisation step on samples [base, base + B)
+static void step_launch(float* p, const float* X, const float* T, long long base, int B, float* rec, float* g,
+                        float* m, float* v, int* t, float lr0, int total, float* loss_slot, hipStream_t st) {
+  hipLaunchKernelGGL(mlp_fwdbwd, dim3((B + 255) / 256), dim3(256), 0, st, p, X, T, base, B, rec);
+  hipLaunchKernelGGL(mlp_grad, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, rec, B, g, loss_slot);
+  hipLaunchKernelGGL(mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, p, g, m, v, t, lr0, total);
+  hipLaunchKernelGGL(mlp_tick, dim3(1), dim3(1), 0, st, t);
+}
+
+// Train for `epochs` over n samples in batches of B; one epoch captured as a
+// graph and replayed.  losses[e] = mean batch loss of epoch e.
+hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
+                     float* rec, float* g, float* m, float* v, int* t, float* batch_loss, double* losses,
+                     hipStream_t st) {
+  const int steps = (int)(n / B);
+  if (steps < 1) return hipErrorInvalidValue;
+  const int total = steps * epochs;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  for (int s = 0; s < steps; ++s)
+    step_launch(p, X, T, (long long)s * B, B, rec, g, m, v, t, lr0, total, batch_loss + s, st);
+  e = hipStreamEndCapture(st, &graph);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return e;
+  std::vector<float> bl(steps);
+  for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
+    e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(bl.data(), batch_loss, steps * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    double acc = 0.0;
+    for (float x : bl) acc += x;
+    losses[ep] = acc / steps;
+  }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  return e;
+}
+
+// gradient and loss of one batch (for checks against the CPU)
+hipError_t mlp_batch_grad(const float* p, const float* X, const float* T, int B, float* rec, float* g,
+                          float* loss, hipStream_t st) {
+  hipLaunchKernelGGL(mlp_fwdbwd, dim3((B + 255) / 256), dim3(256), 0, st, p, X, T, 0ll, B, rec);
+  hipLaunchKernelGGL(mlp_grad, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, rec, B, g, loss);
+  return hipGetLastError();
+}
+
+hipError_t mlp_pack(const float* p, uint32_t* w2bf, hipStream_t st) {
+  hipLaunchKernelGGL(mlp_pack_bf16, dim3(8), dim3(256), 0, st, p, w2bf);
+  return hipGetLastError();
+}
+
+hipError_t mlp_infer_launch(const MlpDev& S, const double* eps, const double* b, long long n, double* s2,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(mlp_infer, dim3(grid_of(n)), dim3(256), 0, st, S, eps, b, n, s2);
+  return hipGetLastError();
+}
+
+}  // namespace mt

@@ -188,7 +188,8 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
   iters = it;
 }
 
-// Single-precision scattering for the FP32 path.  Same Newton solve and
+// Scattering without the small-angle cancellation (the FP32 path, and the
+// surrogate's training targets in FP64).  Same Newton solve and
 // MAGIC constants as scatter(), but the small-angle end is evaluated without
 // cancellation: at the apsis b^2/r + v r/eps - r = 0, so
 //   r - b = v r^2 / (eps (r + b)),
@@ -197,9 +198,8 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
 // which is algebraically the reference's co = (b + delta + roc)/(r + roc),
 // s2 = 1 - co^2 but keeps full relative precision when s2 << 1 (glancing
 // collisions transfer s2 ~ tmin / (ec E), below single-precision epsilon).
-template <int POT>
-__device__ __forceinline__ void scatter_f32(float eps, float b, float& s2, float& ct, float& st, int& iters) {
-  typedef float R;
+template <typename R, int POT>
+__device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, int& iters) {
   if (eps > K(10.0)) {
     s2 = K(1.0) / (K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
     ct = K(1.0) - K(2.0) * s2;
@@ -270,6 +270,11 @@ __device__ __forceinline__ void scatter_f32(float eps, float b, float& s2, float
   ct = K(1.0) - K(2.0) * s2;
   st = K(2.0) * m_sqrt(mx(K(0.0), s2 * (K(1.0) - s2)));
   iters = it;
+}
+
+template <int POT>
+__device__ __forceinline__ void scatter_f32(float eps, float b, float& s2, float& ct, float& st, int& iters) {
+  scatter_stable<float, POT>(eps, b, s2, ct, st, iters);
 }
 
 #undef K

@@ -7,6 +7,7 @@ path: every call goes to the HIP kernels and raises on failure.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from typing import Optional
 
@@ -27,7 +28,8 @@ MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
 EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt_ctx_set_tables",
            "mt_default_tables",
            "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_trim_flight",
-           "mt_stopping_batch", "mt_scatter_batch"]
+           "mt_stopping_batch", "mt_scatter_batch", "mt_surrogate_train", "mt_surrogate_install",
+           "mt_surrogate_eval", "mt_surrogate_grad"]
 
 
 class Target(C.Structure):
@@ -65,12 +67,21 @@ class Tallies(C.Structure):
         ("repl_hist", lp), ("h_vac", ip), ("h_coll", lp), ("h_flights", ip), ("h_x", dp),
         ("h_y", dp), ("h_z", dp), ("h_state", ip), ("h_Eel", dp), ("kernel_ms", C.c_double),
         ("grid", C.c_int), ("block", C.c_int), ("waves", C.c_int), ("queue_peak", C.c_longlong),
-        ("queue_ops", C.c_longlong)])
+        ("queue_ops", C.c_longlong), ("surrogate", C.c_longlong)])
 
 
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("stack_cap", C.c_int),
                 ("queue_cap", C.c_longlong), ("lds_hist", C.c_int)]
+
+
+class SurrogateOpts(C.Structure):
+    _fields_ = [("eps_min", C.c_double), ("eps_max", C.c_double), ("b_max", C.c_double),
+                ("n_train", C.c_longlong), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
+                ("seed", C.c_uint32), ("bf16", C.c_int)]
+
+
+N_SURROGATE_PARAMS = 4417
 
 
 class Ion(C.Structure):
@@ -112,6 +123,12 @@ def lib():
         L.mt_stopping_batch.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, dp,
                                         C.c_longlong, dp]
         L.mt_scatter_batch.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_longlong, dp, dp, dp, ip]
+        fp = C.POINTER(C.c_float)
+        so = C.POINTER(SurrogateOpts)
+        L.mt_surrogate_train.argtypes = [C.c_void_p, so, fp, dp]
+        L.mt_surrogate_install.argtypes = [C.c_void_p, so, fp]
+        L.mt_surrogate_eval.argtypes = [C.c_void_p, dp, dp, C.c_longlong, dp]
+        L.mt_surrogate_grad.argtypes = [C.c_void_p, so, fp, dp, dp, dp, C.c_int, fp, dp]
         _lib = L
     return _lib
 
@@ -216,7 +233,7 @@ class Engine:
         res = {k: getattr(out, k) for k in COUNTERS + ENERGIES}
         res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
                    kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
-                   queue_peak=out.queue_peak, queue_ops=out.queue_ops)
+                   queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate)
         res.update(hists)
         res.update(per)
         return res
@@ -237,6 +254,57 @@ class Engine:
         _check(lib().mt_scatter_batch(self.h, potential, _p(eps, dp), _p(b, dp), n, _p(s2, dp),
                                       _p(ct, dp), _p(st, dp), _p(it, ip)), "mt_scatter_batch")
         return s2, ct, st, it
+
+    # ---------------------------------------------------------- surrogate
+    @staticmethod
+    def surrogate_opts(eps_min=1e-7, eps_max=1e4, b_max=40.0, n_train=10_000_000, epochs=12,
+                       batch=4096, lr=2e-3, seed=7, bf16=False) -> SurrogateOpts:
+        return SurrogateOpts(eps_min, eps_max, b_max, n_train, epochs, batch, lr, seed, int(bf16))
+
+    @staticmethod
+    def surrogate_init(seed: int = 1) -> np.ndarray:
+        """Glorot-uniform weights, zero biases (flat layout of mt_mlp.h)."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros(N_SURROGATE_PARAMS, np.float32)
+        def glorot(n_in, n_out):
+            a = math.sqrt(6.0 / (n_in + n_out))
+            return rng.uniform(-a, a, n_in * n_out).astype(np.float32)
+        p[0:128] = glorot(2, 64)
+        p[192:192 + 4096] = glorot(64, 64)
+        p[4352:4416] = glorot(64, 1)
+        return p
+
+    def surrogate_train(self, opts: SurrogateOpts, params: np.ndarray):
+        params = np.ascontiguousarray(params, np.float32).copy()
+        loss = np.zeros(opts.epochs)
+        _check(lib().mt_surrogate_train(self.h, C.byref(opts), _p(params, C.POINTER(C.c_float)),
+                                        _p(loss, dp)), "mt_surrogate_train")
+        return params, loss
+
+    def surrogate_install(self, opts: SurrogateOpts, params: np.ndarray):
+        params = np.ascontiguousarray(params, np.float32)
+        _check(lib().mt_surrogate_install(self.h, C.byref(opts), _p(params, C.POINTER(C.c_float))),
+               "mt_surrogate_install")
+
+    def surrogate_eval(self, eps, b) -> np.ndarray:
+        eps = np.ascontiguousarray(eps, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        out = np.zeros(len(eps))
+        _check(lib().mt_surrogate_eval(self.h, _p(eps, dp), _p(b, dp), len(eps), _p(out, dp)),
+               "mt_surrogate_eval")
+        return out
+
+    def surrogate_grad(self, opts: SurrogateOpts, params, eps, b, ln_s2):
+        params = np.ascontiguousarray(params, np.float32)
+        eps = np.ascontiguousarray(eps, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        ln_s2 = np.ascontiguousarray(ln_s2, np.float64)
+        g = np.zeros(N_SURROGATE_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().mt_surrogate_grad(self.h, C.byref(opts), _p(params, C.POINTER(C.c_float)),
+                                       _p(eps, dp), _p(b, dp), _p(ln_s2, dp), len(eps),
+                                       _p(g, C.POINTER(C.c_float)), C.byref(loss)), "mt_surrogate_grad")
+        return g, loss.value
 
     def trim_flight(self, ion: Ion, u: np.ndarray, max_events: int = 1 << 16):
         """One TrimBase::trim() flight driven by the uniforms `u`.  Returns
