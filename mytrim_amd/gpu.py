@@ -257,7 +257,7 @@ class Engine:
 
     # ---------------------------------------------------------- surrogate
     @staticmethod
-    def surrogate_opts(eps_min=1e-7, eps_max=1e4, b_max=40.0, n_train=10_000_000, epochs=12,
+    def surrogate_opts(eps_min=1e-7, eps_max=10.0, b_max=40.0, n_train=10_000_000, epochs=12,
                        batch=4096, lr=2e-3, seed=7, bf16=False) -> SurrogateOpts:
         return SurrogateOpts(eps_min, eps_max, b_max, n_train, epochs, batch, lr, seed, int(bf16))
 
