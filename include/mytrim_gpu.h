@@ -142,6 +142,8 @@ typedef struct {
   int stack_cap;      /* per-wave LDS recoil stack entries */
   long long queue_cap;/* HBM queue entries (power of two) */
   int lds_hist;       /* bin vacancy/replacement depths in LDS first */
+  int graph;          /* >= 0: replay the run sequence (reset, kernel, flush,
+                         copies) from a cached hipGraph; < 0: launch directly */
 } mt_tuning;
 
 /* One flight (TrimBase::trim) for the drop-in single-call path. */
@@ -196,6 +198,14 @@ int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* tuning);
 
 int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts,
                     mt_tallies* out);
+
+/* Autotuner (no counterpart in the reference: the reference's tuning knob is
+ * its thread count, apps/runmytrim.C:59-68).  Runs the first `sample`
+ * histories of `pkas` under each candidate launch shape (block size, LDS stack
+ * depth, resident blocks per CU), `reps` times each, keeps the fastest
+ * (collisions per kernel second) in the context and returns it in *best. */
+int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, long long sample,
+                int reps, mt_tuning* best, double* best_rate);
 
 /* Single flight with the caller's uniform stream u[0..n_u): consumes draws in
  * the reference's order (trim.C:71, 143, 147, 322-333).  Returns MT_E_LIMIT

@@ -21,6 +21,12 @@ hipError_t cascade_occupancy(int arith, int potential, int geo, bool perhist, in
 hipError_t cascade_launch(const Params& P, int arith, int potential, bool perhist, int grid, int block, size_t lds,
                           hipStream_t stream);
 int cascade_stack_bytes(int cap);
+const void* cascade_function(int arith, int potential, int geo, bool perhist);
+int zero_spans_grid(const ZeroSpans& z);
+const void* zero_spans_function();
+const void* flush_energy_function();
+hipError_t zero_spans_launch(const ZeroSpans& z, hipStream_t st);
+hipError_t flush_energy_launch(const double* we, int waves, double* out, hipStream_t st);
 hipError_t stopping_launch(const unsigned char* tables, int species, int mat, const double* E, long long n,
                            double* out, hipStream_t stream);
 hipError_t scatter_launch(int potential, const double* eps, const double* b, long long n, double* s2,
@@ -89,6 +95,61 @@ struct DBuf {
   }
 };
 
+// pinned host staging (hipHostMalloc) for the run's results
+struct HBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(n, 16), hipHostMallocDefault);
+    if (e == hipSuccess) bytes = std::max<size_t>(n, 16);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// result block: control words, counters, energies, flushed energy sums
+constexpr int kResCtrl = 0;
+constexpr int kResCounters = kResCtrl + kNumCtrl;
+constexpr int kResEnergy = kResCounters + kNumCounters;
+constexpr int kResReduced = kResEnergy + kNumEnergy;
+constexpr int kResWords = kResReduced + 4;
+
+// what a cached run graph was built for (its memcpy nodes and kernel shape)
+struct RunGraphKey {
+  const void* fn = nullptr;
+  int grid = 0, block = 0;
+  size_t lds = 0;
+  void *res = nullptr, *hist = nullptr;
+  size_t hist_bytes = 0;
+  void *stage_res = nullptr, *stage_hist = nullptr;
+  bool operator==(const RunGraphKey& o) const {
+    return fn == o.fn && grid == o.grid && block == o.block && lds == o.lds && res == o.res && hist == o.hist &&
+           hist_bytes == o.hist_bytes && stage_res == o.stage_res && stage_hist == o.stage_hist;
+  }
+};
+
+struct RunGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraphNode_t n_zero = nullptr, n_kern = nullptr, n_flush = nullptr;
+  RunGraphKey key;
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key = RunGraphKey{};
+  }
+};
+
 uint64_t next_pow2(uint64_t x) {
   uint64_t p = 1;
   while (p < x) p <<= 1;
@@ -109,8 +170,9 @@ struct mt_ctx {
   mt_tuning tune{};
   DBuf tables, sh, cl, cx, cy, cz, cr;
   DBuf pka_sp, pka_E, pka_pos, pka_dir;
-  DBuf ctrl, counters, wave_energy, energy;
-  DBuf hist[5];
+  DBuf res, wave_energy, hist_all;
+  HBuf stage_res, stage_hist;
+  RunGraph graph;
   DBuf per[8];
   DBuf q[12];
   uint64_t qcap = 0;
@@ -125,11 +187,12 @@ struct mt_ctx {
   DBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
 
   ~mt_ctx() {
-    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &pka_sp, &pka_E, &pka_pos, &pka_dir, &ctrl,
-                   &counters, &wave_energy, &energy, &scratch_a, &scratch_b, &scratch_c, &scratch_d,
-                   &scratch_e};
+    graph.release();
+    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
+                   &wave_energy, &hist_all, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e};
     for (DBuf* b : all) b->release();
-    for (auto& b : hist) b.release();
+    stage_res.release();
+    stage_hist.release();
     for (auto& b : per) b.release();
     for (auto& b : q) b.release();
     for (auto& kv : flight_tables) kv.second->release();
@@ -416,30 +479,38 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     for (int i = 0; i < 12; ++i) HIPCHECK(ctx->q[i].ensure(qcap * sz[i]));
     ctx->qcap = qcap;
   }
-  HIPCHECK(hipMemsetAsync(ctx->q[11].p, 0, qcap * sizeof(uint32_t), st));
   int log2cap = 0;
   while ((uint64_t(1) << log2cap) < qcap) ++log2cap;
 
   // tallies
+  // control words, counters, energies and the flushed energy sums share one
+  // allocation (res); the five histograms share another (hist_all); both are
+  // copied back to pinned staging in one transfer each
   const int nb[5] = {o->range_nb + 2, o->lat_nb + 2, o->lat_nb + 2, o->vac_nb + 2, o->vac_nb + 2};
-  for (int i = 0; i < 5; ++i) {
-    HIPCHECK(ctx->hist[i].ensure(nb[i] * sizeof(unsigned long long)));
-    HIPCHECK(hipMemsetAsync(ctx->hist[i].p, 0, nb[i] * sizeof(unsigned long long), st));
-  }
-  HIPCHECK(ctx->ctrl.ensure(kNumCtrl * sizeof(unsigned long long)));
-  HIPCHECK(ctx->counters.ensure(kNumCounters * sizeof(unsigned long long)));
-  HIPCHECK(ctx->energy.ensure(kNumEnergy * sizeof(double)));
+  size_t hoff[6] = {0};
+  for (int i = 0; i < 5; ++i) hoff[i + 1] = hoff[i] + size_t(nb[i]);
+  const size_t hist_bytes = hoff[5] * sizeof(unsigned long long);
+  HIPCHECK(ctx->hist_all.ensure(hist_bytes));
+  HIPCHECK(ctx->res.ensure(kResWords * sizeof(unsigned long long)));
   HIPCHECK(ctx->wave_energy.ensure(size_t(waves) * 2 * kNumEnergy * sizeof(double)));
-  HIPCHECK(hipMemsetAsync(ctx->ctrl.p, 0, kNumCtrl * sizeof(unsigned long long), st));
-  HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, kNumCounters * sizeof(unsigned long long), st));
-  HIPCHECK(hipMemsetAsync(ctx->energy.p, 0, kNumEnergy * sizeof(double), st));
-  HIPCHECK(hipMemsetAsync(ctx->wave_energy.p, 0, size_t(waves) * 2 * kNumEnergy * sizeof(double), st));
+  HIPCHECK(ctx->stage_res.ensure(kResWords * sizeof(unsigned long long)));
+  HIPCHECK(ctx->stage_hist.ensure(hist_bytes));
   const size_t nh = size_t(std::max<long long>(n_hist, 1));
   const size_t per_sz[8] = {4, 8, 4, 8, 8, 8, 4, 8};
+  ZeroSpans zs{};
+  auto span = [&zs](void* p, size_t bytes) {
+    zs.p[zs.n] = p;
+    zs.bytes[zs.n] = bytes;
+    ++zs.n;
+  };
+  span(ctx->q[11].p, qcap * sizeof(uint32_t));
+  span(ctx->res.p, kResWords * sizeof(unsigned long long));
+  span(ctx->hist_all.p, hist_bytes);
+  span(ctx->wave_energy.p, size_t(waves) * 2 * kNumEnergy * sizeof(double));
   if (perhist) {
     for (int i = 0; i < 8; ++i) {
       HIPCHECK(ctx->per[i].ensure(nh * per_sz[i]));
-      HIPCHECK(hipMemsetAsync(ctx->per[i].p, 0, nh * per_sz[i], st));
+      span(ctx->per[i].p, (nh * per_sz[i] + 3) & ~size_t(3));
     }
   }
 
@@ -463,7 +534,8 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   P.pka_pos = ctx->pka_pos.as<double>();
   P.pka_dir = ctx->pka_dir.as<double>();
   P.seed = pk->seed;
-  P.ctrl = ctx->ctrl.as<unsigned long long>();
+  unsigned long long* res = ctx->res.as<unsigned long long>();
+  P.ctrl = res + kResCtrl;
   Queue& Q = P.q;
   Q.px = ctx->q[0].as<double>();
   Q.py = ctx->q[1].as<double>();
@@ -482,14 +554,15 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   P.admit_limit = (long long)(qcap / 4);
   P.max_ic = 10000000;
   P.max_wait_ticks = (unsigned long long)(60.0 * ctx->clock_khz * 1000.0);
-  P.counters = ctx->counters.as<unsigned long long>();
+  P.counters = res + kResCounters;
   P.wave_energy = ctx->wave_energy.as<double>();
-  P.energy = ctx->energy.as<double>();
-  P.range_hist = ctx->hist[0].as<unsigned long long>();
-  P.lat_y_hist = ctx->hist[1].as<unsigned long long>();
-  P.lat_z_hist = ctx->hist[2].as<unsigned long long>();
-  P.vac_hist = ctx->hist[3].as<unsigned long long>();
-  P.repl_hist = ctx->hist[4].as<unsigned long long>();
+  P.energy = reinterpret_cast<double*>(res + kResEnergy);
+  unsigned long long* hists = ctx->hist_all.as<unsigned long long>();
+  P.range_hist = hists + hoff[0];
+  P.lat_y_hist = hists + hoff[1];
+  P.lat_z_hist = hists + hoff[2];
+  P.vac_hist = hists + hoff[3];
+  P.repl_hist = hists + hoff[4];
   P.range_x0 = o->range_x0;
   P.range_bw = o->range_bw;
   P.range_nb = o->range_nb;
@@ -518,20 +591,76 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     P.mlp.bf16 = ctx->mlp_bf16;
   }
 
-  HIPCHECK(hipEventRecord(ctx->ev0, st));
-  HIPCHECK(cascade_launch(P, o->arith, pot, perhist, grid, block, lds, st));
-  HIPCHECK(hipEventRecord(ctx->ev1, st));
+  // reset -> [event] -> persistent kernel -> [event] -> energy flush -> copies
+  // to pinned staging: replayed from a cached graph, or launched one by one
+  double* red = reinterpret_cast<double*>(res + kResReduced);
+  if (ctx->tune.graph >= 0) {
+    const void* kfn = cascade_function(o->arith, pot, ctx->tgt.geo.kind, perhist);
+    const RunGraphKey key{kfn, grid, block, lds, ctx->res.p, ctx->hist_all.p, hist_bytes, ctx->stage_res.p,
+                          ctx->stage_hist.p};
+    hipKernelNodeParams zp{}, kp{}, fp{};
+    void* zargs[1] = {&zs};
+    zp.func = const_cast<void*>(zero_spans_function());
+    zp.gridDim = dim3(zero_spans_grid(zs));
+    zp.blockDim = dim3(256);
+    zp.kernelParams = zargs;
+    void* kargs[1] = {&P};
+    kp.func = const_cast<void*>(kfn);
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(block);
+    kp.sharedMemBytes = (unsigned)lds;
+    kp.kernelParams = kargs;
+    const double* we_p = ctx->wave_energy.as<double>();
+    int waves_arg = waves;
+    void* fargs[3] = {&we_p, &waves_arg, &red};
+    fp.func = const_cast<void*>(flush_energy_function());
+    fp.gridDim = dim3(1);
+    fp.blockDim = dim3(64);
+    fp.kernelParams = fargs;
+    RunGraph& G = ctx->graph;
+    if (!G.exec || !(G.key == key)) {
+      G.release();
+      HIPCHECK(hipGraphCreate(&G.graph, 0));
+      hipGraphNode_t prev, n;
+      HIPCHECK(hipGraphAddKernelNode(&G.n_zero, G.graph, nullptr, 0, &zp));
+      HIPCHECK(hipGraphAddEventRecordNode(&n, G.graph, &G.n_zero, 1, ctx->ev0));
+      prev = n;
+      HIPCHECK(hipGraphAddKernelNode(&G.n_kern, G.graph, &prev, 1, &kp));
+      HIPCHECK(hipGraphAddEventRecordNode(&n, G.graph, &G.n_kern, 1, ctx->ev1));
+      prev = n;
+      HIPCHECK(hipGraphAddKernelNode(&G.n_flush, G.graph, &prev, 1, &fp));
+      HIPCHECK(hipGraphAddMemcpyNode1D(&n, G.graph, &G.n_flush, 1, ctx->stage_res.p, ctx->res.p,
+                                       kResWords * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      prev = n;
+      HIPCHECK(hipGraphAddMemcpyNode1D(&n, G.graph, &prev, 1, ctx->stage_hist.p, ctx->hist_all.p, hist_bytes,
+                                       hipMemcpyDeviceToHost));
+      HIPCHECK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+      G.key = key;
+    } else {
+      HIPCHECK(hipGraphExecKernelNodeSetParams(G.exec, G.n_zero, &zp));
+      HIPCHECK(hipGraphExecKernelNodeSetParams(G.exec, G.n_kern, &kp));
+      HIPCHECK(hipGraphExecKernelNodeSetParams(G.exec, G.n_flush, &fp));
+    }
+    HIPCHECK(hipGraphLaunch(G.exec, st));
+  } else {
+    HIPCHECK(zero_spans_launch(zs, st));
+    HIPCHECK(hipEventRecord(ctx->ev0, st));
+    HIPCHECK(cascade_launch(P, o->arith, pot, perhist, grid, block, lds, st));
+    HIPCHECK(hipEventRecord(ctx->ev1, st));
+    HIPCHECK(flush_energy_launch(ctx->wave_energy.as<double>(), waves, red, st));
+    HIPCHECK(hipMemcpyAsync(ctx->stage_res.p, ctx->res.p, kResWords * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(ctx->stage_hist.p, ctx->hist_all.p, hist_bytes, hipMemcpyDeviceToHost, st));
+  }
   HIPCHECK(hipStreamSynchronize(st));
   float ms = 0.0f;
   HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
 
-  unsigned long long ctrl[kNumCtrl], cnt[kNumCounters];
-  double en[kNumEnergy];
-  std::vector<double> we(size_t(waves) * 2 * kNumEnergy);
-  HIPCHECK(hipMemcpy(ctrl, ctx->ctrl.p, sizeof(ctrl), hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(cnt, ctx->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(en, ctx->energy.p, sizeof(en), hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(we.data(), ctx->wave_energy.p, we.size() * sizeof(double), hipMemcpyDeviceToHost));
+  const unsigned long long* sr = static_cast<const unsigned long long*>(ctx->stage_res.p);
+  const unsigned long long* ctrl = sr + kResCtrl;
+  const unsigned long long* cnt = sr + kResCounters;
+  const double* en = reinterpret_cast<const double*>(sr + kResEnergy);
+  const double* rsum = reinterpret_cast<const double*>(sr + kResReduced);
 
   out->n_coll = (long long)cnt[cColl];
   out->n_flight = (long long)cnt[cFlight];
@@ -546,21 +675,9 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   out->disp = (long long)cnt[cDisp];
   out->range_n = (long long)cnt[cRangeN];
   out->surrogate = (long long)cnt[cSurr];
-  // compensated sum of the per-wave energy partials
-  double sums[2] = {0.0, 0.0}, errs[2] = {0.0, 0.0};
-  for (int w = 0; w < waves; ++w)
-    for (int q = 0; q < 2; ++q) {
-      const double* p = &we[size_t(w) * 2 * kNumEnergy + 2 * q];
-      for (int z = 0; z < 2; ++z) {
-        const double x = p[z];
-        const double t = sums[q] + x;
-        errs[q] += (std::fabs(sums[q]) >= std::fabs(x)) ? ((sums[q] - t) + x) : ((x - t) + sums[q]);
-        sums[q] = t;
-      }
-    }
   out->E0 = E0;
-  out->Eel = sums[0] + errs[0];
-  out->Enuc = sums[1] + errs[1];
+  out->Eel = rsum[0] + rsum[1];
+  out->Enuc = rsum[2] + rsum[3];
   out->Eunf = en[eEunf];
   out->Elost = en[eElost];
   out->Edrop = en[eEdrop];
@@ -569,7 +686,9 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   out->range_sum2 = en[eRangeSum2];
   long long* hdst[5] = {out->range_hist, out->lat_y_hist, out->lat_z_hist, out->vac_hist, out->repl_hist};
   for (int i = 0; i < 5; ++i)
-    if (hdst[i]) HIPCHECK(hipMemcpy(hdst[i], ctx->hist[i].p, nb[i] * sizeof(long long), hipMemcpyDeviceToHost));
+    if (hdst[i])
+      std::memcpy(hdst[i], static_cast<const unsigned long long*>(ctx->stage_hist.p) + hoff[i],
+                  nb[i] * sizeof(long long));
   if (perhist) {
     void* pdst[8] = {out->h_vac, out->h_coll, out->h_flights, out->h_x,
                      out->h_y,   out->h_z,    out->h_state,   out->h_Eel};
@@ -592,6 +711,65 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     if (errbits & kErrNewton) m += " newton-cap";
     return fail(MT_E_DEVICE, m);
   }
+  return MT_OK;
+}
+
+int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long long sample, int reps,
+                mt_tuning* best, double* best_rate) {
+  if (!ctx || !pk || !o || !best) return fail(MT_E_ARG, "NULL argument");
+  if (sample < 1 || reps < 1) return fail(MT_E_ARG, "need sample >= 1 and reps >= 1");
+  mt_primaries s = *pk;
+  s.n_hist = std::min(sample, pk->n_hist);
+  if (s.n_hist < 1) return fail(MT_E_ARG, "no histories to sample");
+  mt_run_opts ro = *o;
+  ro.per_history = 0;
+  const mt_tuning keep = ctx->tune;
+  // candidate shapes: wave count per block x LDS stack depth x resident blocks per CU
+  // (0 = as many as fit); queue capacity stays at its default
+  std::vector<mt_tuning> cand;
+  for (int block : {64, 128, 256, 512})
+    for (int cap : {32, 64, 128})
+      for (int bpc : {0, 1, 2}) {
+        mt_tuning t{};
+        t.block = block;
+        t.stack_cap = cap;
+        t.blocks_per_cu = bpc;
+        t.queue_cap = keep.queue_cap;
+        t.lds_hist = keep.lds_hist;
+        t.graph = keep.graph;
+        cand.push_back(t);
+      }
+  double best_r = -1.0;
+  mt_tuning best_t = keep;
+  std::string first_err;
+  for (const mt_tuning& t : cand) {
+    ctx->tune = t;
+    double r = 0.0;
+    bool ok = true;
+    for (int i = 0; i < reps && ok; ++i) {
+      mt_tallies out{};
+      const int rc = mt_run_cascades(ctx, &s, &ro, &out);
+      if (rc == MT_E_LIMIT) {  // shape does not fit (LDS, occupancy): skip it
+        ok = false;
+      } else if (rc != MT_OK) {
+        ctx->tune = keep;
+        return rc;
+      } else if (out.kernel_ms > 0.0) {
+        r = std::max(r, double(out.n_coll) / (1e-3 * out.kernel_ms));
+      }
+    }
+    if (ok && r > best_r) {
+      best_r = r;
+      best_t = t;
+    }
+  }
+  if (best_r < 0.0) {
+    ctx->tune = keep;
+    return fail(MT_E_LIMIT, "no candidate launch shape fits");
+  }
+  ctx->tune = best_t;
+  *best = best_t;
+  if (best_rate) *best_rate = best_r;
   return MT_OK;
 }
 

@@ -91,6 +91,14 @@ struct Params {
   MlpDev mlp;
 };
 
+// device spans zeroed before a run
+constexpr int kMaxSpans = 16;
+struct ZeroSpans {
+  void* p[kMaxSpans];
+  size_t bytes[kMaxSpans];
+  int n;
+};
+
 // LDS bytes of one wave's recoil stack entry
 constexpr int kStackEntryBytes = 7 * 8 + 8 + 4 + 4 + 4 + 4;  // doubles, key, meta, tag, hist, pad
 

@@ -696,6 +696,64 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
 
 int cascade_stack_bytes(int cap) { return stack_bytes(cap); }
 
+// ------------------------------------------------------------ reset and flush
+// Zero the spans of a run (queue ready words, tallies, control words): a
+// kernel rather than memset nodes, so the whole sequence can be replayed from
+// a graph.  Spans are hipMalloc'd (256-byte aligned) with sizes in bytes that
+// are multiples of 4.
+__global__ void __launch_bounds__(256) zero_spans(ZeroSpans z) {
+  const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int s = 0; s < z.n; ++s) {
+    const size_t words = z.bytes[s] >> 2;
+    uint4* p4 = (uint4*)z.p[s];
+    const size_t n4 = words >> 2;
+    for (size_t i = gid; i < n4; i += stride) p4[i] = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t* p1 = (uint32_t*)z.p[s];
+    for (size_t i = 4 * n4 + gid; i < words; i += stride) p1[i] = 0u;
+  }
+}
+
+// Compensated (Neumaier) sum of the per-wave energy partials: lane l sums
+// waves l, l + 64, ...; lane 0 then folds the 64 partial pairs in lane order.
+// out: Eel sum, Eel compensation, Enuc sum, Enuc compensation.
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+  s = t;
+}
+
+__global__ void __launch_bounds__(64) flush_energy(const double* we, int waves, double* out) {
+  __shared__ double part[4][64];
+  const int l = threadIdx.x;
+  double s0 = 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
+  for (int w = l; w < waves; w += 64) {
+    const double* p = we + (size_t)w * 2 * kNumEnergy;
+    neumaier(s0, c0, p[0]);
+    neumaier(s0, c0, p[1]);
+    neumaier(s1, c1, p[2]);
+    neumaier(s1, c1, p[3]);
+  }
+  part[0][l] = s0;
+  part[1][l] = c0;
+  part[2][l] = s1;
+  part[3][l] = c1;
+  __syncthreads();
+  if (l == 0) {
+    double a0 = 0.0, e0 = 0.0, a1 = 0.0, e1 = 0.0;
+    for (int i = 0; i < 64; ++i) {
+      neumaier(a0, e0, part[0][i]);
+      e0 += part[1][i];
+      neumaier(a1, e1, part[2][i]);
+      e1 += part[3][i];
+    }
+    out[0] = a0;
+    out[1] = e0;
+    out[2] = a1;
+    out[3] = e1;
+  }
+}
+
 }  // namespace mt
 
 // ---------------------------------------------------------------- launchers
@@ -740,6 +798,30 @@ hipError_t cascade_launch(const Params& P, int arith, int potential, bool perhis
                           hipStream_t stream) {
   CascadeKernel k = pick(arith, potential, P.geo.kind, perhist);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, stream, P);
+  return hipGetLastError();
+}
+
+const void* cascade_function(int arith, int potential, int geo, bool perhist) {
+  return (const void*)pick(arith, potential, geo, perhist);
+}
+
+int zero_spans_grid(const ZeroSpans& z) {
+  size_t most = 0;
+  for (int s = 0; s < z.n; ++s) most = z.bytes[s] > most ? z.bytes[s] : most;
+  const size_t blocks = (most / 16 + 255) / 256;
+  return (int)(blocks < 1 ? 1 : blocks > 4096 ? 4096 : blocks);
+}
+
+const void* zero_spans_function() { return (const void*)zero_spans; }
+const void* flush_energy_function() { return (const void*)flush_energy; }
+
+hipError_t zero_spans_launch(const ZeroSpans& z, hipStream_t st) {
+  hipLaunchKernelGGL(zero_spans, dim3(zero_spans_grid(z)), dim3(256), 0, st, z);
+  return hipGetLastError();
+}
+
+hipError_t flush_energy_launch(const double* we, int waves, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(flush_energy, dim3(1), dim3(64), 0, st, we, waves, out);
   return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
 # every symbol include/mytrim_gpu.h declares
 EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt_ctx_set_tables",
            "mt_default_tables",
-           "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_trim_flight",
+           "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_autotune", "mt_trim_flight",
            "mt_stopping_batch", "mt_scatter_batch", "mt_surrogate_train", "mt_surrogate_install",
            "mt_surrogate_eval", "mt_surrogate_grad"]
 
@@ -72,7 +72,7 @@ class Tallies(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("stack_cap", C.c_int),
-                ("queue_cap", C.c_longlong), ("lds_hist", C.c_int)]
+                ("queue_cap", C.c_longlong), ("lds_hist", C.c_int), ("graph", C.c_int)]
 
 
 class SurrogateOpts(C.Structure):
@@ -118,6 +118,8 @@ def lib():
         L.mt_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.mt_run_cascades.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
                                       C.POINTER(Tallies)]
+        L.mt_autotune.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts), C.c_longlong,
+                                  C.c_int, C.POINTER(Tuning), dp]
         L.mt_trim_flight.argtypes = [C.c_void_p, C.POINTER(Ion), dp, C.c_longlong, lp,
                                      C.POINTER(Event), C.c_int, ip]
         L.mt_stopping_batch.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, dp,
@@ -167,8 +169,8 @@ class Engine:
         except Exception:
             pass
 
-    def tune(self, block=0, blocks_per_cu=0, stack_cap=0, queue_cap=0, lds_hist=0):
-        t = Tuning(block, blocks_per_cu, stack_cap, queue_cap, lds_hist)
+    def tune(self, block=0, blocks_per_cu=0, stack_cap=0, queue_cap=0, lds_hist=0, graph=0):
+        t = Tuning(block, blocks_per_cu, stack_cap, queue_cap, lds_hist, graph)
         _check(lib().mt_ctx_set_tuning(self.h, C.byref(t)), "mt_ctx_set_tuning")
 
     def set_target(self, prob: Problem):
@@ -196,22 +198,36 @@ class Engine:
         del keep
         self.problem = prob
 
-    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64) -> dict:
-        """Full cascades for every history of `prob` (sets the target if needed)."""
+    def _launch_args(self, prob: Problem, per_history: bool, arith: int):
         if self.problem is not prob:
             self.set_target(prob)
         p = prob.primaries
-        pZ = np.ascontiguousarray(p.Z, np.int32)
-        pm = np.ascontiguousarray(p.m, np.float64)
-        pE = np.ascontiguousarray(p.E, np.float64)
-        ppos = np.ascontiguousarray(p.pos, np.float64)
-        pdir = np.ascontiguousarray(p.dir, np.float64)
+        keep = [np.ascontiguousarray(p.Z, np.int32), np.ascontiguousarray(p.m, np.float64),
+                np.ascontiguousarray(p.E, np.float64), np.ascontiguousarray(p.pos, np.float64),
+                np.ascontiguousarray(p.dir, np.float64)]
+        pZ, pm, pE, ppos, pdir = keep
         pk = Primaries(prob.n_hist, prob.hist0, 1 if p.uniform else 0, _p(pZ, ip), _p(pm, dp),
                        _p(pE, dp), _p(ppos, dp), _p(pdir, dp), prob.Ef, prob.seed & 0xFFFFFFFF)
         o = RunOpts(prob.follow_max_gen, int(prob.kp), arith, prob.range_bins.x0,
                     prob.range_bins.bw, prob.range_bins.nb, prob.lat_y0, prob.lat_z0, prob.lat_bw,
                     prob.lat_nb, prob.vac_bins.x0, prob.vac_bins.bw, prob.vac_bins.nb,
                     int(per_history))
+        return pk, o, keep
+
+    def autotune(self, prob: Problem, sample: int = 2000, reps: int = 1, arith: int = MT_FP64):
+        """Pick the fastest launch shape on the first `sample` histories of `prob`
+        (kept in the context for later runs); returns (Tuning, collisions/s)."""
+        pk, o, keep = self._launch_args(prob, False, arith)
+        best = Tuning()
+        rate = C.c_double()
+        _check(lib().mt_autotune(self.h, C.byref(pk), C.byref(o), int(sample), int(reps),
+                                 C.byref(best), C.byref(rate)), "mt_autotune")
+        del keep
+        return best, rate.value
+
+    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64) -> dict:
+        """Full cascades for every history of `prob` (sets the target if needed)."""
+        pk, o, keep = self._launch_args(prob, per_history, arith)
         out = Tallies()
         hists = {"range_hist": np.zeros(prob.range_bins.nb + 2, np.int64),
                  "lat_y_hist": np.zeros(prob.lat_nb + 2, np.int64),
@@ -236,6 +252,7 @@ class Engine:
                    queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate)
         res.update(hists)
         res.update(per)
+        del keep
         return res
 
     def stopping(self, Z1: int, m1: float, mat: int, E: np.ndarray) -> np.ndarray:

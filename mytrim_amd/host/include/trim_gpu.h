@@ -31,6 +31,9 @@ class TrimGPU : public TrimBase {
     Real vac_x0 = 0.0, vac_bin = 1.0;
     int vac_bins = 10000;
     bool per_history = false;
+    // FP64 everywhere; FP32 physics with an FP64 ledger and positions; FP32
+    // with the learned scattering surrogate (installed on the device context)
+    enum Arith { FP64 = 0, FP32 = 1, FP32_SURROGATE = 2 } arith = FP64;
   };
 
   struct Tallies {
@@ -52,6 +55,9 @@ class TrimGPU : public TrimBase {
   // one primary per history
   Tallies runCascades(const std::vector<IonBase*>& pkas, uint32_t seed, const Options& opt,
                       long long hist0 = 0);
+  // time candidate launch shapes on `sample` copies of `pka`, keep the fastest
+  // for later runCascades calls; returns its collisions per kernel second
+  double autotune(const IonBase& pka, long long sample, const Options& opt, int reps = 1);
 };
 
 }  // namespace MyTRIM_NS

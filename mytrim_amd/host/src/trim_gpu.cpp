@@ -10,11 +10,17 @@ namespace MyTRIM_NS {
 
 namespace {
 
-TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<int>& Z,
-                     const std::vector<double>& m, const std::vector<double>& E, const std::vector<double>& pos,
-                     const std::vector<double>& dir, double Ef, uint32_t seed, long long hist0,
-                     const TrimGPU::Options& o) {
-  mt_primaries pk = {};
+struct Launch {
+  mt_primaries pk;
+  mt_run_opts ro;
+};
+
+Launch launch_args(long long n, bool uniform, const std::vector<int>& Z, const std::vector<double>& m,
+                   const std::vector<double>& E, const std::vector<double>& pos, const std::vector<double>& dir,
+                   double Ef, uint32_t seed, long long hist0, const TrimGPU::Options& o) {
+  Launch L;
+  mt_primaries& pk = L.pk;
+  pk = {};
   pk.n_hist = n;
   pk.hist0 = hist0;
   pk.pka_uniform = uniform ? 1 : 0;
@@ -25,10 +31,11 @@ TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<i
   pk.dir = dir.data();
   pk.Ef = Ef;
   pk.seed = seed;
-  mt_run_opts ro = {};
+  mt_run_opts& ro = L.ro;
+  ro = {};
   ro.follow_max_gen = o.follow_max_gen;
   ro.kp = o.kinchin_pease ? 1 : 0;
-  ro.arith = MT_FP64;
+  ro.arith = int(o.arith);
   ro.range_x0 = o.range_x0;
   ro.range_bw = o.range_bin;
   ro.range_nb = o.range_bins;
@@ -40,6 +47,14 @@ TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<i
   ro.vac_bw = o.vac_bin;
   ro.vac_nb = o.vac_bins;
   ro.per_history = o.per_history ? 1 : 0;
+  return L;
+}
+
+TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<int>& Z,
+                     const std::vector<double>& m, const std::vector<double>& E, const std::vector<double>& pos,
+                     const std::vector<double>& dir, double Ef, uint32_t seed, long long hist0,
+                     const TrimGPU::Options& o) {
+  Launch L = launch_args(n, uniform, Z, m, E, pos, dir, Ef, seed, hist0, o);
   TrimGPU::Tallies t;
   t.range_hist.assign(size_t(o.range_bins + 2), 0);
   t.lat_y_hist.assign(size_t(o.lat_bins + 2), 0);
@@ -70,7 +85,7 @@ TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<i
     out.h_z = t.h_z.data();
     out.h_Eel = t.h_Eel.data();
   }
-  device::check(mt_run_cascades(ctx, &pk, &ro, &out), "mt_run_cascades");
+  device::check(mt_run_cascades(ctx, &L.pk, &L.ro, &out), "mt_run_cascades");
   t.collisions = out.n_coll;
   t.flights = out.n_flight;
   t.vacancies = out.vac;
@@ -127,6 +142,16 @@ TrimGPU::Tallies TrimGPU::runCascades(const std::vector<IonBase*>& pkas, uint32_
   _simconf->EelTotal += t.Eel;
   _simconf->EnucTotal += t.Enuc;
   return t;
+}
+
+double TrimGPU::autotune(const IonBase& p, long long sample, const Options& o, int reps) {
+  std::vector<int> Z{p._Z};
+  std::vector<double> m{p._m}, E{p._E}, pos{p._pos(0), p._pos(1), p._pos(2)}, dir{p._dir(0), p._dir(1), p._dir(2)};
+  Launch L = launch_args(sample, true, Z, m, E, pos, dir, p._Ef, 1u, 0, o);
+  mt_tuning best = {};
+  double rate = 0.0;
+  device::check(mt_autotune(device(), &L.pk, &L.ro, sample, reps, &best, &rate), "mt_autotune");
+  return rate;
 }
 
 }  // namespace MyTRIM_NS

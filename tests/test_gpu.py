@@ -278,3 +278,60 @@ def test_fp32_tallies_within_one_percent_of_fp64(eng, name, prob):
     assert b["Eel"] / b["E0"] == pytest.approx(a["Eel"] / a["E0"], rel=0.01)
     tot = b["Eel"] + b["Enuc"] + b["Eunf"] + b["Elost"] + b["Edrop"] + b["Evac"]
     assert tot == pytest.approx(b["E0"], rel=1e-11)
+
+
+# ---------------------------------------------------------------- launch machinery
+def test_graph_replays_match_direct_launches(eng):
+    """The cached run graph (reset kernel, persistent kernel, flush kernel,
+    copies to pinned memory), replayed several times with direct launches and
+    host work in between, gives the direct launch's tallies every time."""
+    prob = configs.c1_he_fe(400, seed=77)
+    eng.tune(graph=-1)
+    ref = eng.run(prob, per_history=True)
+    try:
+        for rep in range(3):
+            eng.tune(graph=0)
+            g = eng.run(prob, per_history=True)
+            eng.tune(graph=-1)
+            d = eng.run(prob.with_(n_hist=50, hist0=1000 + rep), per_history=True)   # eager work between
+            assert d["n_coll"] > 0
+            for k in ["n_coll", "n_flight", "vac", "repl", "subst", "inter", "lost", "disp", "range_n"]:
+                assert g[k] == ref[k], (rep, k)
+            for k in ["range_hist", "lat_y_hist", "lat_z_hist", "vac_hist", "repl_hist", "h_vac", "h_coll",
+                      "h_flights", "h_state"]:
+                assert np.array_equal(g[k], ref[k]), (rep, k)
+            for k in ["Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac", "range_sum"]:
+                assert g[k] == pytest.approx(ref[k], rel=1e-12, abs=1e-9), (rep, k)
+    finally:
+        eng.tune()
+
+
+def test_graph_survives_shape_changes(eng):
+    """Histogram sizes and batch sizes change between calls: the graph is
+    rebuilt or updated and the tallies stay those of a direct launch."""
+    a = configs.c1_he_fe(120, seed=5)
+    b = configs.c5_c_w(30, seed=5)
+    out = {}
+    try:
+        for mode in (-1, 0):
+            eng.tune(graph=mode)
+            out[mode] = [eng.run(p, per_history=True) for p in (a, b, a.with_(n_hist=60))]
+    finally:
+        eng.tune()
+    for x, y in zip(out[-1], out[0]):
+        assert x["n_coll"] == y["n_coll"] and x["vac"] == y["vac"]
+        assert np.array_equal(x["range_hist"], y["range_hist"])
+        assert np.array_equal(x["h_coll"], y["h_coll"])
+
+
+def test_autotune_picks_a_working_shape(eng):
+    prob = configs.c1_he_fe(3000, seed=3)
+    best, rate = eng.autotune(prob, sample=500)
+    print("autotune: block %d, blocks/CU %d, stack %d -> %.3g coll/s" %
+          (best.block, best.blocks_per_cu, best.stack_cap, rate))
+    assert best.block in (64, 128, 256, 512) and best.stack_cap in (32, 64, 128)
+    assert rate > 0
+    r = eng.run(prob.with_(n_hist=300), per_history=True)
+    eng.tune()
+    s = eng.run(prob.with_(n_hist=300), per_history=True)
+    assert np.array_equal(r["h_coll"], s["h_coll"]) and r["vac"] == s["vac"]
