@@ -3,16 +3,21 @@
 
 Workload: 500 keV Cu ions into a 2 um Cu cube (8.96 g/cm3, Ed 30 eV, Eb 3 eV),
 1e6 histories, FP64, every recoil followed (mytrim_amd.configs.c2_cu_cu).  A
-step is one batch of histories; the K timed steps together cover all
---histories histories (distinct random streams per history), the W warm-up
-steps run on extra histories.
+step is one batch of histories (one persistent-kernel launch); the K timed
+steps together cover all --histories histories (distinct random streams per
+history), the W warm-up steps run on extra histories.  The launch shape is
+autotuned on a sample first (outside the timed region).
 
 Prints ONE JSON line: collisions/s (value) and histories/s of the whole job,
-the roofline of the cascade kernel (FP64 VALU work per collision, frozen by
-counting the operations of the CPU restatement, over the kernel time measured
-with HIP events on its stream), and the CPU baseline: the reference's
-algorithm restated in C++ (std::mt19937 draws), timed on this host's cores on
-a bounded sample of the same workload.
+the same workload in FP32 and FP32 + learned scattering surrogate on a
+sub-batch, the roofline of the cascade kernel (FP64 work per collision,
+frozen by counting the operations of the CPU restatement, over the kernel time
+measured with HIP events on its stream), and the CPU baseline: the
+reference's algorithm restated in C++ (std::mt19937 draws), timed on this
+host's cores on a bounded sample of the same workload.
+
+--copy-calibration runs only a device-to-device copy of known size (the
+reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes).
 """
 import argparse
 import json
@@ -23,7 +28,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP64_VALU_PEAK_TFLOPS = 78.6      # MI355X dense FP64 vector
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix alike)
+FP32_VALU_PEAK_TFLOPS = 157.3
 METRIC = "collisions/sec + ion histories/sec on 1 MI355X (plain FP64 vs FP32+surrogate)"
 
 
@@ -57,6 +63,39 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def copy_calibration(mib: int, reps: int):
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}), flush=True)
+
+
+def train_surrogate(eng, gpu, n_train: int, epochs: int):
+    opts = gpu.Engine.surrogate_opts(n_train=n_train, epochs=epochs, bf16=True)
+    t0 = time.perf_counter()
+    params, loss = eng.surrogate_train(opts, gpu.Engine.surrogate_init(1))
+    t_train = time.perf_counter() - t0
+    eng.surrogate_install(opts, params)
+    return {"train_s": t_train, "final_loss": float(loss[-1]), "n_train": n_train, "epochs": epochs,
+            "domain": {"eps": [opts.eps_min, opts.eps_max], "b": [0.0, opts.b_max]}}
+
+
+def timed(eng, prob, arith, n_hist, steps):
+    bounds = [n_hist * i // steps for i in range(steps + 1)]
+    coll, kms, res = 0, 0.0, []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r = eng.run(prob.with_(hist0=bounds[i], n_hist=bounds[i + 1] - bounds[i]), arith=arith)
+        coll += r["n_coll"]
+        kms += r["kernel_ms"]
+        res.append(r)
+    return coll, time.perf_counter() - t0, kms, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,45 +103,72 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=int, default=1000000)
     ap.add_argument("--config", default="c2")
+    ap.add_argument("--autotune-sample", type=int, default=1000, help="0 = default launch shape")
+    ap.add_argument("--variant-histories", type=int, default=100000,
+                    help="histories for the FP32 and FP32+surrogate lines (0 = skip)")
+    ap.add_argument("--surrogate-train", type=int, default=10_000_000)
+    ap.add_argument("--surrogate-epochs", type=int, default=8)
     ap.add_argument("--cpu-histories", type=int, default=0, help="0 = size to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X; the "
-              "histories would be split across GPUs as described in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU: this build runs on a single MI355X. Histories are "
+              "independent, so N GPUs would each take a contiguous range of history ids; "
+              "no multi-GPU run is made here.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 5)
+        return
 
     from mytrim_amd import configs, gpu
     build = {"c1": configs.c1_he_fe, "c2": configs.c2_cu_cu, "c3": configs.c3_ag_layers,
-             "c5": configs.c5_c_w}[args.config]
+             "c4": configs.c4_mixed_uo2, "c5": configs.c5_c_w}[args.config]
     prob = build(args.histories)
     eng = gpu.Engine(0)
     eng.set_target(prob)
-    # warm-up on histories beyond the timed range
+    tuning = None
+    if args.autotune_sample > 0:
+        best, rate = eng.autotune(prob.with_(hist0=2 * args.histories, n_hist=args.autotune_sample),
+                                  sample=args.autotune_sample)
+        tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
+                  "sample_histories": args.autotune_sample, "sample_rate": rate}
     wsize = max(1, min(20000, args.histories // max(args.steps, 1)))
     for i in range(args.warmup):
         eng.run(prob.with_(hist0=args.histories + i * wsize, n_hist=wsize))
-    # timed steps: K batches covering [0, histories)
-    bounds = [args.histories * i // args.steps for i in range(args.steps + 1)]
-    coll = 0
-    kernel_ms = 0.0
-    steps = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        r = eng.run(prob.with_(hist0=bounds[i], n_hist=bounds[i + 1] - bounds[i]))
-        coll += r["n_coll"]
-        kernel_ms += r["kernel_ms"]
-        steps.append(r)
-    wall = time.perf_counter() - t0
+    coll, wall, kernel_ms, steps = timed(eng, prob, gpu.MT_FP64, args.histories, args.steps)
+
+    variants = None
+    if args.variant_histories > 0:
+        nv = min(args.variant_histories, args.histories)
+        c64, w64, k64, _ = timed(eng, prob, gpu.MT_FP64, nv, 1)
+        c32, w32, k32, _ = timed(eng, prob, gpu.MT_FP32, nv, 1)
+        sur = train_surrogate(eng, gpu, args.surrogate_train, args.surrogate_epochs)
+        cs, ws, ks, rs = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
+        variants = {
+            "histories": nv,
+            "fp64": {"collisions_per_s": c64 / w64, "kernel_ms": k64},
+            "fp32": {"collisions_per_s": c32 / w32, "kernel_ms": k32, "speedup_vs_fp64": (c32 / w32) / (c64 / w64)},
+            "fp32_surrogate": {"collisions_per_s": cs / ws, "kernel_ms": ks,
+                               "speedup_vs_fp64": (cs / ws) / (c64 / w64),
+                               "surrogate_share": rs[0]["surrogate"] / max(rs[0]["n_coll"], 1),
+                               "training": sur},
+        }
     eng.close()
 
     w = work_per_collision(args.config)
-    achieved = coll * w["flop_eq"] / (kernel_ms * 1e-3) / 1e12
+    n_launch = len(steps)
+    avg_ms = kernel_ms / n_launch
+    flop_per_launch = coll * w["flop_eq"] / n_launch
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_collision")
+            bpc = json.load(f).get("bytes_per_collision")
+        if bpc:
+            traffic = bpc * coll / n_launch
+    s0 = steps[0]
     out = {
         "metric": METRIC,
         "value": coll / wall,
@@ -117,16 +183,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": prob.name, "histories": args.histories, "ion": "Cu 63.546 u, 500 keV",
-                   "target": "Cu 8.96 g/cm3, 2 um cube, Ed 30 eV, Eb 3 eV",
+        "config": {"workload": prob.name, "histories": args.histories,
                    "cascades": "full (every recoil followed)", "arith": "FP64",
                    "collisions": coll, "collisions_per_history": coll / args.histories,
-                   "grid": steps[0]["grid"], "block": steps[0]["block"], "waves": steps[0]["waves"]},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-                     "traffic": traffic,
-                     "kernel": "cascade_fp64", "kernel_ms": kernel_ms,
-                     "flop_eq_per_collision": w["flop_eq"]},
+                   "grid": s0["grid"], "block": s0["block"], "waves": s0["waves"],
+                   "queue_peak": max(r["queue_peak"] for r in steps), "tuning": tuning},
+        "roofline": {"bound": "mfma", "pipe": "FP64 VALU (the FP64 vector and matrix peaks are equal)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "cascade_kernel<0,0,0,false>", "launches": n_launch,
+                     "avg_kernel_ms": avg_ms, "flop_eq_per_collision": w["flop_eq"],
+                     "flop_eq_per_launch": flop_per_launch},
+        "variants": variants,
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
