@@ -170,7 +170,7 @@ struct mt_ctx {
   mt_tuning tune{};
   DBuf tables, sh, cl, cx, cy, cz, cr;
   DBuf pka_sp, pka_E, pka_pos, pka_dir;
-  DBuf res, wave_energy, hist_all;
+  DBuf res, wave_energy, hist_all, h_out;
   HBuf stage_res, stage_hist;
   RunGraph graph;
   DBuf per[8];
@@ -189,7 +189,7 @@ struct mt_ctx {
   ~mt_ctx() {
     graph.release();
     DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
-                   &wave_energy, &hist_all, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e};
+                   &wave_energy, &hist_all, &h_out, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e};
     for (DBuf* b : all) b->release();
     stage_res.release();
     stage_hist.release();
@@ -470,10 +470,20 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   const int waves = grid * nwb;
   if (waves > 0xffff) return fail(MT_E_LIMIT, "too many waves for the idle counter");
 
-  // queue
-  uint64_t qcap = ctx->tune.queue_cap ? uint64_t(ctx->tune.queue_cap)
-                                      : std::min<uint64_t>(uint64_t(1) << 26,
-                                                           std::max<uint64_t>(uint64_t(1) << 16, next_pow2(uint64_t(n_hist) * 64)));
+  // flights per cascade before any has completed: the Kinchin-Pease count
+  // E / (2 Ed) of the most energetic primary against the lowest threshold
+  double ed_min = 1e300;
+  for (const HostMaterial& m : ctx->tgt.mat)
+    for (const HostElement& e : m.el) ed_min = std::min(ed_min, e.Edisp);
+  double e_max = 0.0;
+  for (long long i = 0; i < n_entries; ++i) e_max = std::max(e_max, pk->E[i]);
+  const double f_init = 1.0 + e_max / (2.0 * std::max(ed_min, 1.0));
+
+  // queue: room for every flight of the batch up to 2^27 entries (80 B each)
+  const double want = std::max(double(n_hist), 1.0) * f_init;
+  uint64_t qcap = ctx->tune.queue_cap
+                      ? uint64_t(ctx->tune.queue_cap)
+                      : next_pow2(uint64_t(std::min(std::max(want, 65536.0), double(uint64_t(1) << 27))));
   if (ctx->qcap != qcap) {
     const size_t sz[12] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 8, 4};
     for (int i = 0; i < 12; ++i) HIPCHECK(ctx->q[i].ensure(qcap * sz[i]));
@@ -507,6 +517,8 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   span(ctx->res.p, kResWords * sizeof(unsigned long long));
   span(ctx->hist_all.p, hist_bytes);
   span(ctx->wave_energy.p, size_t(waves) * 2 * kNumEnergy * sizeof(double));
+  HIPCHECK(ctx->h_out.ensure(nh * sizeof(int)));
+  span(ctx->h_out.p, nh * sizeof(int));
   if (perhist) {
     for (int i = 0; i < 8; ++i) {
       HIPCHECK(ctx->per[i].ensure(nh * per_sz[i]));
@@ -552,6 +564,10 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   Q.mask = qcap - 1;
   Q.log2cap = log2cap;
   P.admit_limit = (long long)(qcap / 4);
+  P.budget = double(qcap) / 2.0;
+  P.f_init = f_init;
+  P.c_floor = waves;
+  P.h_out = ctx->h_out.as<int>();
   P.max_ic = 10000000;
   P.max_wait_ticks = (unsigned long long)(60.0 * ctx->clock_khz * 1000.0);
   P.counters = res + kResCounters;

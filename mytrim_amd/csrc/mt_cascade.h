@@ -29,7 +29,11 @@ enum Ctrl {
   kError = 4,      // error bits
   kQueueOps = 5,   // wave-level queue transfers
   kQueuePeak = 6,  // largest tail - head seen at a push
-  kNumCtrl = 8
+  // cascade bookkeeping, on cache lines (128 B) of their own
+  kLive = 16,      // admitted cascades not yet complete
+  kCascDone = 17,  // completed cascades
+  kFlightsEnded = 32,
+  kNumCtrl = 48
 };
 
 enum ErrBits {
@@ -66,6 +70,13 @@ struct Params {
   unsigned long long* ctrl;
   Queue q;
   long long admit_limit;        // no new primaries while the queue holds more
+  // cascade budget: admitted-but-incomplete cascades times the flights per
+  // cascade (f_init until 64 cascades completed, then flights ended per
+  // completed cascade, an overestimate) stays below `budget` queue entries;
+  // at least c_floor cascades may always be live
+  double budget, f_init;
+  long long c_floor;
+  int* h_out;                   // per history: flights queued or running
   long long max_ic;             // collisions per flight before kErrRunaway
   unsigned long long max_wait_ticks;  // wall-clock ticks before kErrTimeout
   // tallies

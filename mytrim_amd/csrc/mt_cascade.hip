@@ -206,6 +206,8 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
   // wave-uniform state
   int scount = 0;          // entries in this wave's LDS stack
   bool wave_idle = false;  // registered idle
+  int backoff = 1;         // idle sleep rounds
+  unsigned long long fe_pending = 0;  // ended flights not yet added to kFlightsEnded
   unsigned long long idle_since = 0;
   unsigned long long c_coll = 0, c_flight = 0, c_vac = 0, c_repl = 0, c_subst = 0, c_inter = 0,
                      c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0, c_surr = 0;
@@ -255,7 +257,17 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
         const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
         const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
         const bool may_pop = t0 > h0;
-        const bool may_pka = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
+        bool may_pka = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
+        long long n_adm = 0;
+        if (may_pka) {  // cascade budget (see Params)
+          const uint64_t live = ld_relaxed(&ctrl[kLive]);
+          const uint64_t done = ld_relaxed(&ctrl[kCascDone]);
+          const double fhat =
+              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : P.f_init;
+          const long long cmax = max((long long)(P.budget / fmax(fhat, 1.0)), P.c_floor);
+          n_adm = min((long long)n_idle, cmax - (long long)live);
+          may_pka = n_adm > 0 && (long long)(t0 - h0) < P.admit_limit;
+        }
         if (wave_idle && (may_pop || may_pka)) {
           if (lane == 0) atomicAdd(&ctrl[kState], (1ull << 32) - 1ull);  // epoch += 1, idle -= 1
           wave_idle = false;
@@ -297,6 +309,10 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
                   mine = false;
                   break;
                 }
+                if (ld_relaxed(&ctrl[kError]) != 0ull) {  // another wave aborted the run
+                  mine = false;
+                  break;
+                }
               }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -327,15 +343,17 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
           }
         }
         if (n_idle > 0 && may_pka) {
-          const uint64_t h = ld_relaxed(&ctrl[kHead]);
-          const uint64_t t = ld_relaxed(&ctrl[kTail]);
-          if ((long long)(t - h) < P.admit_limit) {
+          n_adm = min(n_adm, (long long)n_idle);
+          {
             uint64_t first = 0;
-            if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_idle);
+            if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_adm);
             first = bcast64(first, 0);
+            const long long got = min(n_adm, max(0ll, P.n_hist - (long long)first));
+            if (lane == 0 && got > 0) atomicAdd(&ctrl[kLive], (unsigned long long)got);
             const int r = lane_rank(idle_m);
             const long long j = (long long)first + r;
-            if (!active && r < n_idle && j < P.n_hist) {
+            if (!active && r < got) {
+              __hip_atomic_store(&P.h_out[j], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               const long long e = P.pka_uniform ? 0 : j;
               F.px = P.pka_pos[3 * e + 0];
               F.py = P.pka_pos[3 * e + 1];
@@ -356,6 +374,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
           }
         }
       }
+      if (__ballot(active) != 0ull) backoff = 1;
       if (__ballot(active) == 0ull) {
         // nothing to run: idle protocol and termination test
         if (!wave_idle) {
@@ -375,7 +394,10 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
           err |= kErrTimeout;
           break;
         }
-        __builtin_amdgcn_s_sleep(8);
+        if (ld_relaxed(&ctrl[kError]) != 0ull) break;  // a wave aborted the run
+        // back off (up to ~30 us) so idle waves do not crowd the control words
+        for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(127);
+        backoff = min(2 * backoff, 8);
         continue;
       }
     }
@@ -560,6 +582,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
       const int r = lane_rank(push_m);
       const uint32_t meta = (uint32_t)rsp | ((uint32_t)(F.gen + 1) << 8);
       const uint64_t rkey = child_key(F.key, (uint32_t)F.ic);
+      if (push) atomicAdd(&P.h_out[F.hist - (uint32_t)P.hist0], 1);
       if (push && r < room) {
         const int j = scount + r;
         S.px[j] = rx;
@@ -589,7 +612,10 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
         // past base: the occupancy is a signed difference
         const long long fill = (long long)(base + (uint64_t)n_over) - (long long)head;
         if (lane == 0 && fill > 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
-        if (fill > (long long)P.q.mask) err |= kErrQueueFull;  // caller discards the run
+        if (fill > (long long)P.q.mask) {  // abort: every wave leaves, the caller reports the error
+          err |= kErrQueueFull;
+          if (lane == 0) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
+        }
         const bool over = push && r >= room;
         uint64_t slot = 0;
         uint32_t lap = 0;
@@ -619,6 +645,17 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
     const uint64_t fin_m = __ballot(fin);
     if (fin_m) {
       c_flight += __popcll(fin_m);
+      bool closed = false;  // this flight was its cascade's last
+      if (fin) closed = atomicAdd(&P.h_out[F.hist - (uint32_t)P.hist0], -1) == 1;
+      const uint64_t closed_m = __ballot(closed);
+      fe_pending += __popcll(fin_m);
+      if (lane == 0) {
+        if (fe_pending >= 256ull) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
+        if (closed_m) {
+          atomicAdd(&ctrl[kCascDone], (unsigned long long)__popcll(closed_m));
+          atomicAdd(&ctrl[kLive], (unsigned long long)(-(long long)__popcll(closed_m)));
+        }
+      }
       c_vacuum += __popcll(__ballot(fin && vacuum));
       if (fin) {
         if (vacuum) atomicAdd(&P.energy[eEvac], F.E);
@@ -646,6 +683,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
         }
         active = false;
       }
+      if (fe_pending >= 256ull) fe_pending = 0;
       c_rangen += __popcll(__ballot(fin && F.gen == 0 && !vacuum && state != 4));
     }
   }
@@ -677,6 +715,7 @@ __global__ void __launch_bounds__(256) cascade_kernel(Params P) {
     atomicAdd(&C[cRangeN], c_rangen);
     atomicAdd(&C[cSurr], c_surr);
     atomicAdd(&ctrl[kQueueOps], c_qops);
+    if (fe_pending) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
     const int gw = blockIdx.x * nwb + wid;
     double* we = P.wave_energy + (size_t)gw * (2 * kNumEnergy);
     we[2 * eEel] = eel_s;

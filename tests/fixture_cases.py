@@ -37,7 +37,7 @@ def tally_cases(quick=False):
     s = 0.1 if quick else 1.0
     return {
         "c1": (configs.c1_he_fe(2000), 1),
-        "c2": (configs.c2_cu_cu(int(800 * s)), 8),
+        "c2": (configs.c2_cu_cu(int(4000 * s)), 8),
         "c3": (configs.c3_ag_layers(int(800 * s)), 8),
         "c4": (c4_uniform(4000.0, int(400 * s)), 8),
         "c5": (configs.c5_c_w(int(2000 * s)), 8),

@@ -100,8 +100,10 @@ def parse_tally(txt):
     return res
 
 
-def tallies(quick):
+def tallies(quick, only=None):
     for name, (prob, nproc) in tally_cases(quick).items():
+        if only and name not in only:
+            continue
         chunk = (prob.n_hist + nproc - 1) // nproc
         parts = []
         specs = []
@@ -141,6 +143,10 @@ def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     quick = "--quick" in sys.argv
+    only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--tally=")]
+    if only:  # regenerate the named tally fixtures only
+        tallies(quick, only)
+        return
     stopping_grid()
     average_table()
     traces()

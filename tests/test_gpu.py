@@ -141,7 +141,7 @@ def test_history_offsets_select_the_same_streams(eng):
     ("c1", lambda n: configs.c1_he_fe(n, seed=31), 20000),
     ("vac_cu_10keV", lambda n: configs.vacancy_case("cu", 1e4, n, seed=31), 20000),
     ("c5", lambda n: configs.c5_c_w(n, seed=31), 2000),
-    ("c2", lambda n: configs.c2_cu_cu(n, seed=31), 800),
+    ("c2", lambda n: configs.c2_cu_cu(n, seed=31), 3000),
 ])
 def test_cascade_statistics_vs_reference(eng, name, prob, n):
     ref = load_tally(name)
