@@ -12,7 +12,12 @@ computed by the reference library; the harness only drives its API.
                       the first collisions of one history, std::mt19937 stream
   tally_<case>.json   full-cascade tallies (FIFO drain of apps/runmytrim.C:71-93),
                       one std::mt19937 seed per history
-Usage:  python tests/golden/make_fixtures.py [--quick]
+  runmytrim/<case>/   a JSON deck of the reference (tests/json, validation/) with
+                      fewer ions, and the output file and totals the reference's
+                      runmytrim loop writes for it (oracle/_ref/runmytrim_ref:
+                      the reference's driver loop and output modules; jsoncpp is
+                      absent, so the deck is handed over as a flat spec)
+Usage:  python tests/golden/make_fixtures.py [--quick] [--tally=NAME] [--runmytrim]
 """
 import json
 import os
@@ -139,10 +144,94 @@ def tallies(quick, only=None):
               tot["vac"] / prob.n_hist, flush=True)
 
 
+RUNMYTRIM_REF = os.path.join(ROOT, "oracle", "_ref", "runmytrim_ref")
+REF = "/root/reference"
+# case -> (deck under the reference tree, overrides of its options / ion blocks)
+RUNMYTRIM_CASES = {
+    "cu_on_cu_150keV": ("tests/json/cu_on_cu.json", {"number": 5}),
+    "cu_on_cu_150keV_threads": ("tests/json/cu_on_cu.json", {"number": 6, "threads": 3}),
+    "cu_on_cu_150keV_evac": ("tests/json/cu_on_cu.json", {"number": 3, "type": "vacenergycount"}),
+    "cu_on_cu_150keV_range": ("validation/cu_on_cu/cu_on_cu.json", {"number": 200, "threads": 1}),
+    "xe_on_uo2_scale10": ("tests/json/xe_on_uo2_scale10.json", {"number": 1}),
+    "cu_on_cu_vac_1keV": ("validation/vacancy_count/cu_on_cu_vac.json", {}),
+    "si_on_c_vac": ("validation/vacancy_count/si_on_c_vac.json", {"number": 200}),
+    "cu_on_cu_range_1keV": ("validation/vacancy_count/cu_on_cu_range.json", {"number": 2000}),
+}
+
+
+def strip_comments(text):
+    out = []
+    for line in text.splitlines():
+        i = line.find("//")
+        out.append(line if i < 0 else line[:i])
+    return "\n".join(out)
+
+
+def runmytrim_spec(deck, base):
+    m = deck["mytrim"]
+    opt, ion, out = m.get("options", {}), m["ion"], m["output"]
+    lines = ["type %s" % out["type"], "base %s" % base, "seed %d" % opt["seed"],
+             "threads %d" % opt.get("threads", 1),
+             # the reference leaves primaries_only uninitialised when the deck omits it
+             "primaries_only %d" % int(bool(out.get("primaries_only", False)))]
+    if "scale" in opt:
+        lines.append("scale %r" % float(opt["scale"]))
+    ion_line = "ion %d %r %r %d" % (ion["Z"], float(ion["mass"]), float(ion["energy"]), ion["number"])
+    if "final_energy" in ion:
+        ion_line += " %r" % float(ion["final_energy"])
+    lines.append(ion_line)
+    for L in m["sample"]["layers"]:
+        lines.append("layer %r %r" % (float(L["thickness"]), float(L["rho"])))
+        for e in L["elements"]:
+            lines.append("el %d %r %r %s %s" % (e["Z"], float(e["mass"]), float(e["fraction"]),
+                                                repr(float(e["edisp"])) if "edisp" in e else "-",
+                                                repr(float(e["elbind"])) if "elbind" in e else "-"))
+    return "\n".join(lines) + "\n"
+
+
+def runmytrim_fixtures():
+    for name, (rel, over) in RUNMYTRIM_CASES.items():
+        with open(os.path.join(REF, rel)) as f:
+            deck = json.loads(strip_comments(f.read()))
+        m = deck["mytrim"]
+        for k, v in over.items():
+            if k in ("number",):
+                m["ion"][k] = v
+            elif k in ("threads", "seed", "scale"):
+                m.setdefault("options", {})[k] = v
+            elif k == "type":
+                m["output"]["type"] = v
+        m["output"]["base"] = name
+        dst = os.path.join(HERE, "runmytrim", name)
+        os.makedirs(dst, exist_ok=True)
+        with tempfile.TemporaryDirectory() as tmp:
+            spec = os.path.join(tmp, "spec.txt")
+            with open(spec, "w") as f:
+                f.write(runmytrim_spec(deck, name))
+            res = subprocess.run([RUNMYTRIM_REF, spec], cwd=tmp, capture_output=True, text=True, check=True)
+            totals = dict((k, float(v)) for k, v in (l.split() for l in res.stdout.splitlines()))
+            suffix = {"vaccount": "_vac.dat", "vacenergycount": "_evac.dat", "range": "_ranges.dat"}
+            outname = name + suffix[m["output"]["type"]]
+            with open(os.path.join(tmp, outname)) as f:
+                data = f.read()
+        with open(os.path.join(dst, outname), "w") as f:
+            f.write(data)
+        with open(os.path.join(dst, "deck.json"), "w") as f:
+            json.dump(deck, f, indent=1)
+        totals["source"] = rel
+        totals["npka"] = m["ion"]["number"]
+        with open(os.path.join(dst, "totals.json"), "w") as f:
+            json.dump(totals, f, indent=1)
+        print(name, totals, flush=True)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     quick = "--quick" in sys.argv
+    if "--runmytrim" in sys.argv:
+        runmytrim_fixtures()
+        return
     only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--tally=")]
     if only:  # regenerate the named tally fixtures only
         tallies(quick, only)
