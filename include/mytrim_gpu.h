@@ -137,7 +137,7 @@ typedef struct {
 
 /* Tuning knobs of the persistent kernel (0 = default). */
 typedef struct {
-  int block;          /* threads per workgroup (multiple of 64) */
+  int block;          /* threads per workgroup (multiple of 64, at most 256) */
   int blocks_per_cu;  /* resident workgroups per CU */
   int stack_cap;      /* per-wave LDS recoil stack entries */
   long long queue_cap;/* HBM queue entries (power of two) */

@@ -286,7 +286,8 @@ int mt_default_tables(double* rows, int n_rows, int n_cols) {
 
 int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* t) {
   if (!ctx || !t) return fail(MT_E_ARG, "NULL argument");
-  if (t->block && (t->block % 64 || t->block > 1024)) return fail(MT_E_ARG, "block must be a multiple of 64");
+  if (t->block && (t->block % 64 || t->block > kCascadeMaxBlock))
+    return fail(MT_E_ARG, "block must be a multiple of 64 and at most 256 (the kernel's launch bound)");
   if (t->queue_cap && (t->queue_cap & (t->queue_cap - 1))) return fail(MT_E_ARG, "queue_cap must be a power of two");
   if (t->stack_cap < 0 || t->stack_cap > 1024) return fail(MT_E_ARG, "stack_cap out of range");
   ctx->tune = *t;
@@ -743,7 +744,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // candidate shapes: wave count per block x LDS stack depth x resident blocks per CU
   // (0 = as many as fit); queue capacity stays at its default
   std::vector<mt_tuning> cand;
-  for (int block : {64, 128, 256, 512})
+  for (int block : {64, 128, 256})
     for (int cap : {32, 64, 128})
       for (int bpc : {0, 1, 2}) {
         mt_tuning t{};

@@ -102,6 +102,9 @@ struct Params {
   MlpDev mlp;
 };
 
+// launch bound of the persistent kernel (threads per workgroup)
+constexpr int kCascadeMaxBlock = 256;
+
 // device spans zeroed before a run
 constexpr int kMaxSpans = 16;
 struct ZeroSpans {

@@ -170,7 +170,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
 }  // namespace
 
 template <int ARITH, int POT, int GEO, bool PERHIST>
-__global__ void __launch_bounds__(256) cascade_kernel(Params P) {
+__global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
   extern __shared__ __align__(16) unsigned char lds[];
   {
     const uint4* src = (const uint4*)P.tables;

@@ -225,12 +225,65 @@ def runmytrim_fixtures():
         print(name, totals, flush=True)
 
 
+RUNSTOPPING_REF = os.path.join(ROOT, "oracle", "_ref", "runstopping_ref")
+# case -> deck under the reference tree (its "stopping" block)
+RUNSTOPPING_CASES = {
+    "h_on_fe": "validation/h_on_fe/input.json",
+    "he_on_fe": "validation/he_on_fe/input.json",
+    "c_on_w": "validation/c_on_w/input.json",
+}
+# extra decks written here: the array and single-number energy forms
+RUNSTOPPING_EXTRA = {
+    "xe_on_uo2_list": {"stopping": {"ion": {"Z": 54, "mass": 131.904,
+                                            "energy": [10, 1000, 30000, 1e6, 1e8]},
+                                    "material": {"rho": 10.97, "elements": [
+                                        {"Z": 92, "mass": 238.03, "fraction": 1},
+                                        {"Z": 8, "mass": 15.999, "fraction": 2}]}}},
+    "u_on_u_single": {"stopping": {"ion": {"Z": 92, "mass": 238.03, "energy": 2e6},
+                                   "material": {"rho": 19.05, "elements": [
+                                       {"Z": 92, "mass": 238.03, "fraction": 1}]}}},
+}
+
+
+def runstopping_fixtures():
+    decks = {}
+    for name, rel in RUNSTOPPING_CASES.items():
+        with open(os.path.join(REF, rel)) as f:
+            decks[name] = json.loads(strip_comments(f.read()))
+    decks.update(RUNSTOPPING_EXTRA)
+    for name, deck in decks.items():
+        st = deck["stopping"]
+        lines = ["material %r" % float(st["material"]["rho"])]
+        for e in st["material"]["elements"]:
+            lines.append("el %d %r %r" % (e["Z"], float(e["mass"]), float(e["fraction"])))
+        lines.append("ion %d %r" % (st["ion"]["Z"], float(st["ion"]["mass"])))
+        E = st["ion"]["energy"]
+        if isinstance(E, dict):
+            mode = "step" if "step" in E else "mult"
+            lines.append("%s %r %r %r" % (mode, float(E["begin"]), float(E["end"]), float(E[mode])))
+        else:
+            lines.append("energies " + " ".join(repr(float(x)) for x in (E if isinstance(E, list) else [E])))
+        dst = os.path.join(HERE, "runstopping", name)
+        os.makedirs(dst, exist_ok=True)
+        with tempfile.TemporaryDirectory() as tmp:
+            spec = os.path.join(tmp, "spec.txt")
+            with open(spec, "w") as f:
+                f.write("\n".join(lines) + "\n")
+            out = subprocess.run([RUNSTOPPING_REF, spec], capture_output=True, text=True, check=True).stdout
+        with open(os.path.join(dst, "deck.json"), "w") as f:
+            json.dump({"stopping": st}, f, indent=1)
+        with open(os.path.join(dst, "stdout.txt"), "w") as f:
+            f.write(out)
+        print(name, len(out.splitlines()), "lines", flush=True)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     quick = "--quick" in sys.argv
     if "--runmytrim" in sys.argv:
         runmytrim_fixtures()
+        runstopping_fixtures()
         return
     only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--tally=")]
     if only:  # regenerate the named tally fixtures only

@@ -250,6 +250,10 @@ def test_invalid_input_raises(eng):
     bad = configs.c1_he_fe(4).with_(materials=[Material(7.87, [Element(120, 55.0)])])
     with pytest.raises(gpu.MtError):
         eng.set_target(bad)
+    for kw in ({"block": 512}, {"block": 100}, {"queue_cap": 1000}, {"stack_cap": 5000}):
+        with pytest.raises(gpu.MtError):
+            eng.tune(**kw)
+    eng.tune()
 
 
 # ---------------------------------------------------------------- FP32 physics path
@@ -329,7 +333,7 @@ def test_autotune_picks_a_working_shape(eng):
     best, rate = eng.autotune(prob, sample=500)
     print("autotune: block %d, blocks/CU %d, stack %d -> %.3g coll/s" %
           (best.block, best.blocks_per_cu, best.stack_cap, rate))
-    assert best.block in (64, 128, 256, 512) and best.stack_cap in (32, 64, 128)
+    assert best.block in (64, 128, 256) and best.stack_cap in (32, 64, 128)
     assert rate > 0
     r = eng.run(prob.with_(n_hist=300), per_history=True)
     eng.tune()
