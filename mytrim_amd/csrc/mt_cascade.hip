@@ -208,6 +208,18 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
   bool wave_idle = false;  // registered idle
   int backoff = 1;         // idle sleep rounds
   unsigned long long fe_pending = 0;  // ended flights not yet added to kFlightsEnded
+  // per lane: outstanding-flight count of the history before this lane's last
+  // flight-end decrement, not yet examined (1 means the cascade closed)
+  int pend_old = 0;
+  bool pend = false;
+  auto close_pending = [&]() {
+    const uint64_t closed_m = __ballot(pend && pend_old == 1);
+    pend = false;
+    if (lane == 0 && closed_m) {
+      atomicAdd(&ctrl[kCascDone], (unsigned long long)__popcll(closed_m));
+      atomicAdd(&ctrl[kLive], (unsigned long long)(-(long long)__popcll(closed_m)));
+    }
+  };
   unsigned long long idle_since = 0;
   unsigned long long c_coll = 0, c_flight = 0, c_vac = 0, c_repl = 0, c_subst = 0, c_inter = 0,
                      c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0, c_surr = 0;
@@ -223,9 +235,128 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
 
   while (true) {
     // ------------------------------------------------------------ refill
+    // Idle lanes take flights from the wave's LDS stack.  When the stack
+    // cannot serve them, the wave first tops it up in bulk (up to half its
+    // capacity) from the HBM queue and then from the primaries, so the
+    // shared control words are touched once per batch, not per flight.
     uint64_t idle_m = __ballot(!active);
     if (idle_m) {
-      int n_idle = __popcll(idle_m);
+      const int n_idle = __popcll(idle_m);
+      if (scount < n_idle) {
+        const int want = min(cap - scount, max(n_idle, cap / 2) - scount);
+        const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
+        const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
+        const bool may_pop = t0 > h0;
+        bool may_pka = !may_pop && ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
+        long long n_adm = 0;
+        if (may_pka) {  // cascade budget (see Params)
+          const uint64_t live = ld_relaxed(&ctrl[kLive]);
+          const uint64_t done = ld_relaxed(&ctrl[kCascDone]);
+          const double fhat =
+              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : P.f_init;
+          const long long cmax = max((long long)(P.budget / fmax(fhat, 1.0)), P.c_floor);
+          n_adm = min((long long)want, cmax - (long long)live);
+          may_pka = n_adm > 0;
+        }
+        if (wave_idle && (may_pop || may_pka)) {
+          if (lane == 0) atomicAdd(&ctrl[kState], (1ull << 32) - 1ull);  // epoch += 1, idle -= 1
+          wave_idle = false;
+        }
+        if (may_pop) {
+          // claim up to `want` queue positions with one CAS
+          uint64_t got = 0, base = 0;
+          if (lane == 0) {
+            uint64_t h = h0, t = t0;
+            for (int tries = 0; tries < 4 && t > h; ++tries) {
+              const uint64_t k = min((uint64_t)want, t - h);
+              unsigned long long expect = h;
+              if (__hip_atomic_compare_exchange_strong(&ctrl[kHead], &expect, h + k, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                got = k;
+                base = h;
+                break;
+              }
+              h = expect;  // the head another wave moved it to
+              t = ld_relaxed(&ctrl[kTail]);
+            }
+          }
+          got = bcast64(got, 0);
+          base = bcast64(base, 0);
+          if (got > 0) {
+            ++c_qops;
+            bool mine = lane < (int)got;
+            uint64_t slot = 0;
+            if (mine) {
+              const uint64_t pos = base + (uint64_t)lane;
+              slot = pos & P.q.mask;
+              const uint32_t lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
+              const unsigned long long t_start = (unsigned long long)wall_clock64();
+              while (__hip_atomic_load(&P.q.ready[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                     lap) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((unsigned long long)wall_clock64() - t_start > P.max_wait_ticks) {
+                  err |= kErrTimeout;
+                  mine = false;
+                  break;
+                }
+                if (ld_relaxed(&ctrl[kError]) != 0ull) {  // another wave aborted the run
+                  mine = false;
+                  break;
+                }
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (mine) {
+              const int j = scount + lane;
+              const uint32_t m = P.q.meta[slot];
+              if ((int)(m & 0xffu) >= T.n_species) err |= kErrBadEntry;
+              S.px[j] = P.q.px[slot];
+              S.py[j] = P.q.py[slot];
+              S.pz[j] = P.q.pz[slot];
+              S.dx[j] = P.q.dx[slot];
+              S.dy[j] = P.q.dy[slot];
+              S.dz[j] = P.q.dz[slot];
+              S.E[j] = P.q.E[slot];
+              S.key[j] = P.q.key[slot];
+              S.meta[j] = (int)(m & 0xffu) < T.n_species ? m : 0u;
+              S.tag[j] = P.q.tag[slot];
+              S.hist[j] = P.q.hist[slot];
+            }
+            const uint64_t bad = __ballot((err & (kErrTimeout | kErrBadEntry)) != 0u);
+            if (bad == 0ull) {
+              scount += (int)got;
+            } else if (lane == __ffsll((long long)bad) - 1) {  // abort the run: every wave leaves
+              atomicOr((unsigned int*)&ctrl[kError], err & (kErrTimeout | kErrBadEntry));
+            }
+          }
+        } else if (may_pka) {
+          // admit primaries into the stack
+          uint64_t first = 0;
+          if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_adm);
+          first = bcast64(first, 0);
+          const long long got = min(n_adm, max(0ll, P.n_hist - (long long)first));
+          if (lane == 0 && got > 0) atomicAdd(&ctrl[kLive], (unsigned long long)got);
+          const long long jh = (long long)first + lane;
+          if (lane < got) {
+            __hip_atomic_store(&P.h_out[jh], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const long long e = P.pka_uniform ? 0 : jh;
+            const int j = scount + lane;
+            S.px[j] = P.pka_pos[3 * e + 0];
+            S.py[j] = P.pka_pos[3 * e + 1];
+            S.pz[j] = P.pka_pos[3 * e + 2];
+            S.dx[j] = P.pka_dir[3 * e + 0];
+            S.dy[j] = P.pka_dir[3 * e + 1];
+            S.dz[j] = P.pka_dir[3 * e + 2];
+            S.E[j] = P.pka_E[e];
+            S.key[j] = 0ull;
+            S.meta[j] = (uint32_t)P.pka_species[e];
+            S.tag[j] = -1;
+            S.hist[j] = (uint32_t)(P.hist0 + jh);
+          }
+          scount += (int)max(got, 0ll);
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
       const int take = min(n_idle, scount);
       if (take > 0) {
         const int r = lane_rank(idle_m);
@@ -249,130 +380,6 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
           active = true;
         }
         scount -= take;
-        idle_m = __ballot(!active);
-        n_idle = __popcll(idle_m);
-      }
-      if (n_idle > 0) {
-        // HBM queue: claim up to n_idle positions with one CAS
-        const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
-        const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
-        const bool may_pop = t0 > h0;
-        bool may_pka = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
-        long long n_adm = 0;
-        if (may_pka) {  // cascade budget (see Params)
-          const uint64_t live = ld_relaxed(&ctrl[kLive]);
-          const uint64_t done = ld_relaxed(&ctrl[kCascDone]);
-          const double fhat =
-              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : P.f_init;
-          const long long cmax = max((long long)(P.budget / fmax(fhat, 1.0)), P.c_floor);
-          n_adm = min((long long)n_idle, cmax - (long long)live);
-          may_pka = n_adm > 0 && (long long)(t0 - h0) < P.admit_limit;
-        }
-        if (wave_idle && (may_pop || may_pka)) {
-          if (lane == 0) atomicAdd(&ctrl[kState], (1ull << 32) - 1ull);  // epoch += 1, idle -= 1
-          wave_idle = false;
-        }
-        if (may_pop) {
-          uint64_t got = 0, base = 0;
-          if (lane == 0) {
-            for (int tries = 0; tries < 8; ++tries) {
-              const uint64_t h = ld_relaxed(&ctrl[kHead]);
-              const uint64_t t = ld_relaxed(&ctrl[kTail]);
-              if (t <= h) break;
-              const uint64_t k = min((uint64_t)n_idle, t - h);
-              unsigned long long expect = h;
-              if (__hip_atomic_compare_exchange_strong(&ctrl[kHead], &expect, h + k, __ATOMIC_RELAXED,
-                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                got = k;
-                base = h;
-                break;
-              }
-            }
-          }
-          got = bcast64(got, 0);
-          base = bcast64(base, 0);
-          if (got > 0) {
-            ++c_qops;
-            const int r = lane_rank(idle_m);
-            bool mine = !active && r < (int)got;
-            uint64_t slot = 0;
-            if (mine) {
-              const uint64_t pos = base + (uint64_t)r;
-              slot = pos & P.q.mask;
-              const uint32_t lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
-              const unsigned long long t_start = (unsigned long long)wall_clock64();
-              while (__hip_atomic_load(&P.q.ready[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-                     lap) {
-                __builtin_amdgcn_s_sleep(1);
-                if ((unsigned long long)wall_clock64() - t_start > P.max_wait_ticks) {
-                  err |= kErrTimeout;
-                  mine = false;
-                  break;
-                }
-                if (ld_relaxed(&ctrl[kError]) != 0ull) {  // another wave aborted the run
-                  mine = false;
-                  break;
-                }
-              }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (mine) {
-              F.px = P.q.px[slot];
-              F.py = P.q.py[slot];
-              F.pz = P.q.pz[slot];
-              F.dx = P.q.dx[slot];
-              F.dy = P.q.dy[slot];
-              F.dz = P.q.dz[slot];
-              F.E = P.q.E[slot];
-              F.key = P.q.key[slot];
-              const uint32_t m = P.q.meta[slot];
-              F.sp = (int)(m & 0xffu);
-              F.gen = (int)(m >> 8);
-              F.tag = P.q.tag[slot];
-              F.hist = P.q.hist[slot];
-              F.ic = 0;
-              F.eel = 0.0;
-              if (F.sp >= T.n_species) {
-                err |= kErrBadEntry;
-              } else {
-                active = true;
-              }
-            }
-            idle_m = __ballot(!active);
-            n_idle = __popcll(idle_m);
-          }
-        }
-        if (n_idle > 0 && may_pka) {
-          n_adm = min(n_adm, (long long)n_idle);
-          {
-            uint64_t first = 0;
-            if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_adm);
-            first = bcast64(first, 0);
-            const long long got = min(n_adm, max(0ll, P.n_hist - (long long)first));
-            if (lane == 0 && got > 0) atomicAdd(&ctrl[kLive], (unsigned long long)got);
-            const int r = lane_rank(idle_m);
-            const long long j = (long long)first + r;
-            if (!active && r < got) {
-              __hip_atomic_store(&P.h_out[j], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              const long long e = P.pka_uniform ? 0 : j;
-              F.px = P.pka_pos[3 * e + 0];
-              F.py = P.pka_pos[3 * e + 1];
-              F.pz = P.pka_pos[3 * e + 2];
-              F.dx = P.pka_dir[3 * e + 0];
-              F.dy = P.pka_dir[3 * e + 1];
-              F.dz = P.pka_dir[3 * e + 2];
-              F.E = P.pka_E[e];
-              F.sp = P.pka_species[e];
-              F.gen = 0;
-              F.tag = -1;
-              F.hist = (uint32_t)(P.hist0 + j);
-              F.key = 0;
-              F.ic = 0;
-              F.eel = 0.0;
-              active = true;
-            }
-          }
-        }
       }
       if (__ballot(active) != 0ull) backoff = 1;
       if (__ballot(active) == 0ull) {
@@ -395,6 +402,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
           break;
         }
         if (ld_relaxed(&ctrl[kError]) != 0ull) break;  // a wave aborted the run
+        close_pending();
         // back off (up to ~30 us) so idle waves do not crowd the control words
         for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(127);
         backoff = min(2 * backoff, 8);
@@ -645,17 +653,15 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
     const uint64_t fin_m = __ballot(fin);
     if (fin_m) {
       c_flight += __popcll(fin_m);
-      bool closed = false;  // this flight was its cascade's last
-      if (fin) closed = atomicAdd(&P.h_out[F.hist - (uint32_t)P.hist0], -1) == 1;
-      const uint64_t closed_m = __ballot(closed);
-      fe_pending += __popcll(fin_m);
-      if (lane == 0) {
-        if (fe_pending >= 256ull) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
-        if (closed_m) {
-          atomicAdd(&ctrl[kCascDone], (unsigned long long)__popcll(closed_m));
-          atomicAdd(&ctrl[kLive], (unsigned long long)(-(long long)__popcll(closed_m)));
-        }
+      // the decrement's old value is read one iteration later (close_pending)
+      // so its latency overlaps the next collision
+      close_pending();
+      if (fin) {
+        pend_old = atomicAdd(&P.h_out[F.hist - (uint32_t)P.hist0], -1);
+        pend = true;
       }
+      fe_pending += __popcll(fin_m);
+      if (lane == 0 && fe_pending >= 256ull) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
       c_vacuum += __popcll(__ballot(fin && vacuum));
       if (fin) {
         if (vacuum) atomicAdd(&P.energy[eEvac], F.E);

@@ -96,6 +96,8 @@ int orc_uo2(int seed, double r, double cbf, int nev, char* erec, int erec_len, c
 int orc_run(const orc_run_spec* spec, orc_run_out* out);
 /* algorithmic work per collision: 16 doubles (see runs.cpp) */
 int orc_workcount(const orc_run_spec* spec, double* out);
+/* (eps, b, E, target Z) of every collision of the spec's histories */
+long long orc_scatter_inputs(const orc_run_spec* spec, double* rows, long long max_rows);
 /* per-collision trace of history `h` (rows of 21 doubles, see runs.cpp) */
 int orc_trace(const orc_run_spec* spec, long long h, double* rows, int max_rows);
 

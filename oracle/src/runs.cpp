@@ -553,6 +553,50 @@ int orc_trace(const orc_run_spec* sp, long long h, double* rows, int max_rows) {
   return hk.n < max_rows ? hk.n : max_rows;
 }
 
+// Scattering inputs of every collision of histories [hist0, hist0 + n_hist)
+// (std::mt19937 draws): rows of (eps, b, E before the collision, target Z).
+namespace {
+struct ScatterInputHooks : Hooks {
+  double* rows;
+  long long max_rows, n = 0;
+  void checkPKAState() override {
+    if (n < max_rows) {
+      double* r = rows + 4 * n;
+      r[0] = eps_in;
+      r[1] = b_in;
+      r[2] = E_in;
+      r[3] = element->Z;
+    }
+    ++n;
+  }
+};
+}  // namespace
+
+long long orc_scatter_inputs(const orc_run_spec* sp, double* rows, long long max_rows) {
+  const orc_run_spec& s = *sp;
+  std::vector<std::unique_ptr<Material>> mats;
+  std::unique_ptr<Sample> smp = build_sample(s, mats);
+  ScatterInputHooks hk;
+  hk.rows = rows;
+  hk.max_rows = max_rows;
+  MtRng rng;
+  MtDraws mtd(rng);
+  const Config cfg{s.tmin, s.tau, s.cw, s.scale, s.potential};
+  for (long long h = s.hist0; h < s.hist0 + s.n_hist && hk.n < max_rows; ++h) {
+    std::deque<Ion*> q;
+    q.push_back(make_primary(s, h));
+    rng.seed(mt_history_seed(s.seed, h));
+    while (!q.empty()) {
+      Ion* ion = q.front();
+      q.pop_front();
+      smp->averages(*ion, cfg.tmin);
+      trim(ion, q, *smp, cfg, mtd, hk);
+      delete ion;
+    }
+  }
+  return hk.n < max_rows ? hk.n : max_rows;
+}
+
 // Algorithmic work per collision of the reference's loop on `spec`
 // (std::mt19937 draws): out = flop_eq, collisions, flights, uniforms, then
 // add mul div cmp sqrt exp log pow cbrt sin cos atan2 counts per collision.

@@ -181,6 +181,16 @@ def workcount(prob) -> dict:
     return dict(zip(WORK_FIELDS, out.tolist()))
 
 
+def scatter_inputs(prob, max_rows: int) -> np.ndarray:
+    """(eps, b, E, target Z) of every collision of the problem's histories."""
+    spec, keep = make_spec(prob, 0)
+    rows = np.zeros((max_rows, 4))
+    lib().orc_scatter_inputs.restype = C.c_longlong
+    n = lib().orc_scatter_inputs(C.byref(spec), _p(rows, dp), C.c_longlong(max_rows))
+    del keep
+    return rows[:n]
+
+
 def trace(prob, h: int, max_rows: int, rng: int = 0) -> np.ndarray:
     L = lib()
     s, keep = make_spec(prob, rng)
