@@ -163,6 +163,7 @@ struct mt_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int cus = 0;
+  int lds_max = 64 * 1024;  // LDS bytes one workgroup may use
   double clock_khz = 100000.0;
   std::vector<CoefRow> coef;
   HostTarget tgt;
@@ -229,6 +230,9 @@ int mt_ctx_create(int device, mt_ctx** out) {
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
   int khz = 0;
   if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
+  int lds = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  if (e == hipSuccess && lds > 0) c->lds_max = lds;
   if (e != hipSuccess) {
     delete c;
     return fail(MT_E_HIP, std::string("context set-up: ") + hipGetErrorString(e));
@@ -458,9 +462,10 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
   size_t lds = blob.size() + size_t(nwb) * cascade_stack_bytes(cap);
   int lds_hist = ctx->tune.lds_hist >= 0 ? 1 : 0;
-  if (lds + lds_hist_bytes > 64 * 1024) lds_hist = 0;
+  const size_t lds_max = size_t(ctx->lds_max);
+  if (lds + lds_hist_bytes > lds_max) lds_hist = 0;
   if (lds_hist) lds += lds_hist_bytes;
-  if (lds > 64 * 1024) return fail(MT_E_LIMIT, "tables and stacks exceed 64 KiB of LDS");
+  if (lds > lds_max) return fail(MT_E_LIMIT, "tables and stacks exceed the LDS of a workgroup");
   const int pot = ctx->tgt.potential;
   const bool perhist = o->per_history != 0;
   int bpc = 0;

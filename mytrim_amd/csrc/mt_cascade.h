@@ -20,20 +20,20 @@ struct Queue {
   int log2cap;
 };
 
-// control words (one HBM allocation)
+// control words (one HBM allocation); the words waves contend on sit on
+// cache lines (128 B = 16 words) of their own
 enum Ctrl {
-  kState = 0,      // epoch << 32 | live waves << 16 | idle waves
-  kPkaNext = 1,    // next primary to start
-  kHead = 2,       // next queue position to pop
-  kTail = 3,       // next queue position to push
-  kError = 4,      // error bits
-  kQueueOps = 5,   // wave-level queue transfers
-  kQueuePeak = 6,  // largest tail - head seen at a push
-  // cascade bookkeeping, on cache lines (128 B) of their own
-  kLive = 16,      // admitted cascades not yet complete
-  kCascDone = 17,  // completed cascades
-  kFlightsEnded = 32,
-  kNumCtrl = 48
+  kState = 0,           // epoch << 32 | live waves << 16 | idle waves
+  kPkaNext = 16,        // next primary to start
+  kHead = 32,           // next queue position to pop
+  kTail = 48,           // next queue position to push
+  kError = 64,          // error bits
+  kQueueOps = 65,       // wave-level queue transfers
+  kQueuePeak = 80,      // largest tail - head seen at a push
+  kLive = 96,           // admitted cascades not yet complete
+  kCascDone = 97,       // completed cascades
+  kFlightsEnded = 112,
+  kNumCtrl = 128
 };
 
 enum ErrBits {

@@ -207,6 +207,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
   int scount = 0;          // entries in this wave's LDS stack
   bool wave_idle = false;  // registered idle
   int backoff = 1;         // idle sleep rounds
+  int poll_skip = 0;       // iterations before the shared queue is polled again
+  long long peak_seen = 0; // largest queue fill this wave has reported
   unsigned long long fe_pending = 0;  // ended flights not yet added to kFlightsEnded
   // per lane: outstanding-flight count of the history before this lane's last
   // flight-end decrement, not yet examined (1 means the cascade closed)
@@ -242,7 +244,10 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
     uint64_t idle_m = __ballot(!active);
     if (idle_m) {
       const int n_idle = __popcll(idle_m);
-      if (scount < n_idle) {
+      if (poll_skip > 0) --poll_skip;
+      // a wave whose last poll found nothing waits a few iterations before
+      // polling again, unless all its lanes are idle
+      if (scount < n_idle && (poll_skip == 0 || n_idle == 64)) {
         const int want = min(cap - scount, max(n_idle, cap / 2) - scount);
         const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
         const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
@@ -355,6 +360,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
           }
           scount += (int)max(got, 0ll);
         }
+        if (!may_pop && !may_pka) poll_skip = 4;
         __builtin_amdgcn_wave_barrier();
       }
       const int take = min(n_idle, scount);
@@ -619,7 +625,10 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
         // consumers may already have claimed reserved slots, so head can be
         // past base: the occupancy is a signed difference
         const long long fill = (long long)(base + (uint64_t)n_over) - (long long)head;
-        if (lane == 0 && fill > 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
+        if (fill > peak_seen) {  // this wave's own maximum gates the shared one
+          peak_seen = fill;
+          if (lane == 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
+        }
         if (fill > (long long)P.q.mask) {  // abort: every wave leaves, the caller reports the error
           err |= kErrQueueFull;
           if (lane == 0) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
