@@ -248,7 +248,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
       // a wave whose last poll found nothing waits a few iterations before
       // polling again, unless all its lanes are idle
       if (scount < n_idle && (poll_skip == 0 || n_idle == 64)) {
-        const int want = min(cap - scount, max(n_idle, cap / 2) - scount);
+        // one lane loads one entry, so a batch holds at most 64
+        const int want = min(64, min(cap - scount, max(n_idle, cap / 2) - scount));
         const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
         const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
         const bool may_pop = t0 > h0;
@@ -383,9 +384,15 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
           F.hist = S.hist[j];
           F.ic = 0;
           F.eel = 0.0;
-          active = true;
+          // every index a flight uses is checked here, where all flights start
+          if (F.sp < T.n_species && F.hist - (uint32_t)P.hist0 < (uint32_t)P.n_hist)
+            active = true;
+          else
+            err |= kErrBadEntry;
         }
         scount -= take;
+        const uint64_t bad = __ballot((err & kErrBadEntry) != 0u);
+        if (bad && lane == __ffsll((long long)bad) - 1) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrBadEntry);
       }
       if (__ballot(active) != 0ull) backoff = 1;
       if (__ballot(active) == 0ull) {

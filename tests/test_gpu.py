@@ -339,3 +339,17 @@ def test_autotune_picks_a_working_shape(eng):
     eng.tune()
     s = eng.run(prob.with_(n_hist=300), per_history=True)
     assert np.array_equal(r["h_coll"], s["h_coll"]) and r["vac"] == s["vac"]
+
+
+@pytest.mark.parametrize("block,cap", [(64, 256), (64, 512), (128, 256)])
+def test_deep_lds_stacks_match_oracle(eng, block, cap):
+    """LDS stacks deeper than a wave: refills still move at most one entry per
+    lane, and the histories stay those of the oracle."""
+    prob = configs.c2_cu_cu(4, seed=8)
+    eng.tune(block=block, stack_cap=cap)
+    try:
+        g = eng.run(prob, per_history=True)
+    finally:
+        eng.tune()
+    o = O.run(prob, rng=1, nthreads=4, per_history=True)
+    assert np.array_equal(g["h_vac"], o["h_vac"]) and np.array_equal(g["h_coll"], o["h_coll"])
