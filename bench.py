@@ -74,14 +74,14 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}), flush=True)
 
 
-def train_surrogate(eng, gpu, n_train: int, epochs: int):
-    opts = gpu.Engine.surrogate_opts(n_train=n_train, epochs=epochs, bf16=True)
+def train_surrogate(eng, gpu, prob, n_train: int, epochs: int):
+    opts = gpu.Engine.surrogate_opts(prob, n_train=n_train, epochs=epochs, bf16=True)
     t0 = time.perf_counter()
     params, loss = eng.surrogate_train(opts, gpu.Engine.surrogate_init(1))
     t_train = time.perf_counter() - t0
     eng.surrogate_install(opts, params)
     return {"train_s": t_train, "final_loss": float(loss[-1]), "n_train": n_train, "epochs": epochs,
-            "domain": {"eps": [opts.eps_min, opts.eps_max], "b": [0.0, opts.b_max]}}
+            "domain": {k: v for k, v in eng.surrogate_norm(opts).items() if k in ("eps_lo", "eps_hi", "b_hi")}}
 
 
 def timed(eng, prob, arith, n_hist, steps):
@@ -106,8 +106,8 @@ def main():
     ap.add_argument("--autotune-sample", type=int, default=1000, help="0 = default launch shape")
     ap.add_argument("--variant-histories", type=int, default=100000,
                     help="histories for the FP32 and FP32+surrogate lines (0 = skip)")
-    ap.add_argument("--surrogate-train", type=int, default=10_000_000)
-    ap.add_argument("--surrogate-epochs", type=int, default=8)
+    ap.add_argument("--surrogate-train", type=int, default=4_000_000)
+    ap.add_argument("--surrogate-epochs", type=int, default=60)
     ap.add_argument("--cpu-histories", type=int, default=0, help="0 = size to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -143,7 +143,7 @@ def main():
         nv = min(args.variant_histories, args.histories)
         c64, w64, k64, _ = timed(eng, prob, gpu.MT_FP64, nv, 1)
         c32, w32, k32, _ = timed(eng, prob, gpu.MT_FP32, nv, 1)
-        sur = train_surrogate(eng, gpu, args.surrogate_train, args.surrogate_epochs)
+        sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)
         cs, ws, ks, rs = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
         variants = {
             "histories": nv,

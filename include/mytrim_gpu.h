@@ -168,16 +168,21 @@ typedef struct {
 } mt_event;
 
 /* Scattering surrogate (no counterpart in the reference): an MLP
- * 2 -> 64 -> 64 -> 1 (tanh) from (log10 eps, b) to ln sin^2(theta/2), trained
- * on the device on the engine's own universal-potential scattering; used by
+ * 3 -> 64 -> 64 -> 1 (tanh) from (log10 eps, log10 b, b) to ln sin^2(theta/2),
+ * trained on the device on the engine's own universal-potential scattering
+ * over the (eps, b) pairs cascades in the context's target draw; used by
  * MT_FP32_SURROGATE runs inside its domain, the MAGIC solve outside it. */
-#define MT_SURROGATE_PARAMS 4417
+#define MT_SURROGATE_PARAMS 4481
+#define MT_SURROGATE_NORM 12
 typedef struct {
-  double eps_min, eps_max; /* reduced-energy domain, sampled log-uniform */
-  double b_max;            /* reduced impact parameters in [0, b_max] */
+  int n_ions;              /* species the cascades start with; target atoms are added */
+  const int* ion_Z;
+  const double* ion_m;
+  double E_min, E_max;     /* flight energies [eV] the training pairs cover */
+  double eps_max;          /* largest reduced energy handed to the surrogate */
   long long n_train;       /* training pairs generated on the device */
   int epochs, batch;
-  double lr;               /* Adam step size (halved at 1/3 and 2/3 of the run) */
+  double lr;               /* Adam step size, cosine-annealed to lr / 100 */
   uint32_t seed;
   int bf16;                /* inference with BF16 dot products (else FP32 FMA) */
 } mt_surrogate_opts;
@@ -214,6 +219,12 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, 
 int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, long long* n_used,
                    mt_event* events, int max_events, int* n_events);
 
+/* Input / output scaling and domain the options give in the context's target:
+ * e_mid e_inv_half lb_mid lb_inv_half b_mid b_inv_half b_floor y_off y_scale
+ * eps_lo eps_hi b_hi (x0 = (log10 eps - e_mid) e_inv_half, x1 = (log10
+ * max(b, b_floor) - lb_mid) lb_inv_half, x2 = (b - b_mid) b_inv_half,
+ * ln s2 = y y_scale + y_off). */
+int mt_surrogate_norm(mt_ctx* ctx, const mt_surrogate_opts* opts, float* norm);
 /* Train the surrogate from `params` (initial values in, trained values out);
  * epoch_loss[e] = mean squared error (normalised units) of epoch e. */
 int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* opts, float* params, double* epoch_loss);

@@ -43,8 +43,8 @@ struct FlightParams {
   long long* n_used;
 };
 hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t stream);
-hipError_t mlp_make_data(uint32_t seed, long long n, double lmin, double lmax, double bmax, const MlpNorm& nm,
-                         float* X, float* T, hipStream_t st);
+hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo,
+                         const MlpNorm& nm, float* X, float* T, hipStream_t st);
 hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
                      float* rec, float* g, float* m, float* v, int* t, float* batch_loss, double* losses,
                      hipStream_t st);
@@ -181,7 +181,7 @@ struct mt_ctx {
   std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
   DBuf f_ion, f_u, f_ev, f_used;
   // scattering surrogate
-  DBuf mlp_p, mlp_w2bf, mlp_X, mlp_T, mlp_rec, mlp_g, mlp_m, mlp_v, mlp_t, mlp_bl;
+  DBuf mlp_p, mlp_w2bf, mlp_X, mlp_T, mlp_rec, mlp_g, mlp_m, mlp_v, mlp_t, mlp_bl, mlp_tri;
   MlpNorm mlp_norm{};
   bool mlp_ready = false;
   int mlp_bf16 = 0;
@@ -201,7 +201,7 @@ struct mt_ctx {
     f_u.release();
     f_ev.release();
     f_used.release();
-    DBuf* ml[] = {&mlp_p, &mlp_w2bf, &mlp_X, &mlp_T, &mlp_rec, &mlp_g, &mlp_m, &mlp_v, &mlp_t, &mlp_bl};
+    DBuf* ml[] = {&mlp_p, &mlp_w2bf, &mlp_X, &mlp_T, &mlp_rec, &mlp_g, &mlp_m, &mlp_v, &mlp_t, &mlp_bl, &mlp_tri};
     for (DBuf* b : ml) b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -854,48 +854,103 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
   return MT_OK;
 }
 
-static std::string surrogate_norm(const mt_surrogate_opts* o, MlpNorm& nm) {
-  if (!(o->eps_min > 0.0) || !(o->eps_max > o->eps_min)) return "need 0 < eps_min < eps_max";
-  if (!(o->b_max > 0.0)) return "need b_max > 0";
-  const double l0 = std::log10(o->eps_min), l1 = std::log10(o->eps_max);
+// Training triples and scaling of the surrogate for the context's target:
+// every (species, material, element) the cascades of `o->ion_*` touch, with
+// the flight-energy range that keeps eps = fi E <= eps_max.
+static std::string surrogate_setup(const mt_ctx* ctx, const mt_surrogate_opts* o, MlpNorm& nm,
+                                   std::vector<MlpTriple>* out) {
+  if (!ctx->has_target) return "no target set";
+  if (ctx->tgt.potential != MT_UNIVERSAL) return "the surrogate is trained on the universal potential";
+  if (o->n_ions < 1 || !o->ion_Z || !o->ion_m) return "need at least one ion species";
+  if (!(o->E_min > 0.0) || !(o->E_max > o->E_min)) return "need 0 < E_min < E_max";
+  if (!(o->eps_max > 0.0)) return "need eps_max > 0";
+  std::vector<Species> species;
+  std::vector<int> ps;
+  std::string msg = collect_species(ctx->tgt, o->ion_Z, o->ion_m, o->n_ions, species, ps);
+  if (!msg.empty()) return msg;
+  std::vector<unsigned char> blob;
+  msg = build_tables(ctx->coef, ctx->tgt, species, blob);
+  if (!msg.empty()) return msg;
+  const TableHeader* H = reinterpret_cast<const TableHeader*>(blob.data());
+  const MatConst* mat = reinterpret_cast<const MatConst*>(blob.data() + H->off_mat);
+  const ElemConst* el = reinterpret_cast<const ElemConst*>(blob.data() + H->off_el);
+  std::vector<MlpTriple> tri;
+  double eps_lo = 1e300, eps_hi = 0.0, b_hi = 0.0;
+  for (int sp = 0; sp < H->n_species; ++sp)
+    for (int k = 0; k < H->n_mat; ++k) {
+      const MatConst& mc = mat[sp * H->n_mat + k];
+      for (int i = 0; i < mc.n_el; ++i) {
+        const ElemConst& ec = el[sp * H->n_el + mc.el_begin + i];
+        MlpTriple t{mc.a, mc.f, mc.epsdg, ec.ai, ec.fi, std::min(o->E_max, o->eps_max / ec.fi)};
+        if (!(t.E_hi > o->E_min)) continue;
+        tri.push_back(t);
+        eps_lo = std::min(eps_lo, ec.fi * o->E_min);
+        eps_hi = std::max(eps_hi, ec.fi * t.E_hi);
+        // pmax falls with energy: the widest impact parameter is at E_min
+        const double eeg = std::sqrt(o->E_min * mc.f * mc.epsdg);
+        b_hi = std::max(b_hi, mc.a / (eeg + std::sqrt(eeg) + 0.125 * std::pow(eeg, 0.1)) / ec.ai);
+      }
+    }
+  if (tri.empty()) return "no collision of these ions reaches the surrogate's domain";
+  const double l0 = std::log10(eps_lo), l1 = std::log10(eps_hi);
+  const double b_floor = 1e-4 * b_hi;
+  const double m0 = std::log10(b_floor), m1 = std::log10(b_hi);
   nm.e_mid = float(0.5 * (l0 + l1));
-  nm.e_inv_half = float(2.0 / (l1 - l0));
-  nm.b_mid = float(0.5 * o->b_max);
-  nm.b_inv_half = float(2.0 / o->b_max);
-  nm.y_off = -12.0f;
-  nm.y_scale = 12.0f;
-  nm.eps_lo = float(o->eps_min);
-  nm.eps_hi = float(o->eps_max);
-  nm.b_hi = float(o->b_max);
+  nm.e_inv_half = float(2.0 / std::max(l1 - l0, 1e-6));
+  nm.lb_mid = float(0.5 * (m0 + m1));
+  nm.lb_inv_half = float(2.0 / (m1 - m0));
+  nm.b_mid = float(0.5 * b_hi);
+  nm.b_inv_half = float(2.0 / b_hi);
+  nm.b_floor = float(b_floor);
+  nm.y_off = -8.0f;
+  nm.y_scale = 8.0f;
+  nm.eps_lo = float(eps_lo);
+  nm.eps_hi = float(eps_hi);
+  nm.b_hi = float(b_hi);
+  if (out) out->swap(tri);
   return "";
+}
+
+int mt_surrogate_norm(mt_ctx* ctx, const mt_surrogate_opts* o, float* norm) {
+  if (!ctx || !o || !norm) return fail(MT_E_ARG, "NULL argument");
+  MlpNorm nm;
+  const std::string msg = surrogate_setup(ctx, o, nm, nullptr);
+  if (!msg.empty()) return fail(MT_E_ARG, msg);
+  static_assert(sizeof(MlpNorm) == kMlpNormFloats * sizeof(float), "MlpNorm is 12 floats");
+  std::memcpy(norm, &nm, sizeof(nm));
+  return MT_OK;
 }
 
 int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* o, float* params, double* epoch_loss) {
   if (!ctx || !o || !params || !epoch_loss) return fail(MT_E_ARG, "NULL argument");
   MlpNorm nm;
-  std::string msg = surrogate_norm(o, nm);
+  std::vector<MlpTriple> tri;
+  std::string msg = surrogate_setup(ctx, o, nm, &tri);
   if (!msg.empty()) return fail(MT_E_ARG, msg);
   if (o->batch < 64 || o->batch > 65536 || o->epochs < 1 || o->n_train < o->batch || o->n_train > (1ll << 31))
     return fail(MT_E_ARG, "bad batch / epochs / n_train");
+  if (!(o->lr > 0.0)) return fail(MT_E_ARG, "need lr > 0");
   HIPCHECK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const long long n = o->n_train;
   const int B = o->batch;
   const int steps = int(n / B);
   HIPCHECK(ctx->mlp_p.ensure(kMlpParams * sizeof(float)));
-  HIPCHECK(ctx->mlp_X.ensure(size_t(n) * 2 * sizeof(float)));
+  HIPCHECK(ctx->mlp_X.ensure(size_t(n) * kMlpIn * sizeof(float)));
   HIPCHECK(ctx->mlp_T.ensure(size_t(n) * sizeof(float)));
-  HIPCHECK(ctx->mlp_rec.ensure(size_t(B) * 260 * sizeof(float)));
-  HIPCHECK(ctx->mlp_g.ensure(kMlpParams * sizeof(float)));
+  HIPCHECK(ctx->mlp_rec.ensure(size_t(B) * kMlpRecord * sizeof(float)));
+  HIPCHECK(ctx->mlp_g.ensure((kMlpParams + 1) * sizeof(float)));
   HIPCHECK(ctx->mlp_m.ensure(kMlpParams * sizeof(float)));
   HIPCHECK(ctx->mlp_v.ensure(kMlpParams * sizeof(float)));
   HIPCHECK(ctx->mlp_t.ensure(sizeof(int)));
   HIPCHECK(ctx->mlp_bl.ensure(size_t(steps) * sizeof(float)));
+  HIPCHECK(ctx->mlp_tri.ensure(tri.size() * sizeof(MlpTriple)));
+  HIPCHECK(hipMemcpyAsync(ctx->mlp_tri.p, tri.data(), tri.size() * sizeof(MlpTriple), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(ctx->mlp_p.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemsetAsync(ctx->mlp_m.p, 0, kMlpParams * sizeof(float), st));
   HIPCHECK(hipMemsetAsync(ctx->mlp_v.p, 0, kMlpParams * sizeof(float), st));
   HIPCHECK(hipMemsetAsync(ctx->mlp_t.p, 0, sizeof(int), st));
-  HIPCHECK(mlp_make_data(o->seed, n, std::log10(o->eps_min), std::log10(o->eps_max), o->b_max, nm,
+  HIPCHECK(mlp_make_data(o->seed, n, ctx->mlp_tri.as<MlpTriple>(), int(tri.size()), o->E_min, nm,
                          ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), st));
   HIPCHECK(mlp_train(ctx->mlp_p.as<float>(), ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), n, B, o->epochs,
                      float(o->lr), ctx->mlp_rec.as<float>(), ctx->mlp_g.as<float>(), ctx->mlp_m.as<float>(),
@@ -908,7 +963,7 @@ int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* o, float* params, d
 int mt_surrogate_install(mt_ctx* ctx, const mt_surrogate_opts* o, const float* params) {
   if (!ctx || !o || !params) return fail(MT_E_ARG, "NULL argument");
   MlpNorm nm;
-  std::string msg = surrogate_norm(o, nm);
+  std::string msg = surrogate_setup(ctx, o, nm, nullptr);
   if (!msg.empty()) return fail(MT_E_ARG, msg);
   HIPCHECK(hipSetDevice(ctx->device));
   HIPCHECK(ctx->mlp_p.ensure(kMlpParams * sizeof(float)));
@@ -950,12 +1005,11 @@ int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* o, const float* para
   if (!ctx || !o || !params || !eps || !b || !ln_s2 || !grad || !loss) return fail(MT_E_ARG, "NULL argument");
   if (n < 1 || n > 65536) return fail(MT_E_ARG, "batch out of range");
   MlpNorm nm;
-  std::string msg = surrogate_norm(o, nm);
+  std::string msg = surrogate_setup(ctx, o, nm, nullptr);
   if (!msg.empty()) return fail(MT_E_ARG, msg);
-  std::vector<float> X(2 * size_t(n)), T(n);
+  std::vector<float> X(kMlpIn * size_t(n)), T(n);
   for (int i = 0; i < n; ++i) {
-    X[2 * i] = float((std::log10(eps[i]) - nm.e_mid) * nm.e_inv_half);
-    X[2 * i + 1] = float((b[i] - nm.b_mid) * nm.b_inv_half);
+    mlp_inputs(nm, float(eps[i]), float(b[i]), X[3 * i], X[3 * i + 1], X[3 * i + 2]);
     T[i] = float((ln_s2[i] - nm.y_off) / nm.y_scale);
   }
   HIPCHECK(hipSetDevice(ctx->device));
@@ -963,7 +1017,7 @@ int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* o, const float* para
   HIPCHECK(ctx->scratch_a.ensure(kMlpParams * sizeof(float)));
   HIPCHECK(ctx->scratch_b.ensure(X.size() * sizeof(float)));
   HIPCHECK(ctx->scratch_c.ensure(T.size() * sizeof(float) + 16));
-  HIPCHECK(ctx->scratch_d.ensure(size_t(n) * 260 * sizeof(float)));
+  HIPCHECK(ctx->scratch_d.ensure(size_t(n) * kMlpRecord * sizeof(float)));
   HIPCHECK(ctx->scratch_e.ensure((kMlpParams + 1) * sizeof(float)));
   HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(ctx->scratch_b.p, X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, st));

@@ -1,28 +1,44 @@
-// Learned surrogate of the CM scattering: an MLP 2 -> 64 -> 64 -> 1 (tanh)
-// mapping (log10 eps, b) to ln sin^2(theta/2), trained on the engine's own
-// universal-potential scattering (scatter_stable<double>).  Inference runs per
-// lane on the VALU: layer 2 either in FP32 FMAs or in BF16 dot products
-// (v_dot2_f32_bf16, no MFMA), weights read with wave-uniform (scalar) loads.
+// Learned surrogate of the CM scattering: an MLP 3 -> 64 -> 64 -> 1 (tanh)
+// mapping (log10 eps, log10 b, b) to ln sin^2(theta/2), trained on the
+// engine's own universal-potential scattering (scatter_stable<double>) over
+// the (eps, b) pairs a cascade in the current target actually draws.
+// Inference runs per lane on the VALU: layer 2 either in FP32 FMAs or in BF16
+// dot products (v_dot2_f32_bf16, no MFMA), weights read with wave-uniform
+// (scalar) loads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace mt {
 
+constexpr int kMlpIn = 3;
 constexpr int kMlpH = 64;
-constexpr int kOffW1 = 0;                     // W1[j][i], j < 64, i < 2
-constexpr int kOffB1 = 128;                   // b1[j]
-constexpr int kOffW2 = 192;                   // W2[k][j] (row k = output neuron)
-constexpr int kOffB2 = kOffW2 + 64 * 64;      // b2[k]
-constexpr int kOffW3 = kOffB2 + 64;           // W3[k]
-constexpr int kOffB3 = kOffW3 + 64;           // b3
-constexpr int kMlpParams = kOffB3 + 1;        // 4417
+constexpr int kOffW1 = 0;                     // W1[j][i], j < 64, i < 3
+constexpr int kOffB1 = kOffW1 + kMlpH * kMlpIn;  // b1[j]
+constexpr int kOffW2 = kOffB1 + kMlpH;        // W2[k][j] (row k = output neuron)
+constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;  // b2[k]
+constexpr int kOffW3 = kOffB2 + kMlpH;        // W3[k]
+constexpr int kOffB3 = kOffW3 + kMlpH;        // b3
+constexpr int kMlpParams = kOffB3 + 1;        // 4481
+// floats per training sample written by the forward/backward kernel
+constexpr int kMlpRecord = 4 + 4 * kMlpH + 1;
 
+// input / output scaling and the trained domain
 struct MlpNorm {
-  float e_mid, e_inv_half;   // x0 = (log10 eps - e_mid) * e_inv_half
-  float b_mid, b_inv_half;   // x1 = (b - b_mid) * b_inv_half
-  float y_off, y_scale;      // ln s2 = y * y_scale + y_off
-  float eps_lo, eps_hi, b_hi;  // domain; outside it the MAGIC path runs
+  float e_mid, e_inv_half;    // x0 = (log10 eps - e_mid) * e_inv_half
+  float lb_mid, lb_inv_half;  // x1 = (log10 max(b, b_floor) - lb_mid) * lb_inv_half
+  float b_mid, b_inv_half;    // x2 = (b - b_mid) * b_inv_half
+  float b_floor;
+  float y_off, y_scale;       // ln s2 = y * y_scale + y_off
+  float eps_lo, eps_hi, b_hi; // domain; outside it the MAGIC path runs
+};
+constexpr int kMlpNormFloats = 12;
+
+// one (ion species, material, element) combination the cascade samples:
+// eps = fi E, b = pmax(E) sqrt(u) / ai, pmax from (a, f, epsdg) (trim.C:88-95, 143-160)
+struct MlpTriple {
+  double a, f, epsdg, ai, fi;
+  double E_hi;  // largest flight energy with fi E <= eps_hi
 };
 
 struct MlpDev {
@@ -47,19 +63,19 @@ __host__ __device__ __forceinline__ uint32_t bf16_bits(float f) {  // round to n
 
 typedef __bf16 mt_bf16x2 __attribute__((ext_vector_type(2)));
 
-// network output y (normalised ln s2) for normalised inputs (x0, x1)
-__device__ __forceinline__ float mlp_eval(const MlpDev& S, float x0, float x1) {
+__device__ __forceinline__ float mlp_h1(const float* __restrict__ p, int j, float x0, float x1, float x2) {
+  return mlp_tanh(p[kOffW1 + 3 * j] * x0 + p[kOffW1 + 3 * j + 1] * x1 + p[kOffW1 + 3 * j + 2] * x2 + p[kOffB1 + j]);
+}
+
+// network output y (normalised ln s2) for normalised inputs
+__device__ __forceinline__ float mlp_eval(const MlpDev& S, float x0, float x1, float x2) {
   const float* __restrict__ p = S.p;
   float y = p[kOffB3];
   if (S.bf16) {
     uint32_t hp[32];
 #pragma unroll
-    for (int jp = 0; jp < 32; ++jp) {
-      const int j = 2 * jp;
-      const float ha = mlp_tanh(p[kOffW1 + 2 * j] * x0 + p[kOffW1 + 2 * j + 1] * x1 + p[kOffB1 + j]);
-      const float hb = mlp_tanh(p[kOffW1 + 2 * j + 2] * x0 + p[kOffW1 + 2 * j + 3] * x1 + p[kOffB1 + j + 1]);
-      hp[jp] = bf16_bits(ha) | (bf16_bits(hb) << 16);
-    }
+    for (int jp = 0; jp < 32; ++jp)
+      hp[jp] = bf16_bits(mlp_h1(p, 2 * jp, x0, x1, x2)) | (bf16_bits(mlp_h1(p, 2 * jp + 1, x0, x1, x2)) << 16);
     const uint32_t* __restrict__ w2 = S.w2bf;
 #pragma unroll 4
     for (int k = 0; k < 64; ++k) {
@@ -73,8 +89,7 @@ __device__ __forceinline__ float mlp_eval(const MlpDev& S, float x0, float x1) {
   } else {
     float h[64];
 #pragma unroll
-    for (int j = 0; j < 64; ++j)
-      h[j] = mlp_tanh(p[kOffW1 + 2 * j] * x0 + p[kOffW1 + 2 * j + 1] * x1 + p[kOffB1 + j]);
+    for (int j = 0; j < 64; ++j) h[j] = mlp_h1(p, j, x0, x1, x2);
 #pragma unroll 4
     for (int k = 0; k < 64; ++k) {
       float acc = p[kOffB2 + k];
@@ -86,12 +101,19 @@ __device__ __forceinline__ float mlp_eval(const MlpDev& S, float x0, float x1) {
   return y;
 }
 
+__host__ __device__ __forceinline__ void mlp_inputs(const MlpNorm& n, float eps, float b, float& x0, float& x1,
+                                                    float& x2) {
+  x0 = (log10f(eps) - n.e_mid) * n.e_inv_half;
+  x1 = (log10f(fmaxf(b, n.b_floor)) - n.lb_mid) * n.lb_inv_half;
+  x2 = (b - n.b_mid) * n.b_inv_half;
+}
+
 // s2 = sin^2(theta/2) from the surrogate, or a negative value outside its domain
 __device__ __forceinline__ float mlp_s2(const MlpDev& S, float eps, float b) {
-  if (!(eps >= S.n.eps_lo && eps <= S.n.eps_hi && b <= S.n.b_hi)) return -1.0f;
-  const float x0 = (__log10f(eps) - S.n.e_mid) * S.n.e_inv_half;
-  const float x1 = (b - S.n.b_mid) * S.n.b_inv_half;
-  const float y = mlp_eval(S, x0, x1);
+  if (!(eps >= S.n.eps_lo && eps <= S.n.eps_hi && b >= 0.0f && b <= S.n.b_hi)) return -1.0f;
+  float x0, x1, x2;
+  mlp_inputs(S.n, eps, b, x0, x1, x2);
+  const float y = mlp_eval(S, x0, x1, x2);
   return fminf(__expf(y * S.n.y_scale + S.n.y_off), 1.0f);
 }
 

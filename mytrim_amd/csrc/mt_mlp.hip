@@ -1,13 +1,13 @@
 // Training and evaluation kernels of the scattering surrogate (mt_mlp.h).
 //
-//   mlp_data      training pairs: eps log-uniform, b uniform, target
-//                 ln s2 from scatter_stable<double> (universal potential),
-//                 one Philox stream per sample
+//   mlp_data      training pairs sampled as the cascade samples (eps, b),
+//                 target ln s2 from scatter_stable<double> (universal
+//                 potential), one Philox stream per sample
 //   mlp_fwdbwd    one lane per sample: forward pass and the per-sample
 //                 back-propagated deltas (stored for the gradient kernel)
 //   mlp_grad      one lane per parameter: batch sum of delta x activation
 //                 (deterministic, no atomics); the last lane sums the loss
-//   mlp_adam      Adam update, step counter and step-size schedule on device
+//   mlp_adam      Adam update with a cosine step-size schedule on device
 // An epoch (every batch: fwdbwd, grad, adam) is captured once in a hipGraph
 // and replayed.
 #include <hip/hip_runtime.h>
@@ -20,23 +20,40 @@
 
 namespace mt {
 
-// per-sample record written by mlp_fwdbwd: x0 x1 dy h1[64] dh1[64] h2[64] dh2[64] e2
-constexpr int kRec = 3 + 4 * 64 + 1;
+// per-sample record written by mlp_fwdbwd: x0 x1 x2 dy h1[64] dh1[64] h2[64] dh2[64] e2
+constexpr int kRec = kMlpRecord;
+constexpr int kRecH1 = 4, kRecD1 = 68, kRecH2 = 132, kRecD2 = 196, kRecDy = 3;
 
-__global__ void __launch_bounds__(256) mlp_data(uint32_t seed, long long n, double lmin, double lmax, double bmax,
-                                                MlpNorm nm, float* X, float* T) {
+// maximum impact parameter of a triple at flight energy E (trim.C:88-95)
+__device__ __forceinline__ double mlp_pmax(const MlpTriple& t, double E) {
+  const double eeg = sqrt(E * t.f * t.epsdg);
+  return t.a / (eeg + sqrt(eeg) + 0.125 * pow(eeg, 0.1));
+}
+
+// Training pairs drawn the way the cascade draws them: a random triple, a
+// flight energy (half log-uniform, half uniform on [E_lo, E_hi of the
+// triple]), eps = fi E and b = pmax(E) sqrt(u) / ai; labels from the FP64
+// MAGIC solve.  One Philox stream per sample.
+__global__ void __launch_bounds__(256) mlp_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri,
+                                                double E_lo, MlpNorm nm, float* X, float* T) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const P4 r = philox10((uint32_t)i, (uint32_t)(i >> 32), 0x5a17u, 0u, seed, 0x6d6c70u);
-    const double le = lmin + (lmax - lmin) * u53(r.x, r.y);
-    const double eps = pow(10.0, le);
-    const double b = bmax * u32d(r.z);
+    const P4 r2 = philox10((uint32_t)i, (uint32_t)(i >> 32), 0x5a18u, 0u, seed, 0x6d6c70u);
+    const int k = min((int)(u32d(r.x) * n_tri), n_tri - 1);
+    const MlpTriple t = tri[k];
+    const double u = u53(r.y, r.z);
+    const double E = (r2.x & 1u) ? E_lo * pow(t.E_hi / E_lo, u) : E_lo + (t.E_hi - E_lo) * u;
+    const double eps = t.fi * E;
+    const double b = mlp_pmax(t, E) * sqrt(u32d(r.w)) / t.ai;
     double s2, ct, st;
     int it;
     scatter_stable<double, 0>(eps, b, s2, ct, st, it);
-    const double ls2 = log(fmax(s2, 1e-300));
-    X[2 * i] = (float)((le - nm.e_mid) * nm.e_inv_half);
-    X[2 * i + 1] = (float)((b - nm.b_mid) * nm.b_inv_half);
-    T[i] = (float)((ls2 - nm.y_off) / nm.y_scale);
+    float x0, x1, x2;
+    mlp_inputs(nm, (float)eps, (float)b, x0, x1, x2);
+    X[3 * i] = x0;
+    X[3 * i + 1] = x1;
+    X[3 * i + 2] = x2;
+    T[i] = (float)((log(fmax(s2, 1e-300)) - nm.y_off) / nm.y_scale);
   }
 }
 
@@ -45,16 +62,16 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd(const float* __restrict__ p, c
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B) return;
   const long long i = base + s;
-  const float x0 = X[2 * i], x1 = X[2 * i + 1];
+  const float x0 = X[3 * i], x1 = X[3 * i + 1], x2 = X[3 * i + 2];
   float h1[64], h2[64];
 #pragma unroll
-  for (int j = 0; j < 64; ++j) h1[j] = tanhf(p[kOffW1 + 2 * j] * x0 + p[kOffW1 + 2 * j + 1] * x1 + p[kOffB1 + j]);
+  for (int j = 0; j < 64; ++j) h1[j] = mlp_h1(p, j, x0, x1, x2);
   float y = p[kOffB3];
   for (int k = 0; k < 64; ++k) {
     float a = p[kOffB2 + k];
 #pragma unroll
     for (int j = 0; j < 64; ++j) a += p[kOffW2 + 64 * k + j] * h1[j];
-    h2[k] = tanhf(a);
+    h2[k] = mlp_tanh(a);
     y += p[kOffW3 + k] * h2[k];
   }
   const float e = y - T[i];
@@ -62,21 +79,22 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd(const float* __restrict__ p, c
   float* r = rec + (size_t)s * kRec;
   r[0] = x0;
   r[1] = x1;
-  r[2] = dy;
+  r[2] = x2;
+  r[kRecDy] = dy;
   float dh1[64];
 #pragma unroll
   for (int j = 0; j < 64; ++j) dh1[j] = 0.0f;
   for (int k = 0; k < 64; ++k) {
     const float d2 = dy * p[kOffW3 + k] * (1.0f - h2[k] * h2[k]);
-    r[3 + 128 + k] = h2[k];
-    r[3 + 192 + k] = d2;
+    r[kRecH2 + k] = h2[k];
+    r[kRecD2 + k] = d2;
 #pragma unroll
     for (int j = 0; j < 64; ++j) dh1[j] += p[kOffW2 + 64 * k + j] * d2;
   }
 #pragma unroll
   for (int j = 0; j < 64; ++j) {
-    r[3 + j] = h1[j];
-    r[3 + 64 + j] = dh1[j] * (1.0f - h1[j] * h1[j]);
+    r[kRecH1 + j] = h1[j];
+    r[kRecD1 + j] = dh1[j] * (1.0f - h1[j] * h1[j]);
   }
   r[kRec - 1] = e * e;
 }
@@ -93,23 +111,23 @@ __global__ void __launch_bounds__(256) mlp_grad(const float* rec, int B, float* 
   // dL/dparam = sum over samples of (delta of the unit it feeds) x (its input)
   int a_off, b_off;  // record offsets: delta index, input index (-1 = bias)
   if (q < kOffB1) {
-    a_off = 3 + 64 + q / 2;
-    b_off = q % 2;
+    a_off = kRecD1 + q / kMlpIn;
+    b_off = q % kMlpIn;
   } else if (q < kOffW2) {
-    a_off = 3 + 64 + (q - kOffB1);
+    a_off = kRecD1 + (q - kOffB1);
     b_off = -1;
   } else if (q < kOffB2) {
     const int k = (q - kOffW2) / 64, j = (q - kOffW2) % 64;
-    a_off = 3 + 192 + k;
-    b_off = 3 + j;
+    a_off = kRecD2 + k;
+    b_off = kRecH1 + j;
   } else if (q < kOffW3) {
-    a_off = 3 + 192 + (q - kOffB2);
+    a_off = kRecD2 + (q - kOffB2);
     b_off = -1;
   } else if (q < kOffB3) {
-    a_off = 2;
-    b_off = 3 + 128 + (q - kOffW3);
+    a_off = kRecDy;
+    b_off = kRecH2 + (q - kOffW3);
   } else {
-    a_off = 2;
+    a_off = kRecDy;
     b_off = -1;
   }
   for (int s = 0; s < B; ++s) {
@@ -119,15 +137,16 @@ __global__ void __launch_bounds__(256) mlp_grad(const float* rec, int B, float* 
   g[q] = acc;
 }
 
-// Adam (Kingma & Ba 2015) with beta1 0.9, beta2 0.999; the step size halves
-// at 1/3 and 2/3 of the run; t counts steps on the device
+// Adam (Kingma & Ba 2015) with beta1 0.9, beta2 0.999; the step size follows
+// a cosine from lr0 down to lr0 / 100 over the run; t counts steps on the device
 __global__ void __launch_bounds__(256) mlp_adam(float* p, const float* g, float* m, float* v, int* t, float lr0,
                                                 int total_steps) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int step = *t + 1;
   if (q < kMlpParams) {
     const float b1 = 0.9f, b2 = 0.999f;
-    const float lr = lr0 * ((3 * step > 2 * total_steps) ? 0.25f : (3 * step > total_steps) ? 0.5f : 1.0f);
+    const float frac = fminf((float)step / (float)total_steps, 1.0f);
+    const float lr = lr0 * (0.01f + 0.495f * (1.0f + cosf(3.14159265f * frac)));
     m[q] = b1 * m[q] + (1.0f - b1) * g[q];
     v[q] = b2 * v[q] + (1.0f - b2) * g[q] * g[q];
     const float mh = m[q] / (1.0f - powf(b1, (float)step));
@@ -157,9 +176,9 @@ __global__ void mlp_pack_bf16(const float* p, uint32_t* w2bf) {
 // ------------------------------------------------------------------ host side
 static int grid_of(long long n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
 
-hipError_t mlp_make_data(uint32_t seed, long long n, double lmin, double lmax, double bmax, const MlpNorm& nm,
-                         float* X, float* T, hipStream_t st) {
-  hipLaunchKernelGGL(mlp_data, dim3(grid_of(n)), dim3(256), 0, st, seed, n, lmin, lmax, bmax, nm, X, T);
+hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo,
+                         const MlpNorm& nm, float* X, float* T, hipStream_t st) {
+  hipLaunchKernelGGL(mlp_data, dim3(grid_of(n)), dim3(256), 0, st, seed, n, tri, n_tri, E_lo, nm, X, T);
   return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
 EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt_ctx_set_tables",
            "mt_default_tables",
            "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_autotune", "mt_trim_flight",
-           "mt_stopping_batch", "mt_scatter_batch", "mt_surrogate_train", "mt_surrogate_install",
+           "mt_stopping_batch", "mt_scatter_batch", "mt_surrogate_norm", "mt_surrogate_train", "mt_surrogate_install",
            "mt_surrogate_eval", "mt_surrogate_grad"]
 
 
@@ -76,12 +76,15 @@ class Tuning(C.Structure):
 
 
 class SurrogateOpts(C.Structure):
-    _fields_ = [("eps_min", C.c_double), ("eps_max", C.c_double), ("b_max", C.c_double),
+    _fields_ = [("n_ions", C.c_int), ("ion_Z", C.POINTER(C.c_int)), ("ion_m", C.POINTER(C.c_double)),
+                ("E_min", C.c_double), ("E_max", C.c_double), ("eps_max", C.c_double),
                 ("n_train", C.c_longlong), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
                 ("seed", C.c_uint32), ("bf16", C.c_int)]
 
 
-N_SURROGATE_PARAMS = 4417
+N_SURROGATE_PARAMS = 4481
+SURROGATE_NORM_KEYS = ["e_mid", "e_inv_half", "lb_mid", "lb_inv_half", "b_mid", "b_inv_half", "b_floor",
+                       "y_off", "y_scale", "eps_lo", "eps_hi", "b_hi"]
 
 
 class Ion(C.Structure):
@@ -128,6 +131,7 @@ def lib():
         fp = C.POINTER(C.c_float)
         so = C.POINTER(SurrogateOpts)
         L.mt_surrogate_train.argtypes = [C.c_void_p, so, fp, dp]
+        L.mt_surrogate_norm.argtypes = [C.c_void_p, so, fp]
         L.mt_surrogate_install.argtypes = [C.c_void_p, so, fp]
         L.mt_surrogate_eval.argtypes = [C.c_void_p, dp, dp, C.c_longlong, dp]
         L.mt_surrogate_grad.argtypes = [C.c_void_p, so, fp, dp, dp, dp, C.c_int, fp, dp]
@@ -274,9 +278,23 @@ class Engine:
 
     # ---------------------------------------------------------- surrogate
     @staticmethod
-    def surrogate_opts(eps_min=1e-7, eps_max=10.0, b_max=40.0, n_train=10_000_000, epochs=12,
-                       batch=4096, lr=2e-3, seed=7, bf16=False) -> SurrogateOpts:
-        return SurrogateOpts(eps_min, eps_max, b_max, n_train, epochs, batch, lr, seed, int(bf16))
+    def surrogate_opts(prob: Problem, E_min: float = 3.0, eps_max: float = 1.0, n_train=4_000_000, epochs=60,
+                       batch=4096, lr=4e-3, seed=7, bf16=False) -> SurrogateOpts:
+        """Options for the cascades of `prob`: its primary species (target atoms
+        are added by the library) and flight energies up to its largest primary
+        energy.  The returned object keeps its arrays alive."""
+        p = prob.primaries
+        Zs, ms = [], []
+        for Z, m in zip(np.atleast_1d(p.Z), np.atleast_1d(p.m)):
+            if (int(Z), float(m)) not in zip(Zs, ms):
+                Zs.append(int(Z))
+                ms.append(float(m))
+        Za = np.array(Zs, np.int32)
+        ma = np.array(ms, np.float64)
+        o = SurrogateOpts(len(Zs), _p(Za, ip), _p(ma, dp), float(E_min), float(np.max(p.E)), float(eps_max),
+                          int(n_train), int(epochs), int(batch), float(lr), int(seed) & 0xFFFFFFFF, int(bf16))
+        o._keep = (Za, ma)
+        return o
 
     @staticmethod
     def surrogate_init(seed: int = 1) -> np.ndarray:
@@ -286,10 +304,16 @@ class Engine:
         def glorot(n_in, n_out):
             a = math.sqrt(6.0 / (n_in + n_out))
             return rng.uniform(-a, a, n_in * n_out).astype(np.float32)
-        p[0:128] = glorot(2, 64)
-        p[192:192 + 4096] = glorot(64, 64)
-        p[4352:4416] = glorot(64, 1)
+        p[0:192] = glorot(3, 64)
+        p[256:256 + 4096] = glorot(64, 64)
+        p[4416:4480] = glorot(64, 1)
         return p
+
+    def surrogate_norm(self, opts: SurrogateOpts) -> dict:
+        """Input / output scaling and domain of the surrogate for these options."""
+        out = np.zeros(len(SURROGATE_NORM_KEYS), np.float32)
+        _check(lib().mt_surrogate_norm(self.h, C.byref(opts), _p(out, C.POINTER(C.c_float))), "mt_surrogate_norm")
+        return dict(zip(SURROGATE_NORM_KEYS, (float(x) for x in out)))
 
     def surrogate_train(self, opts: SurrogateOpts, params: np.ndarray):
         params = np.ascontiguousarray(params, np.float32).copy()

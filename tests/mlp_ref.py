@@ -1,13 +1,15 @@
 """FP64 numpy restatement of the scattering surrogate (layout of mt_mlp.h):
-forward pass, mean-squared loss and back-propagated gradient.  The checker for
-the device kernels mlp_fwdbwd/mlp_grad; itself pinned by central finite
-differences in tests/test_surrogate.py."""
+inputs, forward pass, mean-squared loss and back-propagated gradient.  The
+checker for the device kernels mlp_fwdbwd/mlp_grad; itself pinned by central
+finite differences in tests/test_surrogate.py."""
 from __future__ import annotations
 
 import numpy as np
 
-H = 64
-OFF_W1, OFF_B1, OFF_W2 = 0, 128, 192
+H, NIN = 64, 3
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + H * NIN
+OFF_W2 = OFF_B1 + H
 OFF_B2 = OFF_W2 + H * H
 OFF_W3 = OFF_B2 + H
 OFF_B3 = OFF_W3 + H
@@ -16,12 +18,12 @@ N_PARAMS = OFF_B3 + 1
 
 def unpack(p):
     p = np.asarray(p, np.float64)
-    return (p[OFF_W1:OFF_B1].reshape(H, 2), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(H, H),
+    return (p[OFF_W1:OFF_B1].reshape(H, NIN), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(H, H),
             p[OFF_B2:OFF_W3], p[OFF_W3:OFF_B3], p[OFF_B3])
 
 
 def forward(p, X):
-    """X: (B, 2) normalised inputs -> y (B,)."""
+    """X: (B, 3) normalised inputs -> y (B,)."""
     W1, b1, W2, b2, W3, b3 = unpack(p)
     h1 = np.tanh(X @ W1.T + b1)
     h2 = np.tanh(h1 @ W2.T + b2)
@@ -49,23 +51,21 @@ def loss_grad(p, X, T):
     return float(np.mean(e * e)), g
 
 
-def normalise(eps, b, eps_min, eps_max, b_max):
-    """Inputs as the device normalises them (float32 constants)."""
-    l0, l1 = np.log10(eps_min), np.log10(eps_max)
-    e_mid, e_inv = np.float32(0.5 * (l0 + l1)), np.float32(2.0 / (l1 - l0))
-    b_mid, b_inv = np.float32(0.5 * b_max), np.float32(2.0 / b_max)
-    x0 = ((np.log10(eps) - np.float64(e_mid)) * np.float64(e_inv)).astype(np.float32)
-    x1 = ((b - np.float64(b_mid)) * np.float64(b_inv)).astype(np.float32)
-    return np.stack([x0, x1], 1).astype(np.float64)
+def inputs(norm, eps, b):
+    """The three network inputs in float32, as mlp_inputs forms them."""
+    f = np.float32
+    e = np.asarray(eps, f)
+    bb = np.asarray(b, f)
+    x0 = (np.log10(e) - f(norm["e_mid"])) * f(norm["e_inv_half"])
+    x1 = (np.log10(np.maximum(bb, f(norm["b_floor"]))) - f(norm["lb_mid"])) * f(norm["lb_inv_half"])
+    x2 = (bb - f(norm["b_mid"])) * f(norm["b_inv_half"])
+    return np.stack([x0, x1, x2], 1).astype(np.float64)
 
 
-Y_OFF, Y_SCALE = -12.0, 12.0
+def target(norm, ln_s2):
+    return ((np.asarray(ln_s2) - norm["y_off"]) / norm["y_scale"]).astype(np.float32).astype(np.float64)
 
 
-def target(ln_s2):
-    return ((np.asarray(ln_s2) - Y_OFF) / Y_SCALE).astype(np.float32).astype(np.float64)
-
-
-def s2_of(p, eps, b, eps_min, eps_max, b_max):
-    y = forward(p, normalise(eps, b, eps_min, eps_max, b_max))
-    return np.minimum(np.exp(y * Y_SCALE + Y_OFF), 1.0)
+def s2_of(p, norm, eps, b):
+    y = forward(p, inputs(norm, eps, b))
+    return np.minimum(np.exp(y * norm["y_scale"] + norm["y_off"]), 1.0)

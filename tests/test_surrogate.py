@@ -1,7 +1,8 @@
-"""Scattering surrogate (MLP 2 -> 64 -> 64 -> 1, tanh): the FP64 numpy
+"""Scattering surrogate (MLP 3 -> 64 -> 64 -> 1, tanh): the FP64 numpy
 restatement pinned by finite differences, the device gradient against it, the
-trained model against the MAGIC scattering of the oracle, and an ablation of
-FP32+surrogate cascades against FP32 and FP64 ones."""
+trained model against the MAGIC scattering of the oracle on the collisions a
+cascade draws, and an ablation of FP32+surrogate cascades against FP32 and
+FP64 ones."""
 from __future__ import annotations
 
 import math
@@ -13,8 +14,6 @@ import mlp_ref as M
 import oracle_lib as O
 from mytrim_amd import configs, gpu
 
-EPS_MIN, EPS_MAX, B_MAX = 1e-7, 10.0, 40.0
-
 
 def random_params(seed, scale=1.0):
     rng = np.random.default_rng(seed)
@@ -25,21 +24,25 @@ def random_params(seed, scale=1.0):
     return p
 
 
-def magic_samples(n, seed):
+def cascade_collisions(prob, n, seed, eps_hi):
+    """(eps, b, ln s2) of collisions the oracle's cascades of `prob` draw,
+    restricted to the surrogate's domain, ln s2 from the oracle's MAGIC."""
+    rows = O.scatter_inputs(prob, 400_000)
+    rows = rows[rows[:, 0] <= eps_hi]
     rng = np.random.default_rng(seed)
-    eps = 10.0 ** rng.uniform(math.log10(EPS_MIN), math.log10(EPS_MAX), n)
-    b = rng.uniform(0.0, B_MAX, n)
-    ls2 = np.array([math.log(max(O.scatter(e, bb, 0)[0], 1e-300)) for e, bb in zip(eps, b)])
-    return eps, b, ls2
+    rows = rows[rng.choice(len(rows), min(n, len(rows)), replace=False)]
+    ls2 = np.array([math.log(max(O.scatter(e, b, 0)[0], 1e-300)) for e, b in rows[:, :2]])
+    return rows[:, 0], rows[:, 1], rows[:, 2], ls2
 
 
 # ----------------------------------------------------------------- CPU checks
 def test_parameter_layout():
-    assert M.N_PARAMS == gpu.N_SURROGATE_PARAMS == 4417
+    assert M.N_PARAMS == gpu.N_SURROGATE_PARAMS == 4481
     p = gpu.Engine.surrogate_init(1)
-    assert p.dtype == np.float32 and p.shape == (4417,)
+    assert p.dtype == np.float32 and p.shape == (4481,)
     assert np.all(p[M.OFF_B1:M.OFF_W2] == 0) and p[M.OFF_B3] == 0
     assert np.abs(p[M.OFF_W2:M.OFF_B2]).max() <= math.sqrt(6 / 128)
+    assert np.abs(p[M.OFF_W1:M.OFF_B1]).max() <= math.sqrt(6 / 67)
 
 
 def test_backprop_matches_finite_differences():
@@ -47,11 +50,11 @@ def test_backprop_matches_finite_differences():
     differences of the numpy loss, on every parameter block."""
     rng = np.random.default_rng(5)
     p = random_params(5, scale=1.5)
-    X = rng.uniform(-1, 1, (24, 2))
+    X = rng.uniform(-1, 1, (24, 3))
     T = rng.uniform(-1.5, 0.5, 24)
     _, g = M.loss_grad(p, X, T)
-    idx = np.concatenate([np.arange(0, 128, 7), np.arange(128, 192, 5), np.arange(192, 4288, 97),
-                          np.arange(4288, 4352, 9), np.arange(4352, 4417, 4), [4416]])
+    idx = np.concatenate([np.arange(0, 192, 7), np.arange(192, 256, 5), np.arange(256, 4352, 97),
+                          np.arange(4352, 4416, 9), np.arange(4416, 4480, 4), [4480]])
     h = 1e-6
     for q in idx:
         pp, pm = p.copy(), p.copy()
@@ -65,60 +68,80 @@ def test_numpy_forward_special_cases():
     """W3 = 0 gives y = b3; a single active path reduces to nested tanh."""
     p = np.zeros(M.N_PARAMS)
     p[M.OFF_B3] = 0.7
-    X = np.array([[0.3, -0.2], [1.0, 1.0]])
+    X = np.array([[0.3, -0.2, 0.1], [1.0, 1.0, -1.0]])
     assert np.allclose(M.forward(p, X), 0.7)
-    p[M.OFF_W1 + 0] = 2.0          # W1[0][0]
+    p[M.OFF_W1 + 2] = 2.0          # W1[0][2]
     p[M.OFF_W2 + 0] = -1.5         # W2[0][0]
     p[M.OFF_W3 + 0] = 0.5
-    want = 0.7 + 0.5 * np.tanh(-1.5 * np.tanh(2.0 * X[:, 0]))
+    want = 0.7 + 0.5 * np.tanh(-1.5 * np.tanh(2.0 * X[:, 2]))
     assert np.allclose(M.forward(p, X), want, rtol=1e-15)
 
 
 # ----------------------------------------------------------------- GPU checks
+PROB = configs.c5_c_w(1000, seed=43)
+
+
 @pytest.fixture(scope="module")
 def eng():
     e = gpu.Engine(0)
+    e.set_target(PROB)
     yield e
     e.close()
 
 
 @pytest.fixture(scope="module")
 def trained(eng):
-    opts = gpu.Engine.surrogate_opts(EPS_MIN, EPS_MAX, B_MAX, n_train=1 << 22, epochs=10,
-                                     batch=4096, lr=2e-3, seed=11)
+    opts = gpu.Engine.surrogate_opts(PROB, n_train=4_000_000, epochs=60, seed=11)
     p, loss = eng.surrogate_train(opts, gpu.Engine.surrogate_init(3))
-    print("surrogate epoch loss:", " ".join("%.3g" % x for x in loss))
+    print("surrogate epoch loss:", " ".join("%.3g" % x for x in loss[::6]), "... %.3g" % loss[-1])
     return opts, p, loss
 
 
 @pytest.mark.gpu
+def test_norm_covers_the_cascade_domain(eng):
+    opts = gpu.Engine.surrogate_opts(PROB)
+    n = eng.surrogate_norm(opts)
+    print(n)
+    assert n["eps_lo"] < 2e-6 and n["eps_hi"] == pytest.approx(1.0, rel=1e-6)
+    assert 30.0 < n["b_hi"] < 40.0 and n["b_floor"] == pytest.approx(1e-4 * n["b_hi"], rel=1e-6)
+
+
+@pytest.mark.gpu
 def test_device_gradient_matches_numpy(eng):
-    opts = gpu.Engine.surrogate_opts(EPS_MIN, EPS_MAX, B_MAX)
+    opts = gpu.Engine.surrogate_opts(PROB)
+    norm = eng.surrogate_norm(opts)
     p = random_params(7).astype(np.float32)
-    eps, b, ls2 = magic_samples(777, 3)
+    eps, b, _, ls2 = cascade_collisions(configs.c5_c_w(3, seed=2), 777, 3, norm["eps_hi"])
     g, loss = eng.surrogate_grad(opts, p, eps, b, ls2)
-    X = M.normalise(eps, b, EPS_MIN, EPS_MAX, B_MAX)
-    want_loss, want = M.loss_grad(p.astype(np.float64), X, M.target(ls2))
+    X = M.inputs(norm, eps, b)
+    want_loss, want = M.loss_grad(p.astype(np.float64), X, M.target(norm, ls2))
     err = np.linalg.norm(g - want) / np.linalg.norm(want)
     print("gradient rel. error %.2e, loss %.6g vs %.6g" % (err, loss, want_loss))
-    assert loss == pytest.approx(want_loss, rel=1e-5)
+    assert loss == pytest.approx(want_loss, rel=1e-4)
     assert err < 1e-4
-    for lo, hi in [(0, 128), (128, 192), (192, 4288), (4288, 4352), (4352, 4416)]:
+    for lo, hi in [(0, 192), (192, 256), (256, 4352), (4352, 4416), (4416, 4480)]:
         blk = np.linalg.norm(g[lo:hi] - want[lo:hi]) / np.linalg.norm(want[lo:hi])
         assert blk < 1e-4, (lo, hi, blk)
 
 
 @pytest.mark.gpu
 def test_training_reduces_loss_and_fits_magic(eng, trained):
+    """On collisions the oracle's own C -> W cascades draw: the median error of
+    ln s2 and the bias of the energy transfer (s2 E weighted) it implies."""
     opts, p, loss = trained
+    norm = eng.surrogate_norm(opts)
     assert np.all(np.isfinite(loss))
-    assert loss[-1] < loss[0] / 10
-    eps, b, ls2 = magic_samples(4000, 9)
-    s2n = M.s2_of(p, eps, b, EPS_MIN, EPS_MAX, B_MAX)
-    dl = np.abs(np.log(s2n) - ls2)
-    print("trained surrogate |d ln s2|: median %.3g, p90 %.3g, max %.3g" %
-          (np.median(dl), np.quantile(dl, 0.9), dl.max()))
-    assert np.median(dl) < 0.1
+    assert loss[-1] < loss[0] / 100
+    eps, b, E, ls2 = cascade_collisions(configs.c5_c_w(20, seed=9), 20000, 9, norm["eps_hi"])
+    s2n = M.s2_of(p.astype(np.float64), norm, eps, b)
+    dl = np.log(s2n) - ls2
+    w = np.exp(ls2) * E
+    bias = float((w * (np.exp(dl) - 1)).sum() / w.sum())
+    print("trained surrogate on cascade collisions: |d ln s2| median %.4f p90 %.4f p99 %.4f max %.3f; "
+          "energy-transfer bias %.4f" % (np.median(abs(dl)), np.quantile(abs(dl), .9), np.quantile(abs(dl), .99),
+                                         abs(dl).max(), bias))
+    assert np.median(abs(dl)) < 0.02
+    assert abs(bias) < 0.02
 
 
 @pytest.mark.gpu
@@ -127,9 +150,12 @@ def test_device_inference_matches_numpy(eng, trained, bf16):
     opts, p, _ = trained
     opts.bf16 = int(bf16)
     eng.surrogate_install(opts, p)
-    eps, b, _ = magic_samples(3000, 13)
+    norm = eng.surrogate_norm(opts)
+    rng = np.random.default_rng(13)
+    eps = 10 ** rng.uniform(math.log10(norm["eps_lo"]), math.log10(norm["eps_hi"]), 3000)
+    b = rng.uniform(0, norm["b_hi"], 3000)
     got = eng.surrogate_eval(eps, b)
-    want = M.s2_of(p.astype(np.float64), eps, b, EPS_MIN, EPS_MAX, B_MAX)
+    want = M.s2_of(p.astype(np.float64), norm, eps, b)
     d = np.abs(np.log(got) - np.log(want))
     print("bf16=%d device vs numpy |d ln s2|: median %.2e max %.2e" % (bf16, np.median(d), d.max()))
     if bf16:
@@ -137,7 +163,8 @@ def test_device_inference_matches_numpy(eng, trained, bf16):
     else:
         assert d.max() < 2e-3
     # outside the domain the device reports -1 (the MAGIC solve runs instead)
-    out = eng.surrogate_eval(np.array([EPS_MAX * 2, 1e-9, 1.0]), np.array([1.0, 1.0, B_MAX * 1.5]))
+    out = eng.surrogate_eval(np.array([norm["eps_hi"] * 2, norm["eps_lo"] / 2, 0.5]),
+                             np.array([1.0, 1.0, norm["b_hi"] * 1.5]))
     assert np.all(out == -1.0)
 
 
@@ -146,35 +173,41 @@ def test_surrogate_cascade_ablation(eng, trained):
     """C -> W at 2 MeV: FP32 + surrogate against FP32 MAGIC and FP64 MAGIC on
     the same histories (vacancies/ion, electronic share, mean range)."""
     opts, p, _ = trained
-    opts.bf16 = 1
-    eng.surrogate_install(opts, p)
-    prob = configs.c5_c_w(1000, seed=43)
     res = {}
-    for name, ar in [("fp64", gpu.MT_FP64), ("fp32", gpu.MT_FP32), ("surr", gpu.MT_FP32_SURROGATE)]:
-        res[name] = eng.run(prob, per_history=True, arith=ar)
-    n = prob.n_hist
+    for bf16 in (0, 1):
+        opts.bf16 = bf16
+        eng.surrogate_install(opts, p)
+        res["surr_bf16" if bf16 else "surr_fp32"] = eng.run(PROB, per_history=True, arith=gpu.MT_FP32_SURROGATE)
+    res["fp64"] = eng.run(PROB, per_history=True, arith=gpu.MT_FP64)
+    res["fp32"] = eng.run(PROB, per_history=True, arith=gpu.MT_FP32)
+    n = PROB.n_hist
     a = res["fp64"]
     for name, r in res.items():
         print("%s: %.1f ms, %.4g coll, surrogate %.4g, vac/ion %.2f, Eel/E0 %.4f, range %.1f" % (
             name, r["kernel_ms"], r["n_coll"], r["surrogate"], r["vac"] / n, r["Eel"] / r["E0"],
             r["range_sum"] / r["range_n"]))
-    s = res["surr"]
-    assert 0 < s["surrogate"] < s["n_coll"]
     assert res["fp64"]["surrogate"] == 0 and res["fp32"]["surrogate"] == 0
-    sig = math.sqrt((np.var(a["h_vac"]) + np.var(s["h_vac"])) / n)
-    assert abs(s["vac"] - a["vac"]) / n <= max(0.05 * a["vac"] / n, 3 * sig)
-    assert s["Eel"] / s["E0"] == pytest.approx(a["Eel"] / a["E0"], rel=0.03)
-    tot = s["Eel"] + s["Enuc"] + s["Eunf"] + s["Elost"] + s["Edrop"] + s["Evac"]
-    assert tot == pytest.approx(s["E0"], rel=1e-11)
+    for name in ("surr_fp32", "surr_bf16"):
+        s = res[name]
+        assert 0 < s["surrogate"] < s["n_coll"]
+        sig = math.sqrt((np.var(a["h_vac"]) + np.var(s["h_vac"])) / n)
+        assert abs(s["vac"] - a["vac"]) / n <= max(0.03 * a["vac"] / n, 3 * sig), name
+        assert s["Eel"] / s["E0"] == pytest.approx(a["Eel"] / a["E0"], rel=0.01), name
+        tot = s["Eel"] + s["Enuc"] + s["Eunf"] + s["Elost"] + s["Edrop"] + s["Evac"]
+        assert tot == pytest.approx(s["E0"], rel=1e-11)
 
 
 @pytest.mark.gpu
-def test_surrogate_requires_install_and_universal_potential(eng):
+def test_surrogate_requires_install_and_universal_potential():
     e2 = gpu.Engine(0)
     try:
         with pytest.raises(RuntimeError):
             e2.run(configs.c5_c_w(4, seed=1), arith=gpu.MT_FP32_SURROGATE)
         with pytest.raises(RuntimeError):
             e2.surrogate_eval(np.array([1.0]), np.array([1.0]))
+        moliere = configs.c5_c_w(4, seed=1).with_(potential=1)
+        e2.set_target(moliere)
+        with pytest.raises(RuntimeError):
+            e2.surrogate_norm(gpu.Engine.surrogate_opts(moliere))
     finally:
         e2.close()
