@@ -103,7 +103,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=int, default=1000000)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--autotune-sample", type=int, default=1000, help="0 = default launch shape")
+    ap.add_argument("--autotune-sample", type=int, default=10000, help="0 = default or given launch shape")
+    ap.add_argument("--block", type=int, default=0, help="fixed launch shape (with --autotune-sample 0)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--stack-cap", type=int, default=0)
     ap.add_argument("--variant-histories", type=int, default=100000,
                     help="histories for the FP32 and FP32+surrogate lines (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=4_000_000)
@@ -128,7 +131,11 @@ def main():
     eng = gpu.Engine(0)
     eng.set_target(prob)
     tuning = None
-    if args.autotune_sample > 0:
+    if args.block or args.blocks_per_cu or args.stack_cap:
+        eng.tune(block=args.block, blocks_per_cu=args.blocks_per_cu, stack_cap=args.stack_cap)
+        tuning = {"block": args.block, "blocks_per_cu": args.blocks_per_cu, "stack_cap": args.stack_cap,
+                  "fixed": True}
+    elif args.autotune_sample > 0:
         best, rate = eng.autotune(prob.with_(hist0=2 * args.histories, n_hist=args.autotune_sample),
                                   sample=args.autotune_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,

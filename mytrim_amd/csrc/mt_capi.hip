@@ -456,8 +456,10 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   HIPCHECK(hipMemcpyAsync(ctx->pka_dir.p, pk->dir, 3 * n_entries * sizeof(double), hipMemcpyHostToDevice, st));
 
   // launch shape
+  // defaults: the shape measured best for Cu -> Cu cascades (one wave per
+  // SIMD, 384-entry LDS stacks)
   const int block = ctx->tune.block ? ctx->tune.block : 256;
-  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 64;
+  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 384;
   const int nwb = block / 64;
   const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
   size_t lds = blob.size() + size_t(nwb) * cascade_stack_bytes(cap);
@@ -470,7 +472,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   const bool perhist = o->per_history != 0;
   int bpc = 0;
   HIPCHECK(cascade_occupancy(o->arith, pot, ctx->tgt.geo.kind, perhist, block, lds, &bpc));
-  if (ctx->tune.blocks_per_cu > 0) bpc = std::min(bpc, ctx->tune.blocks_per_cu);
+  bpc = std::min(bpc, ctx->tune.blocks_per_cu > 0 ? ctx->tune.blocks_per_cu : kDefaultWavesPerCu * 64 / block);
   if (bpc < 1) return fail(MT_E_LIMIT, "cascade kernel does not fit on a CU");
   const int grid = bpc * ctx->cus;
   const int waves = grid * nwb;
@@ -746,21 +748,23 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   mt_run_opts ro = *o;
   ro.per_history = 0;
   const mt_tuning keep = ctx->tune;
-  // candidate shapes: wave count per block x LDS stack depth x resident blocks per CU
-  // (0 = as many as fit); queue capacity stays at its default
+  // candidate shapes (threads per block, resident blocks per CU, LDS stack
+  // entries per wave) around the region measured best on MI355X: one or two
+  // waves per SIMD with deep stacks; queue capacity stays at its default
+  static const int kCand[][3] = {{256, 1, 256}, {256, 1, 384}, {256, 1, 512}, {128, 2, 256}, {128, 2, 384},
+                                 {128, 3, 256}, {128, 4, 192}, {64, 4, 384}, {64, 6, 256}, {64, 8, 192},
+                                 {256, 2, 192}, {64, 4, 512}};
   std::vector<mt_tuning> cand;
-  for (int block : {64, 128, 256})
-    for (int cap : {32, 64, 128})
-      for (int bpc : {0, 1, 2}) {
-        mt_tuning t{};
-        t.block = block;
-        t.stack_cap = cap;
-        t.blocks_per_cu = bpc;
-        t.queue_cap = keep.queue_cap;
-        t.lds_hist = keep.lds_hist;
-        t.graph = keep.graph;
-        cand.push_back(t);
-      }
+  for (const auto& c : kCand) {
+    mt_tuning t{};
+    t.block = c[0];
+    t.blocks_per_cu = c[1];
+    t.stack_cap = c[2];
+    t.queue_cap = keep.queue_cap;
+    t.lds_hist = keep.lds_hist;
+    t.graph = keep.graph;
+    cand.push_back(t);
+  }
   double best_r = -1.0;
   mt_tuning best_t = keep;
   std::string first_err;

@@ -104,6 +104,8 @@ struct Params {
 
 // launch bound of the persistent kernel (threads per workgroup)
 constexpr int kCascadeMaxBlock = 256;
+// resident waves per CU when the tuning leaves it open (one per SIMD)
+constexpr int kDefaultWavesPerCu = 4;
 
 // device spans zeroed before a run
 constexpr int kMaxSpans = 16;
