@@ -175,7 +175,7 @@ struct mt_ctx {
   HBuf stage_res, stage_hist;
   RunGraph graph;
   DBuf per[8];
-  DBuf q[12];
+  DBuf q[kQueueArrays];
   uint64_t qcap = 0;
   // single-flight path: device tables per ion species (Z, m), event buffers
   std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
@@ -493,8 +493,8 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
                       ? uint64_t(ctx->tune.queue_cap)
                       : next_pow2(uint64_t(std::min(std::max(want, 65536.0), double(uint64_t(1) << 27))));
   if (ctx->qcap != qcap) {
-    const size_t sz[12] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 8, 4};
-    for (int i = 0; i < 12; ++i) HIPCHECK(ctx->q[i].ensure(qcap * sz[i]));
+    const size_t sz[kQueueArrays] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 8};
+    for (int i = 0; i < kQueueArrays; ++i) HIPCHECK(ctx->q[i].ensure(qcap * sz[i]));
     ctx->qcap = qcap;
   }
   int log2cap = 0;
@@ -521,7 +521,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     zs.bytes[zs.n] = bytes;
     ++zs.n;
   };
-  span(ctx->q[11].p, qcap * sizeof(uint32_t));
+  span(ctx->q[kQueueReady].p, qcap * sizeof(uint64_t));
   span(ctx->res.p, kResWords * sizeof(unsigned long long));
   span(ctx->hist_all.p, hist_bytes);
   span(ctx->wave_energy.p, size_t(waves) * 2 * kNumEnergy * sizeof(double));
@@ -557,18 +557,17 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   unsigned long long* res = ctx->res.as<unsigned long long>();
   P.ctrl = res + kResCtrl;
   Queue& Q = P.q;
-  Q.px = ctx->q[0].as<double>();
-  Q.py = ctx->q[1].as<double>();
-  Q.pz = ctx->q[2].as<double>();
-  Q.dx = ctx->q[3].as<double>();
-  Q.dy = ctx->q[4].as<double>();
-  Q.dz = ctx->q[5].as<double>();
-  Q.E = ctx->q[6].as<double>();
-  Q.meta = ctx->q[7].as<uint32_t>();
-  Q.tag = ctx->q[8].as<int>();
+  Q.px = ctx->q[0].as<uint64_t>();
+  Q.py = ctx->q[1].as<uint64_t>();
+  Q.pz = ctx->q[2].as<uint64_t>();
+  Q.dx = ctx->q[3].as<uint64_t>();
+  Q.dy = ctx->q[4].as<uint64_t>();
+  Q.dz = ctx->q[5].as<uint64_t>();
+  Q.E = ctx->q[6].as<uint64_t>();
+  Q.key = ctx->q[7].as<uint64_t>();
+  Q.mt = ctx->q[8].as<uint64_t>();
   Q.hist = ctx->q[9].as<uint32_t>();
-  Q.key = ctx->q[10].as<uint64_t>();
-  Q.ready = ctx->q[11].as<uint32_t>();
+  Q.ready = ctx->q[kQueueReady].as<uint64_t>();
   Q.mask = qcap - 1;
   Q.log2cap = log2cap;
   P.admit_limit = (long long)(qcap / 4);

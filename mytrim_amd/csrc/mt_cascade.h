@@ -8,17 +8,22 @@
 
 namespace mt {
 
-// HBM recoil queue: structure-of-arrays ring with a lap tag per slot.
+// HBM recoil queue: structure-of-arrays ring.  Every field is written and
+// read with agent-scope relaxed atomics (coherent across the XCDs without
+// L2 write-back / invalidate fences); the ready word of a slot holds the lap
+// (position / capacity + 1) in its high half and a checksum of the entry in
+// its low half, so a reader that sees the lap but stale fields re-reads.
 struct Queue {
-  double *px, *py, *pz, *dx, *dy, *dz, *E;
-  uint32_t* meta;   // species | (gen << 8)
-  int* tag;
-  uint32_t* hist;
+  uint64_t *px, *py, *pz, *dx, *dy, *dz, *E;  // FP64 bit patterns
   uint64_t* key;
-  uint32_t* ready;  // lap number (position / capacity + 1) once the slot is written
+  uint64_t* mt;     // meta << 32 | tag, meta = species | (gen << 8)
+  uint32_t* hist;
+  uint64_t* ready;
   uint64_t mask;    // capacity - 1
   int log2cap;
 };
+constexpr int kQueueArrays = 11;
+constexpr int kQueueReady = 10;  // index of the ready words among the arrays
 
 // control words (one HBM allocation); the words waves contend on sit on
 // cache lines (128 B = 16 words) of their own

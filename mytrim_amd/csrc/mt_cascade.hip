@@ -100,6 +100,32 @@ struct StackView {  // one wave's LDS stack, structure of arrays
 
 __host__ __device__ constexpr int stack_bytes(int cap) { return ((kStackEntryBytes - 4) * cap + 15) & ~15; }
 
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// checksum of a queue entry and its lap (multiply / xor-shift mixing)
+__device__ __forceinline__ uint32_t entry_sum(uint32_t lap, const uint64_t (&w)[9], uint32_t hist) {
+  uint64_t h = 0x9E3779B97F4A7C15ull * (uint64_t)(lap + 1u);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    h = (h ^ w[i]) * 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+  }
+  h = (h ^ hist) * 0x94D049BB133111EBull;
+  h ^= h >> 29;
+  return (uint32_t)(h >> 32);
+}
+
 // Free path, impact parameter, target atom, electronic loss and scattering
 // of one collision (trim.C:88-272) in precision R.  FP64 follows the
 // reference's expressions; FP32 evaluates the same quantities in single
@@ -297,8 +323,24 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
               slot = pos & P.q.mask;
               const uint32_t lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
               const unsigned long long t_start = (unsigned long long)wall_clock64();
-              while (__hip_atomic_load(&P.q.ready[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-                     lap) {
+              uint64_t w[9];
+              uint32_t wh = 0;
+              // wait for the lap, then for fields that match the checksum
+              while (true) {
+                const uint64_t rw = ld_agent(&P.q.ready[slot]);
+                if ((uint32_t)(rw >> 32) == lap) {
+                  w[0] = ld_agent(&P.q.px[slot]);
+                  w[1] = ld_agent(&P.q.py[slot]);
+                  w[2] = ld_agent(&P.q.pz[slot]);
+                  w[3] = ld_agent(&P.q.dx[slot]);
+                  w[4] = ld_agent(&P.q.dy[slot]);
+                  w[5] = ld_agent(&P.q.dz[slot]);
+                  w[6] = ld_agent(&P.q.E[slot]);
+                  w[7] = ld_agent(&P.q.key[slot]);
+                  w[8] = ld_agent(&P.q.mt[slot]);
+                  wh = ld_agent(&P.q.hist[slot]);
+                  if (entry_sum(lap, w, wh) == (uint32_t)rw) break;
+                }
                 __builtin_amdgcn_s_sleep(1);
                 if ((unsigned long long)wall_clock64() - t_start > P.max_wait_ticks) {
                   err |= kErrTimeout;
@@ -310,23 +352,22 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
                   break;
                 }
               }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (mine) {
-              const int j = scount + lane;
-              const uint32_t m = P.q.meta[slot];
-              if ((int)(m & 0xffu) >= T.n_species) err |= kErrBadEntry;
-              S.px[j] = P.q.px[slot];
-              S.py[j] = P.q.py[slot];
-              S.pz[j] = P.q.pz[slot];
-              S.dx[j] = P.q.dx[slot];
-              S.dy[j] = P.q.dy[slot];
-              S.dz[j] = P.q.dz[slot];
-              S.E[j] = P.q.E[slot];
-              S.key[j] = P.q.key[slot];
-              S.meta[j] = (int)(m & 0xffu) < T.n_species ? m : 0u;
-              S.tag[j] = P.q.tag[slot];
-              S.hist[j] = P.q.hist[slot];
+              if (mine) {
+                const int j = scount + lane;
+                const uint32_t m = (uint32_t)(w[8] >> 32);
+                if ((int)(m & 0xffu) >= T.n_species) err |= kErrBadEntry;
+                S.px[j] = __longlong_as_double((long long)w[0]);
+                S.py[j] = __longlong_as_double((long long)w[1]);
+                S.pz[j] = __longlong_as_double((long long)w[2]);
+                S.dx[j] = __longlong_as_double((long long)w[3]);
+                S.dy[j] = __longlong_as_double((long long)w[4]);
+                S.dz[j] = __longlong_as_double((long long)w[5]);
+                S.E[j] = __longlong_as_double((long long)w[6]);
+                S.key[j] = w[7];
+                S.meta[j] = (int)(m & 0xffu) < T.n_species ? m : 0u;
+                S.tag[j] = (int)(uint32_t)w[8];
+                S.hist[j] = wh;
+              }
             }
             const uint64_t bad = __ballot((err & (kErrTimeout | kErrBadEntry)) != 0u);
             if (bad == 0ull) {
@@ -642,25 +683,33 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
         }
         const bool over = push && r >= room;
         uint64_t slot = 0;
-        uint32_t lap = 0;
+        uint32_t lap = 0, sum = 0;
         if (over) {
           const uint64_t pos = base + (uint64_t)(r - room);
           slot = pos & P.q.mask;
           lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
-          P.q.px[slot] = rx;
-          P.q.py[slot] = ry;
-          P.q.pz[slot] = rz;
-          P.q.dx[slot] = rdx;
-          P.q.dy[slot] = rdy;
-          P.q.dz[slot] = rdz;
-          P.q.E[slot] = rE;
-          P.q.key[slot] = rkey;
-          P.q.meta[slot] = meta;
-          P.q.tag[slot] = rtag;
-          P.q.hist[slot] = F.hist;
+          const uint64_t w[9] = {(uint64_t)__double_as_longlong(rx),  (uint64_t)__double_as_longlong(ry),
+                                 (uint64_t)__double_as_longlong(rz),  (uint64_t)__double_as_longlong(rdx),
+                                 (uint64_t)__double_as_longlong(rdy), (uint64_t)__double_as_longlong(rdz),
+                                 (uint64_t)__double_as_longlong(rE),  rkey,
+                                 ((uint64_t)meta << 32) | (uint32_t)rtag};
+          st_agent(&P.q.px[slot], w[0]);
+          st_agent(&P.q.py[slot], w[1]);
+          st_agent(&P.q.pz[slot], w[2]);
+          st_agent(&P.q.dx[slot], w[3]);
+          st_agent(&P.q.dy[slot], w[4]);
+          st_agent(&P.q.dz[slot], w[5]);
+          st_agent(&P.q.E[slot], w[6]);
+          st_agent(&P.q.key[slot], w[7]);
+          st_agent(&P.q.mt[slot], w[8]);
+          st_agent(&P.q.hist[slot], F.hist);
+          sum = entry_sum(lap, w, F.hist);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (over) __hip_atomic_store(&P.q.ready[slot], lap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // this wave's outstanding memory operations (the entry and the
+        // history's flight-count increment) complete before the ready word;
+        // readers check the checksum in any case
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (over) st_agent(&P.q.ready[slot], ((uint64_t)lap << 32) | sum);
       }
     }
 
