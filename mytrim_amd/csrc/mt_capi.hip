@@ -568,8 +568,14 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   Q.mt = ctx->q[8].as<uint64_t>();
   Q.hist = ctx->q[9].as<uint32_t>();
   Q.ready = ctx->q[kQueueReady].as<uint64_t>();
-  Q.mask = qcap - 1;
-  Q.log2cap = log2cap;
+  // shards of at least 2^12 slots, up to 16
+  int nq = kMaxShards;
+  while (nq > 1 && (qcap / uint64_t(nq)) < 4096) nq >>= 1;
+  int lq = 0;
+  while ((1 << lq) < nq) ++lq;
+  Q.nq = nq;
+  Q.sub_log2 = log2cap - lq;
+  Q.sub_mask = (uint64_t(1) << Q.sub_log2) - 1;
   P.admit_limit = (long long)(qcap / 4);
   P.budget = double(qcap) / 2.0;
   P.f_init = f_init;

@@ -19,9 +19,12 @@ struct Queue {
   uint64_t* mt;     // meta << 32 | tag, meta = species | (gen << 8)
   uint32_t* hist;
   uint64_t* ready;
-  uint64_t mask;    // capacity - 1
-  int log2cap;
+  // the ring is split into nq shards of 2^sub_log2 slots; shard s owns slots
+  // [s << sub_log2, (s + 1) << sub_log2) and its own head / tail words
+  uint64_t sub_mask;
+  int sub_log2, nq;
 };
+constexpr int kMaxShards = 16;
 constexpr int kQueueArrays = 11;
 constexpr int kQueueReady = 10;  // index of the ready words among the arrays
 
@@ -30,16 +33,17 @@ constexpr int kQueueReady = 10;  // index of the ready words among the arrays
 enum Ctrl {
   kState = 0,           // epoch << 32 | live waves << 16 | idle waves
   kPkaNext = 16,        // next primary to start
-  kHead = 32,           // next queue position to pop
-  kTail = 48,           // next queue position to push
   kError = 64,          // error bits
   kQueueOps = 65,       // wave-level queue transfers
   kQueuePeak = 80,      // largest tail - head seen at a push
   kLive = 96,           // admitted cascades not yet complete
   kCascDone = 97,       // completed cascades
   kFlightsEnded = 112,
-  kNumCtrl = 128
+  kQBase = 128,         // shard s: next position to pop at kQBase + 32 s, to push at + 16
+  kNumCtrl = kQBase + 32 * kMaxShards
 };
+__host__ __device__ constexpr int kQHead(int s) { return kQBase + 32 * s; }
+__host__ __device__ constexpr int kQTail(int s) { return kQBase + 32 * s + 16; }
 
 enum ErrBits {
   kErrQueueFull = 1,

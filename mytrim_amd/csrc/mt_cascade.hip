@@ -233,6 +233,12 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
   int scount = 0;          // entries in this wave's LDS stack
   bool wave_idle = false;  // registered idle
   int backoff = 1;         // idle sleep rounds
+  // home queue shard: waves of one XCD (blocks are dealt to the 8 XCDs in
+  // turn) share their shards
+  const int per_xcd = P.q.nq >= 8 ? P.q.nq / 8 : 1;
+  const int home = (P.q.nq >= 8 ? (int)(blockIdx.x & 7u) * per_xcd + wid % per_xcd
+                                : (int)(blockIdx.x * nwb + wid)) & (P.q.nq - 1);
+  int victim = home;       // last shard this wave stole from
   int poll_skip = 0;       // iterations before the shared queue is polled again
   long long peak_seen = 0; // largest queue fill this wave has reported
   unsigned long long fe_pending = 0;  // ended flights not yet added to kFlightsEnded
@@ -276,8 +282,17 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
       if (scount < n_idle && (poll_skip == 0 || n_idle == 64)) {
         // one lane loads one entry, so a batch holds at most 64
         const int want = min(64, min(cap - scount, max(n_idle, cap / 2) - scount));
-        const uint64_t h0 = ld_relaxed(&ctrl[kHead]);
-        const uint64_t t0 = ld_relaxed(&ctrl[kTail]);
+        // the home shard first, then up to three others (rotating victim)
+        int qs = home;
+        uint64_t h0 = ld_relaxed(&ctrl[kQHead(qs)]);
+        uint64_t t0 = ld_relaxed(&ctrl[kQTail(qs)]);
+        for (int v = 0; v < 3 && t0 <= h0 && P.q.nq > 1; ++v) {
+          victim = (victim + 1) & (P.q.nq - 1);
+          if (victim == home) victim = (victim + 1) & (P.q.nq - 1);
+          qs = victim;
+          h0 = ld_relaxed(&ctrl[kQHead(qs)]);
+          t0 = ld_relaxed(&ctrl[kQTail(qs)]);
+        }
         const bool may_pop = t0 > h0;
         bool may_pka = !may_pop && ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
         long long n_adm = 0;
@@ -302,14 +317,14 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
             for (int tries = 0; tries < 4 && t > h; ++tries) {
               const uint64_t k = min((uint64_t)want, t - h);
               unsigned long long expect = h;
-              if (__hip_atomic_compare_exchange_strong(&ctrl[kHead], &expect, h + k, __ATOMIC_RELAXED,
+              if (__hip_atomic_compare_exchange_strong(&ctrl[kQHead(qs)], &expect, h + k, __ATOMIC_RELAXED,
                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 got = k;
                 base = h;
                 break;
               }
               h = expect;  // the head another wave moved it to
-              t = ld_relaxed(&ctrl[kTail]);
+              t = ld_relaxed(&ctrl[kQTail(qs)]);
             }
           }
           got = bcast64(got, 0);
@@ -320,8 +335,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
             uint64_t slot = 0;
             if (mine) {
               const uint64_t pos = base + (uint64_t)lane;
-              slot = pos & P.q.mask;
-              const uint32_t lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
+              slot = ((uint64_t)qs << P.q.sub_log2) | (pos & P.q.sub_mask);
+              const uint32_t lap = (uint32_t)(pos >> P.q.sub_log2) + 1u;
               const unsigned long long t_start = (unsigned long long)wall_clock64();
               uint64_t w[9];
               uint32_t wh = 0;
@@ -446,10 +461,10 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
         const uint64_t s1 = ld_relaxed(&ctrl[kState]);
         if ((s1 & 0xffffull) == ((s1 >> 16) & 0xffffull)) {
           const uint64_t pn = ld_relaxed(&ctrl[kPkaNext]);
-          const uint64_t h = ld_relaxed(&ctrl[kHead]);
-          const uint64_t t = ld_relaxed(&ctrl[kTail]);
+          bool empty = true;
+          for (int q = 0; q < P.q.nq; ++q) empty = empty && ld_relaxed(&ctrl[kQHead(q)]) == ld_relaxed(&ctrl[kQTail(q)]);
           const uint64_t s2 = ld_relaxed(&ctrl[kState]);
-          if (s1 == s2 && h == t && pn >= (unsigned long long)P.n_hist) break;
+          if (s1 == s2 && empty && pn >= (unsigned long long)P.n_hist) break;
         }
         if ((unsigned long long)wall_clock64() - idle_since > P.max_wait_ticks) {
           err |= kErrTimeout;
@@ -665,8 +680,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
         ++c_qops;
         uint64_t base = 0, head = 0;
         if (lane == 0) {
-          base = atomicAdd(&ctrl[kTail], (unsigned long long)n_over);
-          head = ld_relaxed(&ctrl[kHead]);
+          base = atomicAdd(&ctrl[kQTail(home)], (unsigned long long)n_over);
+          head = ld_relaxed(&ctrl[kQHead(home)]);
         }
         base = bcast64(base, 0);
         head = bcast64(head, 0);
@@ -677,7 +692,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
           peak_seen = fill;
           if (lane == 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
         }
-        if (fill > (long long)P.q.mask) {  // abort: every wave leaves, the caller reports the error
+        if (fill > (long long)P.q.sub_mask) {  // abort: every wave leaves, the caller reports the error
           err |= kErrQueueFull;
           if (lane == 0) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
         }
@@ -686,8 +701,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
         uint32_t lap = 0, sum = 0;
         if (over) {
           const uint64_t pos = base + (uint64_t)(r - room);
-          slot = pos & P.q.mask;
-          lap = (uint32_t)(pos >> P.q.log2cap) + 1u;
+          slot = ((uint64_t)home << P.q.sub_log2) | (pos & P.q.sub_mask);
+          lap = (uint32_t)(pos >> P.q.sub_log2) + 1u;
           const uint64_t w[9] = {(uint64_t)__double_as_longlong(rx),  (uint64_t)__double_as_longlong(ry),
                                  (uint64_t)__double_as_longlong(rz),  (uint64_t)__double_as_longlong(rdx),
                                  (uint64_t)__double_as_longlong(rdy), (uint64_t)__double_as_longlong(rdz),
