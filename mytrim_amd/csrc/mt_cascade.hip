@@ -113,18 +113,14 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// checksum of a queue entry and its lap (multiply / xor-shift mixing)
-__device__ __forceinline__ uint32_t entry_sum(uint32_t lap, const uint64_t (&w)[9], uint32_t hist) {
-  uint64_t h = 0x9E3779B97F4A7C15ull * (uint64_t)(lap + 1u);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    h = (h ^ w[i]) * 0xBF58476D1CE4E5B9ull;
-    h ^= h >> 31;
-  }
-  h = (h ^ hist) * 0x94D049BB133111EBull;
-  h ^= h >> 29;
-  return (uint32_t)(h >> 32);
+// checksum of a queue entry and its lap (multiply / xor-shift mixing),
+// accumulated one field at a time
+__device__ __forceinline__ uint64_t sum_init(uint32_t lap) { return 0x9E3779B97F4A7C15ull * (uint64_t)(lap + 1u); }
+__device__ __forceinline__ uint64_t sum_add(uint64_t h, uint64_t w) {
+  h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
+  return h ^ (h >> 31);
 }
+__device__ __forceinline__ uint32_t sum_end(uint64_t h) { return (uint32_t)(h >> 32); }
 
 // Free path, impact parameter, target atom, electronic loss and scattering
 // of one collision (trim.C:88-272) in precision R.  FP64 follows the
@@ -196,7 +192,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
 }  // namespace
 
 template <int ARITH, int POT, int GEO, bool PERHIST>
-__global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
+__global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) {
   extern __shared__ __align__(16) unsigned char lds[];
   {
     const uint4* src = (const uint4*)P.tables;
@@ -259,7 +255,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
                      c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0, c_surr = 0;
   unsigned long long c_qops = 0;
   // per-lane accumulators
-  long long l_newton = 0, l_kpvac = 0;
+  // (32-bit per lane: a lane runs at most a few 1e7 collisions per launch)
+  unsigned l_newton32 = 0, l_kpvac32 = 0;
   double eel_s = 0.0, eel_e = 0.0, enuc_s = 0.0, enuc_e = 0.0;
   unsigned err = 0;
 
@@ -338,23 +335,32 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
               slot = ((uint64_t)qs << P.q.sub_log2) | (pos & P.q.sub_mask);
               const uint32_t lap = (uint32_t)(pos >> P.q.sub_log2) + 1u;
               const unsigned long long t_start = (unsigned long long)wall_clock64();
-              uint64_t w[9];
-              uint32_t wh = 0;
-              // wait for the lap, then for fields that match the checksum
+              const int j = scount + lane;
+              // wait for the lap, then for fields that match the checksum;
+              // fields go straight to this lane's stack slot
               while (true) {
                 const uint64_t rw = ld_agent(&P.q.ready[slot]);
                 if ((uint32_t)(rw >> 32) == lap) {
-                  w[0] = ld_agent(&P.q.px[slot]);
-                  w[1] = ld_agent(&P.q.py[slot]);
-                  w[2] = ld_agent(&P.q.pz[slot]);
-                  w[3] = ld_agent(&P.q.dx[slot]);
-                  w[4] = ld_agent(&P.q.dy[slot]);
-                  w[5] = ld_agent(&P.q.dz[slot]);
-                  w[6] = ld_agent(&P.q.E[slot]);
-                  w[7] = ld_agent(&P.q.key[slot]);
-                  w[8] = ld_agent(&P.q.mt[slot]);
-                  wh = ld_agent(&P.q.hist[slot]);
-                  if (entry_sum(lap, w, wh) == (uint32_t)rw) break;
+                  uint64_t h = sum_init(lap), w;
+                  w = ld_agent(&P.q.px[slot]); S.px[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.py[slot]); S.py[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.pz[slot]); S.pz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.dx[slot]); S.dx[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.dy[slot]); S.dy[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.dz[slot]); S.dz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.E[slot]); S.E[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&P.q.key[slot]); S.key[j] = w; h = sum_add(h, w);
+                  w = ld_agent(&P.q.mt[slot]); h = sum_add(h, w);
+                  const uint32_t m = (uint32_t)(w >> 32);
+                  S.meta[j] = (int)(m & 0xffu) < T.n_species ? m : 0u;
+                  S.tag[j] = (int)(uint32_t)w;
+                  const uint32_t wh = ld_agent(&P.q.hist[slot]);
+                  S.hist[j] = wh;
+                  h = sum_add(h, wh);
+                  if (sum_end(h) == (uint32_t)rw) {
+                    if ((int)(m & 0xffu) >= T.n_species) err |= kErrBadEntry;
+                    break;
+                  }
                 }
                 __builtin_amdgcn_s_sleep(1);
                 if ((unsigned long long)wall_clock64() - t_start > P.max_wait_ticks) {
@@ -366,22 +372,6 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
                   mine = false;
                   break;
                 }
-              }
-              if (mine) {
-                const int j = scount + lane;
-                const uint32_t m = (uint32_t)(w[8] >> 32);
-                if ((int)(m & 0xffu) >= T.n_species) err |= kErrBadEntry;
-                S.px[j] = __longlong_as_double((long long)w[0]);
-                S.py[j] = __longlong_as_double((long long)w[1]);
-                S.pz[j] = __longlong_as_double((long long)w[2]);
-                S.dx[j] = __longlong_as_double((long long)w[3]);
-                S.dy[j] = __longlong_as_double((long long)w[4]);
-                S.dz[j] = __longlong_as_double((long long)w[5]);
-                S.E[j] = __longlong_as_double((long long)w[6]);
-                S.key[j] = w[7];
-                S.meta[j] = (int)(m & 0xffu) < T.n_species ? m : 0u;
-                S.tag[j] = (int)(uint32_t)w[8];
-                S.hist[j] = wh;
               }
             }
             const uint64_t bad = __ballot((err & (kErrTimeout | kErrBadEntry)) != 0u);
@@ -509,7 +499,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
         const ElemConst& ec = T.el[F.sp * T.n_el + mc.el_begin + nn];
         ruth = (it == 0);
         surr = (it < 0);
-        l_newton += it > 0 ? it : 0;
+        l_newton32 += it > 0 ? (unsigned)it : 0u;
         if (it >= kMaxNewton) err |= kErrNewton;
         // energy transfer and momenta (trim.C:275-297)
         double den = ec.ec * s2 * F.E;
@@ -613,7 +603,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
               const double kd = 0.1337 * pow(mc.az, 2.0 / 3.0) / sqrt(mc.am);
               const double Ev = rE / (1.0 + kd * g);
               extra = (int)(0.8 * Ev / (2.0 * ec.Edisp));
-              l_kpvac += extra;
+              l_kpvac32 += (unsigned)extra;
             }
             if (PERHIST) atomicAdd(&P.h_vac[F.hist - (uint32_t)P.hist0], 1 + extra);
             state = -1;  // vacancy marker, ion keeps moving
@@ -703,22 +693,21 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
           const uint64_t pos = base + (uint64_t)(r - room);
           slot = ((uint64_t)home << P.q.sub_log2) | (pos & P.q.sub_mask);
           lap = (uint32_t)(pos >> P.q.sub_log2) + 1u;
-          const uint64_t w[9] = {(uint64_t)__double_as_longlong(rx),  (uint64_t)__double_as_longlong(ry),
-                                 (uint64_t)__double_as_longlong(rz),  (uint64_t)__double_as_longlong(rdx),
-                                 (uint64_t)__double_as_longlong(rdy), (uint64_t)__double_as_longlong(rdz),
-                                 (uint64_t)__double_as_longlong(rE),  rkey,
-                                 ((uint64_t)meta << 32) | (uint32_t)rtag};
-          st_agent(&P.q.px[slot], w[0]);
-          st_agent(&P.q.py[slot], w[1]);
-          st_agent(&P.q.pz[slot], w[2]);
-          st_agent(&P.q.dx[slot], w[3]);
-          st_agent(&P.q.dy[slot], w[4]);
-          st_agent(&P.q.dz[slot], w[5]);
-          st_agent(&P.q.E[slot], w[6]);
-          st_agent(&P.q.key[slot], w[7]);
-          st_agent(&P.q.mt[slot], w[8]);
+          uint64_t h = sum_init(lap), w;
+          w = (uint64_t)__double_as_longlong(rx); st_agent(&P.q.px[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(ry); st_agent(&P.q.py[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rz); st_agent(&P.q.pz[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rdx); st_agent(&P.q.dx[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rdy); st_agent(&P.q.dy[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rdz); st_agent(&P.q.dz[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rE); st_agent(&P.q.E[slot], w); h = sum_add(h, w);
+          st_agent(&P.q.key[slot], rkey);
+          h = sum_add(h, rkey);
+          w = ((uint64_t)meta << 32) | (uint32_t)rtag;
+          st_agent(&P.q.mt[slot], w);
+          h = sum_add(h, w);
           st_agent(&P.q.hist[slot], F.hist);
-          sum = entry_sum(lap, w, F.hist);
+          sum = sum_end(sum_add(h, F.hist));
         }
         // this wave's outstanding memory operations (the entry and the
         // history's flight-count increment) complete before the ready word;
@@ -775,6 +764,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock) cascade_kernel(Params P) {
   }
 
   // ------------------------------------------------------------ flush
+  long long l_newton = (long long)l_newton32, l_kpvac = (long long)l_kpvac32;
   for (int off = 32; off > 0; off >>= 1) {
     l_newton += __shfl_xor(l_newton, off);
     l_kpvac += __shfl_xor(l_kpvac, off);
