@@ -138,8 +138,8 @@ typedef struct {
 /* Tuning knobs of the persistent kernel (0 = default). */
 typedef struct {
   int block;          /* threads per workgroup (multiple of 64, at most 256) */
-  int blocks_per_cu;  /* resident workgroups per CU (0: one wave per SIMD) */
-  int stack_cap;      /* per-wave LDS recoil stack entries (0: 384) */
+  int blocks_per_cu;  /* resident workgroups per CU (0: two waves per SIMD) */
+  int stack_cap;      /* per-wave LDS recoil stack entries (0: 192) */
   long long queue_cap;/* HBM queue entries (power of two) */
   int lds_hist;       /* bin vacancy/replacement depths in LDS first */
   int graph;          /* >= 0: replay the run sequence (reset, kernel, flush,

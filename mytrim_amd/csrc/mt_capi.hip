@@ -46,10 +46,10 @@ hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t strea
 hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo,
                          const MlpNorm& nm, float* X, float* T, hipStream_t st);
 hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
-                     float* rec, float* g, float* m, float* v, int* t, float* batch_loss, double* losses,
-                     hipStream_t st);
-hipError_t mlp_batch_grad(const float* p, const float* X, const float* T, int B, float* rec, float* g,
-                          float* loss, hipStream_t st);
+                     float* rec, float* part, float* g, float* m, float* v, int* t, float* batch_loss,
+                     double* losses, hipStream_t st);
+hipError_t mlp_batch_grad(const float* p, const float* X, const float* T, int B, float* rec, float* part,
+                          float* g, float* loss, hipStream_t st);
 hipError_t mlp_pack(const float* p, uint32_t* w2bf, hipStream_t st);
 hipError_t mlp_infer_launch(const MlpDev& S, const double* eps, const double* b, long long n, double* s2,
                             hipStream_t st);
@@ -181,7 +181,7 @@ struct mt_ctx {
   std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
   DBuf f_ion, f_u, f_ev, f_used;
   // scattering surrogate
-  DBuf mlp_p, mlp_w2bf, mlp_X, mlp_T, mlp_rec, mlp_g, mlp_m, mlp_v, mlp_t, mlp_bl, mlp_tri;
+  DBuf mlp_p, mlp_w2bf, mlp_X, mlp_T, mlp_rec, mlp_part, mlp_g, mlp_m, mlp_v, mlp_t, mlp_bl, mlp_tri;
   MlpNorm mlp_norm{};
   bool mlp_ready = false;
   int mlp_bf16 = 0;
@@ -201,7 +201,8 @@ struct mt_ctx {
     f_u.release();
     f_ev.release();
     f_used.release();
-    DBuf* ml[] = {&mlp_p, &mlp_w2bf, &mlp_X, &mlp_T, &mlp_rec, &mlp_g, &mlp_m, &mlp_v, &mlp_t, &mlp_bl, &mlp_tri};
+    DBuf* ml[] = {&mlp_p,     &mlp_w2bf, &mlp_X, &mlp_T, &mlp_rec, &mlp_part,
+                  &mlp_g,     &mlp_m,    &mlp_v, &mlp_t, &mlp_bl,  &mlp_tri};
     for (DBuf* b : ml) b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -456,10 +457,10 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   HIPCHECK(hipMemcpyAsync(ctx->pka_dir.p, pk->dir, 3 * n_entries * sizeof(double), hipMemcpyHostToDevice, st));
 
   // launch shape
-  // defaults: the shape measured best for Cu -> Cu cascades (one wave per
-  // SIMD, 384-entry LDS stacks)
+  // defaults: the shape measured best for Cu -> Cu cascades (two waves per
+  // SIMD, 192-entry LDS stacks)
   const int block = ctx->tune.block ? ctx->tune.block : 256;
-  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 384;
+  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 192;
   const int nwb = block / 64;
   const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
   size_t lds = blob.size() + size_t(nwb) * cascade_stack_bytes(cap);
@@ -756,9 +757,9 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // candidate shapes (threads per block, resident blocks per CU, LDS stack
   // entries per wave) around the region measured best on MI355X: one or two
   // waves per SIMD with deep stacks; queue capacity stays at its default
-  static const int kCand[][3] = {{256, 1, 256}, {256, 1, 384}, {256, 1, 512}, {128, 2, 256}, {128, 2, 384},
-                                 {128, 3, 256}, {128, 4, 192}, {64, 4, 384}, {64, 6, 256}, {64, 8, 192},
-                                 {256, 2, 192}, {64, 4, 512}};
+  static const int kCand[][3] = {{256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {128, 4, 192}, {128, 4, 128},
+                                 {64, 8, 160}, {64, 8, 128}, {256, 1, 384}, {256, 1, 256}, {128, 3, 256},
+                                 {128, 2, 384}, {64, 6, 192}};
   std::vector<mt_tuning> cand;
   for (const auto& c : kCand) {
     mt_tuning t{};
@@ -948,6 +949,7 @@ int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* o, float* params, d
   HIPCHECK(ctx->mlp_X.ensure(size_t(n) * kMlpIn * sizeof(float)));
   HIPCHECK(ctx->mlp_T.ensure(size_t(n) * sizeof(float)));
   HIPCHECK(ctx->mlp_rec.ensure(size_t(B) * kMlpRecord * sizeof(float)));
+  HIPCHECK(ctx->mlp_part.ensure(size_t((B + kMlpGradChunk - 1) / kMlpGradChunk) * (kMlpParams + 1) * sizeof(float)));
   HIPCHECK(ctx->mlp_g.ensure((kMlpParams + 1) * sizeof(float)));
   HIPCHECK(ctx->mlp_m.ensure(kMlpParams * sizeof(float)));
   HIPCHECK(ctx->mlp_v.ensure(kMlpParams * sizeof(float)));
@@ -962,7 +964,8 @@ int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* o, float* params, d
   HIPCHECK(mlp_make_data(o->seed, n, ctx->mlp_tri.as<MlpTriple>(), int(tri.size()), o->E_min, nm,
                          ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), st));
   HIPCHECK(mlp_train(ctx->mlp_p.as<float>(), ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), n, B, o->epochs,
-                     float(o->lr), ctx->mlp_rec.as<float>(), ctx->mlp_g.as<float>(), ctx->mlp_m.as<float>(),
+                     float(o->lr), ctx->mlp_rec.as<float>(), ctx->mlp_part.as<float>(), ctx->mlp_g.as<float>(),
+                     ctx->mlp_m.as<float>(),
                      ctx->mlp_v.as<float>(), ctx->mlp_t.as<int>(), ctx->mlp_bl.as<float>(), epoch_loss, st));
   HIPCHECK(hipMemcpyAsync(params, ctx->mlp_p.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
@@ -1028,12 +1031,13 @@ int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* o, const float* para
   HIPCHECK(ctx->scratch_c.ensure(T.size() * sizeof(float) + 16));
   HIPCHECK(ctx->scratch_d.ensure(size_t(n) * kMlpRecord * sizeof(float)));
   HIPCHECK(ctx->scratch_e.ensure((kMlpParams + 1) * sizeof(float)));
+  HIPCHECK(ctx->mlp_part.ensure(size_t((n + kMlpGradChunk - 1) / kMlpGradChunk) * (kMlpParams + 1) * sizeof(float)));
   HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(ctx->scratch_b.p, X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(ctx->scratch_c.p, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice, st));
   float* g = ctx->scratch_e.as<float>();
   HIPCHECK(mlp_batch_grad(ctx->scratch_a.as<float>(), ctx->scratch_b.as<float>(), ctx->scratch_c.as<float>(), n,
-                          ctx->scratch_d.as<float>(), g, g + kMlpParams, st));
+                          ctx->scratch_d.as<float>(), ctx->mlp_part.as<float>(), g, g + kMlpParams, st));
   float lf = 0.0f;
   HIPCHECK(hipMemcpyAsync(grad, g, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipMemcpyAsync(&lf, g + kMlpParams, sizeof(float), hipMemcpyDeviceToHost, st));

@@ -113,8 +113,8 @@ struct Params {
 
 // launch bound of the persistent kernel (threads per workgroup)
 constexpr int kCascadeMaxBlock = 256;
-// resident waves per CU when the tuning leaves it open (one per SIMD)
-constexpr int kDefaultWavesPerCu = 4;
+// resident waves per CU when the tuning leaves it open (two per SIMD)
+constexpr int kDefaultWavesPerCu = 8;
 
 // device spans zeroed before a run
 constexpr int kMaxSpans = 16;

@@ -22,6 +22,8 @@ constexpr int kOffB3 = kOffW3 + kMlpH;        // b3
 constexpr int kMlpParams = kOffB3 + 1;        // 4481
 // floats per training sample written by the forward/backward kernel
 constexpr int kMlpRecord = 4 + 4 * kMlpH + 1;
+// samples per partial sum of the gradient reduction
+constexpr int kMlpGradChunk = 128;
 
 // input / output scaling and the trained domain
 struct MlpNorm {

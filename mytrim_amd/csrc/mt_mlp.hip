@@ -5,8 +5,9 @@
 //                 potential), one Philox stream per sample
 //   mlp_fwdbwd    one lane per sample: forward pass and the per-sample
 //                 back-propagated deltas (stored for the gradient kernel)
-//   mlp_grad      one lane per parameter: batch sum of delta x activation
-//                 (deterministic, no atomics); the last lane sums the loss
+//   mlp_grad_*    per parameter: batch sum of delta x activation in fixed
+//                 chunks, then the chunks in order (deterministic, no
+//                 atomics); one extra column sums the loss
 //   mlp_adam      Adam update with a cosine step-size schedule on device
 // An epoch (every batch: fwdbwd, grad, adam) is captured once in a hipGraph
 // and replayed.
@@ -99,17 +100,11 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd(const float* __restrict__ p, c
   r[kRec - 1] = e * e;
 }
 
-__global__ void __launch_bounds__(256) mlp_grad(const float* rec, int B, float* g, float* loss_slot) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q > kMlpParams) return;
-  float acc = 0.0f;
-  if (q == kMlpParams) {  // mean squared error of the batch
-    for (int s = 0; s < B; ++s) acc += rec[(size_t)s * kRec + kRec - 1];
-    *loss_slot = acc / (float)B;
-    return;
-  }
-  // dL/dparam = sum over samples of (delta of the unit it feeds) x (its input)
-  int a_off, b_off;  // record offsets: delta index, input index (-1 = bias)
+constexpr int kGradChunk = kMlpGradChunk;
+
+// record offsets of the delta and the input a parameter's gradient pairs
+// (input -1: a bias)
+__device__ __forceinline__ void grad_operands(int q, int& a_off, int& b_off) {
   if (q < kOffB1) {
     a_off = kRecD1 + q / kMlpIn;
     b_off = q % kMlpIn;
@@ -126,15 +121,43 @@ __global__ void __launch_bounds__(256) mlp_grad(const float* rec, int B, float* 
   } else if (q < kOffB3) {
     a_off = kRecDy;
     b_off = kRecH2 + (q - kOffW3);
-  } else {
+  } else if (q == kOffB3) {
     a_off = kRecDy;
     b_off = -1;
+  } else {  // q == kMlpParams: the squared errors
+    a_off = kRec - 1;
+    b_off = -1;
   }
-  for (int s = 0; s < B; ++s) {
+}
+
+// dL/dparam = sum over samples of (delta of the unit it feeds) x (its input):
+// lane q of block column c sums samples [c * kGradChunk, (c + 1) * kGradChunk)
+// into part[c][q]; the last "parameter" collects the squared errors.  Fixed
+// summation order, no atomics: the result is deterministic.
+__global__ void __launch_bounds__(256) mlp_grad_part(const float* rec, int B, float* part) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > kMlpParams) return;
+  int a_off, b_off;
+  grad_operands(q, a_off, b_off);
+  const int s0 = blockIdx.y * kGradChunk, s1 = min(B, s0 + kGradChunk);
+  float acc = 0.0f;
+  for (int s = s0; s < s1; ++s) {
     const float* r = rec + (size_t)s * kRec;
     acc += r[a_off] * (b_off < 0 ? 1.0f : r[b_off]);
   }
-  g[q] = acc;
+  part[(size_t)blockIdx.y * (kMlpParams + 1) + q] = acc;
+}
+
+__global__ void __launch_bounds__(256) mlp_grad_sum(const float* part, int n_chunks, int B, float* g,
+                                                    float* loss_slot) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > kMlpParams) return;
+  float acc = 0.0f;
+  for (int c = 0; c < n_chunks; ++c) acc += part[(size_t)c * (kMlpParams + 1) + q];
+  if (q == kMlpParams)
+    *loss_slot = acc / (float)B;
+  else
+    g[q] = acc;
 }
 
 // Adam (Kingma & Ba 2015) with beta1 0.9, beta2 0.999; the step size follows
@@ -182,11 +205,19 @@ hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n
   return hipGetLastError();
 }
 
+// part: (B / kGradChunk rounded up) x (kMlpParams + 1) floats
 // one optimisation step on samples [base, base + B)
-static void step_launch(float* p, const float* X, const float* T, long long base, int B, float* rec, float* g,
-                        float* m, float* v, int* t, float lr0, int total, float* loss_slot, hipStream_t st) {
+static void grad_launch(const float* rec, int B, float* part, float* g, float* loss_slot, hipStream_t st) {
+  const int chunks = (B + kGradChunk - 1) / kGradChunk;
+  hipLaunchKernelGGL(mlp_grad_part, dim3((kMlpParams + 1 + 255) / 256, chunks), dim3(256), 0, st, rec, B, part);
+  hipLaunchKernelGGL(mlp_grad_sum, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, part, chunks, B, g,
+                     loss_slot);
+}
+
+static void step_launch(float* p, const float* X, const float* T, long long base, int B, float* rec, float* part,
+                        float* g, float* m, float* v, int* t, float lr0, int total, float* loss_slot, hipStream_t st) {
   hipLaunchKernelGGL(mlp_fwdbwd, dim3((B + 255) / 256), dim3(256), 0, st, p, X, T, base, B, rec);
-  hipLaunchKernelGGL(mlp_grad, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, rec, B, g, loss_slot);
+  grad_launch(rec, B, part, g, loss_slot, st);
   hipLaunchKernelGGL(mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, p, g, m, v, t, lr0, total);
   hipLaunchKernelGGL(mlp_tick, dim3(1), dim3(1), 0, st, t);
 }
@@ -194,8 +225,8 @@ static void step_launch(float* p, const float* X, const float* T, long long base
 // Train for `epochs` over n samples in batches of B; one epoch captured as a
 // graph and replayed.  losses[e] = mean batch loss of epoch e.
 hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
-                     float* rec, float* g, float* m, float* v, int* t, float* batch_loss, double* losses,
-                     hipStream_t st) {
+                     float* rec, float* part, float* g, float* m, float* v, int* t, float* batch_loss,
+                     double* losses, hipStream_t st) {
   const int steps = (int)(n / B);
   if (steps < 1) return hipErrorInvalidValue;
   const int total = steps * epochs;
@@ -204,7 +235,7 @@ hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int 
   hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
   for (int s = 0; s < steps; ++s)
-    step_launch(p, X, T, (long long)s * B, B, rec, g, m, v, t, lr0, total, batch_loss + s, st);
+    step_launch(p, X, T, (long long)s * B, B, rec, part, g, m, v, t, lr0, total, batch_loss + s, st);
   e = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) return e;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -224,10 +255,10 @@ hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int 
 }
 
 // gradient and loss of one batch (for checks against the CPU)
-hipError_t mlp_batch_grad(const float* p, const float* X, const float* T, int B, float* rec, float* g,
-                          float* loss, hipStream_t st) {
+hipError_t mlp_batch_grad(const float* p, const float* X, const float* T, int B, float* rec, float* part,
+                          float* g, float* loss, hipStream_t st) {
   hipLaunchKernelGGL(mlp_fwdbwd, dim3((B + 255) / 256), dim3(256), 0, st, p, X, T, 0ll, B, rec);
-  hipLaunchKernelGGL(mlp_grad, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, rec, B, g, loss);
+  grad_launch(rec, B, part, g, loss, st);
   return hipGetLastError();
 }
 
