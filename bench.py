@@ -84,6 +84,18 @@ def train_surrogate(eng, gpu, prob, n_train: int, epochs: int):
             "domain": {k: v for k, v in eng.surrogate_norm(opts).items() if k in ("eps_lo", "eps_hi", "b_hi")}}
 
 
+def tally_errors(ref, r):
+    """Relative difference of each tally from the FP64 run on the same histories."""
+    def rel(a, b):
+        return (b - a) / a if a else 0.0
+    out = {k: rel(ref[k], r[k]) for k in ("n_coll", "n_flight", "vac", "repl", "disp", "Eel", "Enuc")}
+    out["range_mean"] = rel(ref["range_sum"] / max(ref["range_n"], 1), r["range_sum"] / max(r["range_n"], 1))
+    for h in ("range_hist", "vac_hist"):  # total-variation distance of the histograms
+        a, b = ref[h], r[h]
+        out[h + "_tv"] = 0.5 * float(abs(a / max(a.sum(), 1) - b / max(b.sum(), 1)).sum())
+    return out
+
+
 def timed(eng, prob, arith, n_hist, steps):
     bounds = [n_hist * i // steps for i in range(steps + 1)]
     coll, kms, res = 0, 0.0, []
@@ -148,17 +160,19 @@ def main():
     variants = None
     if args.variant_histories > 0:
         nv = min(args.variant_histories, args.histories)
-        c64, w64, k64, _ = timed(eng, prob, gpu.MT_FP64, nv, 1)
-        c32, w32, k32, _ = timed(eng, prob, gpu.MT_FP32, nv, 1)
+        c64, w64, k64, r64 = timed(eng, prob, gpu.MT_FP64, nv, 1)
+        c32, w32, k32, r32 = timed(eng, prob, gpu.MT_FP32, nv, 1)
         sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)
         cs, ws, ks, rs = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
         variants = {
             "histories": nv,
             "fp64": {"collisions_per_s": c64 / w64, "kernel_ms": k64},
-            "fp32": {"collisions_per_s": c32 / w32, "kernel_ms": k32, "speedup_vs_fp64": (c32 / w32) / (c64 / w64)},
+            "fp32": {"collisions_per_s": c32 / w32, "kernel_ms": k32, "speedup_vs_fp64": (c32 / w32) / (c64 / w64),
+                     "tally_rel_err_vs_fp64": tally_errors(r64[0], r32[0])},
             "fp32_surrogate": {"collisions_per_s": cs / ws, "kernel_ms": ks,
                                "speedup_vs_fp64": (cs / ws) / (c64 / w64),
                                "surrogate_share": rs[0]["surrogate"] / max(rs[0]["n_coll"], 1),
+                               "tally_rel_err_vs_fp64": tally_errors(r64[0], rs[0]),
                                "training": sur},
         }
     eng.close()

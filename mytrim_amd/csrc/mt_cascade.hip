@@ -144,7 +144,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     // free path and maximum impact parameter (trim.C:88-95)
     double eps = E * mc.f;
     const double eeg = sqrt(eps * mc.epsdg);
-    const double pmax = mc.a / (eeg + sqrt(eeg) + 0.125 * pow(eeg, 0.1));
+    const double pmax = mc.a / (eeg + sqrt(eeg) + 0.125 * m_pow(eeg, 0.1));
     ls = 1.0 / (M_PI * (pmax * pmax) * mc.arho);
     if (ic == 1) {
       const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
@@ -598,9 +598,9 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
               atomicAdd(&P.vac_hist[bn], 1ull);
             int extra = 0;
             if (P.kp && F.gen + 1 == P.follow_max_gen) {
-              const double ed = 0.0115 * pow(mc.az, -7.0 / 3.0) * rE;
-              const double g = 3.4008 * pow(ed, 1.0 / 6.0) + 0.40244 * pow(ed, 3.0 / 4.0) + ed;
-              const double kd = 0.1337 * pow(mc.az, 2.0 / 3.0) / sqrt(mc.am);
+              const double ed = 0.0115 * m_pow(mc.az, -7.0 / 3.0) * rE;
+              const double g = 3.4008 * m_pow(ed, 1.0 / 6.0) + 0.40244 * m_pow(ed, 3.0 / 4.0) + ed;
+              const double kd = 0.1337 * m_pow(mc.az, 2.0 / 3.0) / sqrt(mc.am);
               const double Ev = rE / (1.0 + kd * g);
               extra = (int)(0.8 * Ev / (2.0 * ec.Edisp));
               l_kpvac32 += (unsigned)extra;

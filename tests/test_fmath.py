@@ -1,0 +1,49 @@
+"""Accuracy of the kernel's FP64 exp / log / pow (mytrim_amd/csrc/mt_fmath.h).
+
+The header is compiled for the host (the same source the device runs; the
+device build differs only in frexp/ldexp, which are exact either way) into
+tests/native/fmath_check.cpp, which measures the worst error against long
+double references on random arguments:
+
+* exp on [-700, 700], log on [1e-300, 1e300] and on [0.5, 2] (near 1, where
+  the result is small),
+* pow for x in [1e-8, 1e6], y in [-2, 2], the ranges the physics uses
+  (pmax's eeg^0.1, the Kinchin-Pease pows, the stopping fits).
+
+Bounds: exp <= 1 ulp, log <= 2 ulp, pow <= 2 ulp per unit of |y ln x| + 1
+and 1e-14 relative. A dropped series term, a wrong ln 2 split or a missing
+reduction step each raise one of these by orders of magnitude.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def fmath_report(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    exe = str(tmp_path_factory.mktemp("fmath") / "fmath_check")
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "mytrim_amd", "csrc"), "-o", exe,
+                    os.path.join(HERE, "native", "fmath_check.cpp")], check=True, capture_output=True)
+    out = subprocess.run([exe, "400000", "7"], check=True, capture_output=True, text=True).stdout
+    return {k: float(v) for k, v in (line.split() for line in out.strip().splitlines())}
+
+
+def test_exp_within_one_ulp(fmath_report):
+    assert fmath_report["exp_ulp"] <= 1.0
+
+
+def test_log_within_two_ulp(fmath_report):
+    assert fmath_report["log_ulp"] <= 2.0
+
+
+def test_pow_error_tracks_y_log_x(fmath_report):
+    assert fmath_report["pow_ulp_per_unit_ylogx"] <= 2.0
+    assert fmath_report["pow_rel"] <= 1e-14
