@@ -143,17 +143,17 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
   if constexpr (ARITH == 0) {
     // free path and maximum impact parameter (trim.C:88-95)
     double eps = E * mc.f;
-    const double eeg = sqrt(eps * mc.epsdg);
-    const double pmax = mc.a / (eeg + sqrt(eeg) + 0.125 * m_pow(eeg, 0.1));
-    ls = 1.0 / (M_PI * (pmax * pmax) * mc.arho);
+    const double eeg = m_sqrt(eps * mc.epsdg);
+    const double pmax = mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125 * m_pow(eeg, 0.1));
+    ls = m_rcp(M_PI * (pmax * pmax) * mc.arho);
     if (ic == 1) {
       const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
       ls = u53(b0.x, b0.y) * mn(ls, cw);
     }
     // impact parameter and target atom (trim.C:143-160)
-    const double p = pmax * sqrt(u53(blk.x, blk.y));
+    const double p = pmax * m_sqrt(u53(blk.x, blk.y));
     eps = ec.fi * E;
-    const double b = p / ec.ai;
+    const double b = p * m_rcp(ec.ai);
     // electronic loss along ls (trim.C:166-170) and scattering (trim.C:172-272)
     dee = ls * getrstop<double>(T, sp, k, E);
     scatter<double, POT>(eps, b, s2, ct, st, it);
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         ended = true;
       } else {
         {  // v_norm (functions.h:66-70)
-          const double inv = 1.0 / sqrt(F.dx * F.dx + F.dy * F.dy + F.dz * F.dz);
+          const double inv = fx_rsqrt(F.dx * F.dx + F.dy * F.dy + F.dz * F.dz);
           F.dx *= inv;
           F.dy *= inv;
           F.dz *= inv;
@@ -508,10 +508,10 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         kadd(eel_s, eel_e, dee);
         if (PERHIST) F.eel += dee;
         const double m1 = H->species[F.sp].m;
-        const double p1 = sqrt(2.0 * m1 * F.E);
+        const double p1 = m_sqrt(2.0 * m1 * F.E);
         if (den > F.E) den = F.E;
         F.E -= den;
-        const double p2 = sqrt(2.0 * m1 * F.E);
+        const double p2 = m_sqrt(2.0 * m1 * F.E);
         // recoil at the collision site, ion moves on (trim.C:306-318)
         rx = F.px;
         ry = F.py;
@@ -525,38 +525,31 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         rdz = F.dz * p1;
         rE = den - ec.Elbind;
         // azimuth about the flight direction (replaces trim.C:320-333)
+        // u: unit vector perpendicular to d in the plane of its two largest
+        // components (x smallest: (0, dz, -dy); y: (-dz, 0, dx); z: (dy, -dx, 0)),
+        // chosen by selects so that a wave evaluates one reciprocal square root
         double ux, uy, uz;
         {
           const double ax = fabs(F.dx), ay = fabs(F.dy), az = fabs(F.dz);
-          if (ax <= ay && ax <= az) {
-            const double n = 1.0 / sqrt(F.dy * F.dy + F.dz * F.dz);
-            ux = 0.0;
-            uy = F.dz * n;
-            uz = -F.dy * n;
-          } else if (ay <= az) {
-            const double n = 1.0 / sqrt(F.dx * F.dx + F.dz * F.dz);
-            ux = -F.dz * n;
-            uy = 0.0;
-            uz = F.dx * n;
-          } else {
-            const double n = 1.0 / sqrt(F.dx * F.dx + F.dy * F.dy);
-            ux = F.dy * n;
-            uy = -F.dx * n;
-            uz = 0.0;
-          }
+          const bool xs = ax <= ay && ax <= az, ys = !xs && ay <= az;
+          const double c0 = xs ? F.dy : F.dx, c1 = (xs || ys) ? F.dz : F.dy;
+          const double n = fx_rsqrt(c0 * c0 + c1 * c1);
+          ux = xs ? 0.0 : (ys ? -F.dz : F.dy) * n;
+          uy = xs ? F.dz * n : (ys ? 0.0 : -F.dx * n);
+          uz = xs ? -F.dy * n : (ys ? F.dx * n : 0.0);
         }
         const double vx = F.dy * uz - F.dz * uy;
         const double vy = F.dz * ux - F.dx * uz;
         const double vz = F.dx * uy - F.dy * ux;
         double sphi, cphi;
-        sincospi(2.0 * u32d(blk.w), &sphi, &cphi);
+        fx_sincos2pi(u32d(blk.w), sphi, cphi);
         const double qx = cphi * ux + sphi * vx;
         const double qy = cphi * uy + sphi * vy;
         const double qz = cphi * uz + sphi * vz;
         // new direction (trim.C:336-341): psi = atan2(st, ct + my), so
         // cos psi and sin psi are the normalised arguments (st >= 0)
         const double xpsi = ct + ec.my;
-        const double ipsi = 1.0 / sqrt(xpsi * xpsi + st * st);
+        const double ipsi = fx_rsqrt(xpsi * xpsi + st * st);
         const double cpsi = xpsi * ipsi;
         const double spsi = st * ipsi;
         F.dx = F.dx * cpsi + qx * spsi;
@@ -580,7 +573,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           if (follow) {
             push = true;
             kadd(enuc_s, enuc_e, ec.Elbind);
-            const double inv = 1.0 / sqrt(rdx * rdx + rdy * rdy + rdz * rdz);
+            const double inv = fx_rsqrt(rdx * rdx + rdy * rdy + rdz * rdz);
             rdx *= inv;
             rdy *= inv;
             rdz *= inv;
@@ -600,7 +593,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
             if (P.kp && F.gen + 1 == P.follow_max_gen) {
               const double ed = 0.0115 * m_pow(mc.az, -7.0 / 3.0) * rE;
               const double g = 3.4008 * m_pow(ed, 1.0 / 6.0) + 0.40244 * m_pow(ed, 3.0 / 4.0) + ed;
-              const double kd = 0.1337 * m_pow(mc.az, 2.0 / 3.0) / sqrt(mc.am);
+              const double kd = 0.1337 * m_pow(mc.az, 2.0 / 3.0) / m_sqrt(mc.am);
               const double Ev = rE / (1.0 + kd * g);
               extra = (int)(0.8 * Ev / (2.0 * ec.Edisp));
               l_kpvac32 += (unsigned)extra;

@@ -1,9 +1,16 @@
-// FP64 exp / log / pow for the cascade kernel, written from their Taylor
-// series with Cody-Waite argument reduction (no tables, no branches).  The
-// device libm's routines cost about 40 (exp), 95 (log) and 220 (pow) VALU
-// instructions; these cost about 25, 35 and 60 and stay within 1 ulp
-// (exp), 2 ulp (log) and 1.7 ulp per unit of |y log x| (pow) of the exact
-// values (tests/test_fmath.py measures the maxima against long double).
+// FP64 elementary functions for the cascade kernel, without tables or
+// branches:
+//   exp, log, pow   Taylor series with Cody-Waite argument reduction; the
+//                   device libm's routines cost about 40, 95 and 220 VALU
+//                   instructions, these about 25, 35 and 60
+//   rcp, rsqrt,     the hardware reciprocal / reciprocal-square-root
+//   sqrt            estimates (about 2^-23 relative) refined by Newton steps
+//                   (5, 9 and 11 instructions against 8, 29 and 18)
+//   sincos2pi       sin and cos of 2 pi u for u in [0, 1] by quadrant and
+//                   Taylor series (about 35 against sincospi's 67)
+// Errors against exact values stay within 1 ulp (exp, rcp, rsqrt, sqrt),
+// 2 ulp (log, sincos2pi) and 1.7 ulp per unit of |y log x| (pow);
+// tests/test_fmath.py measures the maxima against long double.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -34,6 +41,85 @@ __host__ __device__ __forceinline__ double fx_ldexp(double x, int e) {
 #else
   return std::ldexp(x, e);
 #endif
+}
+
+// Hardware estimates of 1/x and 1/sqrt(x).  The host build (used to test the
+// refinement) starts from single precision, an error of the same order.
+__host__ __device__ __forceinline__ double fx_rcp_seed(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcp(x);
+#else
+  return (double)(1.0f / (float)x);
+#endif
+}
+
+__host__ __device__ __forceinline__ double fx_rsq_seed(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsq(x);
+#else
+  return (double)(1.0f / std::sqrt((float)x));
+#endif
+}
+
+// 1/x for finite nonzero x of magnitude in [1e-300, 1e300]: two Newton steps
+// y += y (1 - x y), each squaring the relative error
+__host__ __device__ __forceinline__ double fx_rcp(double x) {
+  double y = fx_rcp_seed(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+// 1/sqrt(x) for finite x > 0: two Newton steps y += (y / 2)(1 - x y^2)
+__host__ __device__ __forceinline__ double fx_rsqrt(double x) {
+  double y = fx_rsq_seed(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
+
+// sqrt(x) for finite x >= 0: s = x y from one Newton step of 1/sqrt(x),
+// then the correction s += (y / 2)(x - s^2); sqrt(0) = 0
+__host__ __device__ __forceinline__ double fx_sqrt(double x) {
+  double y = fx_rsq_seed(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  const double s = x * y;
+  const double r = fma(0.5 * y, fma(-s, s, x), s);
+  return x == 0.0 ? x : r;
+}
+
+// sin(2 pi u) and cos(2 pi u) for u in [0, 1].  n = rint(4u) picks the
+// quadrant, f = u - n/4 in [-1/8, 1/8] is exact, x = 2 pi f (two-part 2 pi)
+// lies in [-pi/4, pi/4], where the Taylor series to x^17 (sin) and x^16 (cos)
+// leave less than 1e-17 relative
+__host__ __device__ __forceinline__ void fx_sincos2pi(double u, double& s, double& c) {
+  constexpr double k2PiHi = 6.283185307179586232;      // 2 pi rounded to double
+  constexpr double k2PiLo = 2.4492935982947063545e-16;  // 2 pi - k2PiHi
+  const double n = rint(4.0 * u);
+  const double f = fma(n, -0.25, u);
+  const double x = fma(f, k2PiHi, f * k2PiLo);
+  const double z = x * x;
+  double ps = -1.0 / 355687428096000.0;  // -1/17!
+  ps = fma(ps, z, 1.0 / 1307674368000.0);
+  ps = fma(ps, z, -1.0 / 6227020800.0);
+  ps = fma(ps, z, 1.0 / 39916800.0);
+  ps = fma(ps, z, -1.0 / 362880.0);
+  ps = fma(ps, z, 1.0 / 5040.0);
+  ps = fma(ps, z, -1.0 / 120.0);
+  ps = fma(ps, z, 1.0 / 6.0);
+  const double sx = fma(-x * z, ps, x);
+  double pc = 1.0 / 20922789888000.0;  // 1/16!
+  pc = fma(pc, z, -1.0 / 87178291200.0);
+  pc = fma(pc, z, 1.0 / 479001600.0);
+  pc = fma(pc, z, -1.0 / 3628800.0);
+  pc = fma(pc, z, 1.0 / 40320.0);
+  pc = fma(pc, z, -1.0 / 720.0);
+  pc = fma(pc, z, 1.0 / 24.0);
+  pc = fma(pc, z, -0.5);
+  const double cx = fma(z, pc, 1.0);
+  const int q = (int)n;
+  const double sv = (q & 1) ? cx : sx;
+  const double cv = (q & 1) ? sx : cx;
+  s = (q & 2) ? -sv : sv;
+  c = ((q + 1) & 2) ? -cv : cv;
 }
 
 // exp(r) for |r| <= ln2 / 2: Taylor series to r^13 (next term < 4e-18)
@@ -77,7 +163,10 @@ __host__ __device__ __forceinline__ double fx_log(double x) {
     --e;
   }
   const double f = m - 1.0;
-  const double s = f / (m + 1.0);
+  // s = f / (m + 1) from the refined reciprocal and one residual correction
+  const double d = m + 1.0, y = fx_rcp(d);
+  const double s0 = f * y;
+  const double s = fma(fma(-d, s0, f), y, s0);
   const double z = s * s;
   double p = 1.0 / 21.0;
   p = fma(p, z, 1.0 / 19.0);
@@ -96,7 +185,7 @@ __host__ __device__ __forceinline__ double fx_log(double x) {
   double v = fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
   v = (x == 0.0) ? -HUGE_VAL : v;
   v = (x == HUGE_VAL) ? HUGE_VAL : v;
-  return (x >= 0.0) ? v : (x - x) / (x - x);  // negative or NaN: NaN
+  return (x >= 0.0) ? v : __builtin_nan("");  // negative or NaN: NaN
 }
 
 // pow(x, y) = exp(y log x) for x >= 0 (error grows with |y log x|: about

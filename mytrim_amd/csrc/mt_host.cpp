@@ -93,6 +93,13 @@ static StopConst make_stop(const std::vector<CoefRow>& coef, int z1, double m_io
   c.eee = 25.0 * vmin * vmin;
   c.sp_low = host_rpstop(coef, z2, c.eee);
   c.power = (z2 == 6 || ((z2 == 14 || z2 == 32) && z1 <= 19)) ? 0.375 : 0.5;
+  c.inv_m1 = 1.0 / c.m1;
+  c.inv_vfermi = 1.0 / c.vfermi;
+  c.inv_cbrt_fz1 = 1.0 / c.cbrt_fz1;
+  c.inv_cbrt2_fz1 = 1.0 / c.cbrt2_fz1;
+  c.inv_d1 = c.d1 > 0.0 ? 1.0 / c.d1 : 0.0;
+  c.inv_d2 = c.d2 > 0.0 ? 1.0 / c.d2 : 0.0;
+  c.ln_eee = std::log(c.eee);
   return c;
 }
 

@@ -25,13 +25,17 @@ __device__ __forceinline__ R mn(R a, R b) { return (b < a) ? b : a; }  // std::m
 template <typename R>
 __device__ __forceinline__ R sq(R x) { return x * x; }
 
-// precision-dispatched math: FP64 exp/log/pow from mt_fmath.h (a few ulp,
-// a third to a quarter of the library's instructions), single precision
-// the hardware-assisted ocml variants
+// precision-dispatched math: FP64 from mt_fmath.h (within a few ulp, a
+// third to a half of the library's instructions), single precision the
+// hardware-assisted variants
 __device__ __forceinline__ double m_exp(double x) { return fx_exp(x); }
 __device__ __forceinline__ double m_log(double x) { return fx_log(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return fx_pow(x, y); }
-__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return fx_sqrt(x); }
+__device__ __forceinline__ double m_rcp(double x) { return fx_rcp(x); }
+__device__ __forceinline__ float m_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double m_rsqrt(double x) { return fx_rsqrt(x); }
+__device__ __forceinline__ float m_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double m_abs(double x) { return fabs(x); }
 __device__ __forceinline__ float m_exp(float x) { return __expf(x); }
 __device__ __forceinline__ float m_log(float x) { return __logf(x); }
@@ -41,20 +45,24 @@ __device__ __forceinline__ float m_abs(float x) { return fabsf(x); }
 
 #define K(x) R(x)
 
+// The stopping fits below keep the reference's formulas; divisions by
+// per-(ion, element) constants become multiplications by their reciprocals
+// (StopConst.inv_*), powers share one logarithm of their base, and
+// (e / eee)^power = exp(power (log e - log eee)) reuses log e.
 template <typename R>
 __device__ __forceinline__ R rpstop(const StopConst& c, R e) {  // material.C:134-158
-  const R pe0 = K(25.0);
-  const R pe = mx(pe0, e);
-  const R sl = (R(c.pc[0]) * m_pow(pe, R(c.pc[1]))) + (R(c.pc[2]) * m_pow(pe, R(c.pc[3])));
-  const R sh = R(c.pc[4]) / m_pow(pe, R(c.pc[5])) * m_log(R(c.pc[6]) / pe + R(c.pc[7]) * pe);
-  R sp = sl * sh / (sl + sh);
-  if (e <= pe0) sp *= m_pow(e / pe0, R(c.velpwr));
+  const R pe = mx(K(25.0), e);
+  const R lpe = m_log(pe);
+  const R sl = (R(c.pc[0]) * m_exp(R(c.pc[1]) * lpe)) + (R(c.pc[2]) * m_exp(R(c.pc[3]) * lpe));
+  const R sh = R(c.pc[4]) * m_exp(-R(c.pc[5]) * lpe) * m_log(R(c.pc[6]) * m_rcp(pe) + R(c.pc[7]) * pe);
+  R sp = sl * sh * m_rcp(sl + sh);
+  if (e <= K(25.0)) sp *= m_pow(e * K(0.04), R(c.velpwr));
   return sp;
 }
 
 template <typename R>
 __device__ __forceinline__ R rstop(const StopConst& c, R E) {  // material.C:161-282
-  const R e = (K(0.001) * E) / R(c.m1);
+  const R e = (K(0.001) * E) * R(c.inv_m1);
   R se;
   if (c.kind == 1) {
     se = rpstop(c, e);
@@ -70,37 +78,38 @@ __device__ __forceinline__ R rstop(const StopConst& c, R E) {  // material.C:161
     a = K(1.0) + R(c.he_k) * m_exp(-sq(K(7.6) - b));
     heh *= a * a;
     se = rpstop(c, he) * heh * K(4.0);
-    if (e <= he0) se *= m_sqrt(e / he0);
+    if (e <= he0) se *= m_sqrt(e);
   } else {
     const R vf = R(c.vfermi);
-    const R v = m_sqrt(e / K(25.0)) / vf;
+    const R v = m_sqrt(e * K(0.04)) * R(c.inv_vfermi);
     const R v2 = v * v;
     R vr;
     if (v >= K(1.0))
-      vr = v * vf * (K(1.0) + K(1.0) / (K(5.0) * v2));
+      vr = v * vf * (K(1.0) + m_rcp(K(5.0) * v2));
     else
-      vr = R(c.vk) * (K(1.0) + (K(2.0) * v2 / K(3.0)) - v2 * v2 / K(15.0));
-    R yr = mx(K(0.13), vr / R(c.cbrt2_fz1));
+      vr = R(c.vk) * (K(1.0) + v2 * K(2.0 / 3.0) - v2 * v2 * K(1.0 / 15.0));
+    R yr = mx(K(0.13), vr * R(c.inv_cbrt2_fz1));
     yr = mx(yr, R(c.yr_lo));
-    const R yr03 = m_pow(yr, K(0.3));
+    const R yr03 = m_exp(K(0.3) * m_log(yr));
     R a = K(-0.803) * yr03 + K(1.3167) * yr03 * yr03 + K(0.38157) * yr + K(0.008983) * yr * yr;
     const R q = mn(K(1.0), mx(K(0.0), K(1.0) - m_exp(-mn(a, K(50.0)))));
-    const R l0 = (K(0.8) - q * R(c.l0k)) / R(c.cbrt_fz1);
+    const R l0 = (K(0.8) - q * R(c.l0k)) * R(c.inv_cbrt_fz1);
     R l1;
     if (q < K(0.2))
       l1 = K(0.0);
     else if (q < R(c.q1))
-      l1 = R(c.bq) * (q - K(0.2)) / R(c.d1);
+      l1 = R(c.bq) * (q - K(0.2)) * R(c.inv_d1);
     else if (q < R(c.q2))
       l1 = R(c.bq);
     else
-      l1 = R(c.bq) * (K(1.0) - q) / R(c.d2);
+      l1 = R(c.bq) * (K(1.0) - q) * R(c.inv_d2);
     const R l = mx(l1, l0 * R(c.lfctr));
-    R zeta = q + R(c.zk) * (K(1.0) - q) * m_log(K(1.0) + sq(K(4.0) * l * vf / K(1.919)));
-    a = -sq(K(7.6) - mx(K(0.0), m_log(e)));
+    R zeta = q + R(c.zk) * (K(1.0) - q) * m_log(K(1.0) + sq(l * vf * K(4.0 / 1.919)));
+    const R le = m_log(e);
+    a = -sq(K(7.6) - mx(K(0.0), le));
     zeta *= K(1.0) + R(c.z3) * m_exp(a);
     if (yr <= R(c.yr_branch))
-      se = R(c.sp_low) * sq(zeta * R(c.fz1)) * m_pow(e / R(c.eee), R(c.power));
+      se = R(c.sp_low) * sq(zeta * R(c.fz1)) * m_exp(R(c.power) * (le - R(c.ln_eee)));
     else
       se = rpstop(c, e) * sq(zeta * R(c.fz1));
   }
@@ -119,11 +128,89 @@ __device__ __forceinline__ R getrstop(const TableView& T, int s, int k, R E) {
   return se * R(mc.arho);
 }
 
+// Screening sums of the interatomic potential V(r) = S(r) / r with
+// S = sum c_i exp(-d_i r) and D = sum c_i d_i exp(-d_i r) = -S'(r), for the
+// universal (POT 0), Moliere (1) and C-Kr (2) screenings of trim.C:205-232.
+// The reference's v = V and v1 = V' follow as v = S / r, v1 = -(v + D) / r.
+template <typename R, int POT>
+__device__ __forceinline__ void screening(R r, R& S, R& D) {
+  if (POT == 1) {
+    const R ex1 = m_exp(K(-0.3) * r);
+    const R ex2 = sq(ex1) * sq(ex1);
+    const R ex3 = ex2 * (sq(ex2) * sq(ex2));
+    S = K(0.35) * ex1 + K(0.55) * ex2 + K(0.1) * ex3;
+    D = K(0.105) * ex1 + K(0.66) * ex2 + K(0.6) * ex3;
+  } else if (POT == 2) {
+    const R ex1 = m_exp(K(-0.279) * r);
+    const R ex2 = m_exp(K(-0.637) * r);
+    const R ex3 = m_exp(K(-1.1919) * r);
+    S = K(0.191) * ex1 + K(0.474) * ex2 + K(0.335) * ex3;
+    D = K(0.531865) * ex1 + K(0.30181) * ex2 + K(0.6437) * ex3;
+  } else {
+    const R ex1 = K(0.18175) * m_exp(K(-3.1998) * r);
+    const R ex2 = K(0.50986) * m_exp(K(-0.94229) * r);
+    const R ex3 = K(0.28022) * m_exp(K(-0.4029) * r);
+    const R ex4 = K(0.028171) * m_exp(K(-0.20162) * r);
+    S = ex1 + ex2 + ex3 + ex4;
+    D = K(3.1998) * ex1 + K(0.94229) * ex2 + K(0.4029) * ex3 + K(0.20162) * ex4;
+  }
+}
+
+// Newton solve for the apsis (trim.C:200-240): f(r) = b^2/r + v r/eps - r,
+// f'(r) = -b^2/r^2 + (v + v1 r)/eps - 1, r -= f / f' until |q / r| <= 0.001.
+// With v = S/r and v + v1 r = -D the iteration needs one reciprocal of r and
+// one of f' (the reference divides seven times):
+//   f = b^2 (1/r) + S / eps - r,  f' = -b^2 (1/r)^2 - D / eps - 1.
+// Returns the iteration count; v and v1 are those of the last r evaluated
+// (before its update), as in the reference.
+template <typename R, int POT>
+__device__ __forceinline__ int apsis(R eps, R b, R& r, R& v, R& v1) {
+  const R bb = b * b, ieps = m_rcp(eps);
+  R q;
+  int it = 0;
+  do {
+    R S, D;
+    screening<R, POT>(r, S, D);
+    const R ir = m_rcp(r);
+    v = S * ir;
+    v1 = -(v + D) * ir;
+    const R fr = bb * ir + S * ieps - r;
+    const R fr1 = -(bb * (ir * ir)) - D * ieps - K(1.0);
+    q = fr * m_rcp(fr1);
+    r -= q;
+    ++it;
+  } while (m_abs(q) > K(0.001) * m_abs(r) && it < kMaxNewton);
+  return it;
+}
+
+// MAGIC coefficients (trim.C:236-262): cc, aa = 2 eps (1 + c/sqrt(eps)) b^cc
+// and ff for the three screenings; 1/sqrt(eps) comes from one reciprocal
+// square root and the quotients from reciprocals
+template <typename R, int POT>
+__device__ __forceinline__ void magic(R eps, R b, R& aa, R& ff) {
+  const R ise = m_rsqrt(eps);
+  const R sqe = eps * ise;
+  R cc;
+  if (POT == 1) {
+    cc = (K(0.009611) + sqe) * m_rcp(K(0.005175) + sqe);
+    aa = K(2.0) * eps * (K(1.0) + K(0.6743) * ise) * m_pow(b, cc);
+    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6.314) + eps) * m_rcp(K(10.0) + eps));
+  } else if (POT == 2) {
+    cc = (K(0.235809) + sqe) * m_rcp(K(0.126000) + sqe);
+    aa = K(2.0) * eps * (K(1.0) + K(1.0144) * ise) * m_pow(b, cc);
+    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6935.0) + eps) * m_rcp(K(83550.0) + eps));
+  } else {
+    cc = (K(0.011615) + sqe) * m_rcp(K(0.0071222) + sqe);
+    aa = K(2.0) * eps * (K(1.0) + K(0.99229) * ise) * m_pow(b, cc);
+    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(9.3066) + eps) * m_rcp(K(14.813) + eps));
+  }
+}
+
 // trim.C:172-272
 template <typename R, int POT>
 __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& iters) {
   if (eps > K(10.0)) {
-    s2 = K(1.0) / (K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
+    s2 = m_rcp(K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
     const R c2 = K(1.0) - s2;
     ct = K(2.0) * c2 - K(1.0);
     st = m_sqrt(K(1.0) - ct * ct);
@@ -136,53 +223,13 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
     rr = K(-2.7) * m_log(eps * rr);
     if (rr >= b) r = rr;
   }
-  R v, v1, q;
-  int it = 0;
-  do {
-    if (POT == 1) {  // Moliere
-      const R ex1 = m_exp(K(-0.3) * r);
-      const R ex2 = sq(ex1) * sq(ex1);
-      const R ex3 = ex2 * (sq(ex2) * sq(ex2));
-      v = (K(0.35) * ex1 + K(0.55) * ex2 + K(0.1) * ex3) / r;
-      v1 = -(v + K(0.105) * ex1 + K(0.66) * ex2 + K(0.6) * ex3) / r;
-    } else if (POT == 2) {  // C-Kr
-      const R ex1 = m_exp(K(-0.279) * r);
-      const R ex2 = m_exp(K(-0.637) * r);
-      const R ex3 = m_exp(K(-1.1919) * r);
-      v = (K(0.191) * ex1 + K(0.474) * ex2 + K(0.335) * ex3) / r;
-      v1 = -(v + K(0.531865) * ex1 + K(0.30181) * ex2 + K(0.6437) * ex3) / r;
-    } else {  // universal
-      const R ex1 = K(0.18175) * m_exp(K(-3.1998) * r);
-      const R ex2 = K(0.50986) * m_exp(K(-0.94229) * r);
-      const R ex3 = K(0.28022) * m_exp(K(-0.4029) * r);
-      const R ex4 = K(0.028171) * m_exp(K(-0.20162) * r);
-      v = (ex1 + ex2 + ex3 + ex4) / r;
-      v1 = -(v + K(3.1998) * ex1 + K(0.94229) * ex2 + K(0.4029) * ex3 + K(0.20162) * ex4) / r;
-    }
-    const R fr = b * b / r + v * r / eps - r;
-    const R fr1 = -b * b / (r * r) + (v + v1 * r) / eps - K(1.0);
-    q = fr / fr1;
-    r -= q;
-    ++it;
-  } while (m_abs(q / r) > K(0.001) && it < kMaxNewton);
-  const R roc = K(-2.0) * (eps - v) / v1;
-  const R sqe = m_sqrt(eps);
-  R cc, aa, ff;
-  if (POT == 1) {
-    cc = (K(0.009611) + sqe) / (K(0.005175) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + (K(0.6743) / sqe)) * m_pow(b, cc);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6.314) + eps) / (K(10.0) + eps));
-  } else if (POT == 2) {
-    cc = (K(0.235809) + sqe) / (K(0.126000) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + (K(1.0144) / sqe)) * m_pow(b, cc);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6935.0) + eps) / (K(83550.0) + eps));
-  } else {
-    cc = (K(0.011615) + sqe) / (K(0.0071222) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + (K(0.99229) / sqe)) * m_pow(b, cc);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(9.3066) + eps) / (K(14.813) + eps));
-  }
-  const R delta = (r - b) * aa * ff / (ff + K(1.0));
-  const R co = (b + delta + roc) / (r + roc);
+  R v, v1;
+  const int it = apsis<R, POT>(eps, b, r, v, v1);
+  const R roc = K(-2.0) * (eps - v) * m_rcp(v1);
+  R aa, ff;
+  magic<R, POT>(eps, b, aa, ff);
+  const R delta = (r - b) * aa * ff * m_rcp(ff + K(1.0));
+  const R co = (b + delta + roc) * m_rcp(r + roc);
   const R c2 = co * co;
   s2 = K(1.0) - c2;
   ct = K(2.0) * c2 - K(1.0);
@@ -203,7 +250,7 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
 template <typename R, int POT>
 __device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, int& iters) {
   if (eps > K(10.0)) {
-    s2 = K(1.0) / (K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
+    s2 = m_rcp(K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
     ct = K(1.0) - K(2.0) * s2;
     st = K(2.0) * m_sqrt(s2 * (K(1.0) - s2));
     iters = 0;
@@ -215,57 +262,14 @@ __device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, 
     rr = K(-2.7) * m_log(eps * rr);
     if (rr >= b) r = rr;
   }
-  R v, v1, q;
-  int it = 0;
-  auto pot = [](R x, R& vv, R& vv1) {
-    if (POT == 1) {
-      const R ex1 = m_exp(K(-0.3) * x);
-      const R ex2 = sq(ex1) * sq(ex1);
-      const R ex3 = ex2 * (sq(ex2) * sq(ex2));
-      vv = (K(0.35) * ex1 + K(0.55) * ex2 + K(0.1) * ex3) / x;
-      vv1 = -(vv + K(0.105) * ex1 + K(0.66) * ex2 + K(0.6) * ex3) / x;
-    } else if (POT == 2) {
-      const R ex1 = m_exp(K(-0.279) * x);
-      const R ex2 = m_exp(K(-0.637) * x);
-      const R ex3 = m_exp(K(-1.1919) * x);
-      vv = (K(0.191) * ex1 + K(0.474) * ex2 + K(0.335) * ex3) / x;
-      vv1 = -(vv + K(0.531865) * ex1 + K(0.30181) * ex2 + K(0.6437) * ex3) / x;
-    } else {
-      const R ex1 = K(0.18175) * m_exp(K(-3.1998) * x);
-      const R ex2 = K(0.50986) * m_exp(K(-0.94229) * x);
-      const R ex3 = K(0.28022) * m_exp(K(-0.4029) * x);
-      const R ex4 = K(0.028171) * m_exp(K(-0.20162) * x);
-      vv = (ex1 + ex2 + ex3 + ex4) / x;
-      vv1 = -(vv + K(3.1998) * ex1 + K(0.94229) * ex2 + K(0.4029) * ex3 + K(0.20162) * ex4) / x;
-    }
-  };
-  do {
-    pot(r, v, v1);
-    const R fr = b * b / r + v * r / eps - r;
-    const R fr1 = -b * b / (r * r) + (v + v1 * r) / eps - K(1.0);
-    q = fr / fr1;
-    r -= q;
-    ++it;
-  } while (m_abs(q / r) > K(0.001) && it < kMaxNewton);
-  const R roc = K(-2.0) * (eps - v) / v1;
-  const R sqe = m_sqrt(eps);
-  R cc, aa, ff;
-  if (POT == 1) {
-    cc = (K(0.009611) + sqe) / (K(0.005175) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + (K(0.6743) / sqe)) * m_pow(b, cc);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6.314) + eps) / (K(10.0) + eps));
-  } else if (POT == 2) {
-    cc = (K(0.235809) + sqe) / (K(0.126000) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + (K(1.0144) / sqe)) * m_pow(b, cc);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6935.0) + eps) / (K(83550.0) + eps));
-  } else {
-    cc = (K(0.011615) + sqe) / (K(0.0071222) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + (K(0.99229) / sqe)) * m_pow(b, cc);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(9.3066) + eps) / (K(14.813) + eps));
-  }
-  R vr, v1r;
-  pot(r, vr, v1r);
-  const R rmb = vr * r * r / (eps * (r + b));
+  R v, v1;
+  const int it = apsis<R, POT>(eps, b, r, v, v1);
+  const R roc = K(-2.0) * (eps - v) * m_rcp(v1);
+  R aa, ff;
+  magic<R, POT>(eps, b, aa, ff);
+  R Sr, Dr;
+  screening<R, POT>(r, Sr, Dr);
+  const R rmb = Sr * r / (eps * (r + b));  // v(r) r^2 = S(r) r
   const R A = aa * ff / (ff + K(1.0));
   const R omc = rmb * (K(1.0) - A) / (r + roc);
   s2 = omc * (K(2.0) - omc);

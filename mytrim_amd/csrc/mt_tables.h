@@ -47,6 +47,10 @@ struct StopConst {  // per (species, element): rstop(ion, Z2)
   double z3;                    // (1/Z1^2) (0.18 + 0.0015 Z2)
   double eee, sp_low, power;    // low-velocity branch
   double he_k;                  // 0.007 + 0.00005 Z2
+  // reciprocals and logarithms of the constants above, so that the kernel
+  // multiplies where the reference divides
+  double inv_m1, inv_vfermi, inv_cbrt_fz1, inv_cbrt2_fz1, inv_d1, inv_d2;
+  double ln_eee;                // log(eee)
   int kind;                     // 1 proton, 2 helium, 3 heavy ion
   int pad;
 };

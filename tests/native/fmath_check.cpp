@@ -22,6 +22,8 @@ int main(int argc, char** argv) {
   std::mt19937_64 g(argc > 2 ? std::atoll(argv[2]) : 1);
   std::uniform_real_distribution<double> u(0.0, 1.0);
   double e_exp = 0, e_log = 0, e_pow = 0, e_pow_rel = 0, e_powp = 0;
+  double e_rcp = 0, e_rsqrt = 0, e_sqrt = 0, e_sin = 0, e_cos = 0;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
   for (long i = 0; i < n; ++i) {
     const double xe = -700.0 + 1400.0 * u(g);
     e_exp = std::fmax(e_exp, ulp_err(mt::fx_exp(xe), std::exp((long double)xe)));
@@ -37,7 +39,33 @@ int main(int argc, char** argv) {
     const double t = std::fabs(yp * std::log(xp));
     e_pow_rel = std::fmax(e_pow_rel, ep / (t + 1.0));
     e_powp = std::fmax(e_powp, (double)(std::fabs((long double)mt::fx_pow(xp, yp) - wp) / std::fabs(wp)));
+    // rcp / rsqrt / sqrt over [1e-30, 1e30] (the host seed is single precision)
+    const double xr = std::pow(10.0, -30.0 + 60.0 * u(g)) * (u(g) < 0.5 ? -1.0 : 1.0);
+    e_rcp = std::fmax(e_rcp, ulp_err(mt::fx_rcp(xr), 1.0L / (long double)xr));
+    const double xs = std::fabs(xr);
+    e_rsqrt = std::fmax(e_rsqrt, ulp_err(mt::fx_rsqrt(xs), 1.0L / std::sqrt((long double)xs)));
+    e_sqrt = std::fmax(e_sqrt, ulp_err(mt::fx_sqrt(xs), std::sqrt((long double)xs)));
+    // sin / cos of 2 pi u on the (0, 1] grid of 32-bit draws; error in ulp of 1
+    // (absolute), since both pass through zero inside the interval
+    const double uu = ((double)(uint32_t)g() + 1.0) * (1.0 / 4294967296.0);
+    double sn, cs;
+    mt::fx_sincos2pi(uu, sn, cs);
+    const long double ang = two_pi * (long double)uu;
+    e_sin = std::fmax(e_sin, (double)(std::fabs((long double)sn - std::sin(ang)) / 0x1p-53L));
+    e_cos = std::fmax(e_cos, (double)(std::fabs((long double)cs - std::cos(ang)) / 0x1p-53L));
   }
+  const bool zero_ok = mt::fx_sqrt(0.0) == 0.0;
+  // special arguments: log(0) = -inf, log(inf) = inf, log(<0) and log(NaN)
+  // NaN, exp(-inf) = 0, exp(inf) = inf, exp(NaN) NaN, pow(x, 0) = 1,
+  // exp(-740) subnormal, exp(-800) = 0
+  const double inf = HUGE_VAL, nan = std::nan("");
+  const bool special_ok = mt::fx_log(0.0) == -inf && mt::fx_log(inf) == inf && std::isnan(mt::fx_log(-1.0)) &&
+                          std::isnan(mt::fx_log(nan)) && mt::fx_exp(-inf) == 0.0 && mt::fx_exp(inf) == inf &&
+                          std::isnan(mt::fx_exp(nan)) && mt::fx_pow(0.0, 0.0) == 1.0 && mt::fx_pow(2.0, 0.0) == 1.0 &&
+                          mt::fx_exp(-740.0) > 0.0 && mt::fx_exp(-740.0) < 1e-300 && mt::fx_exp(-800.0) == 0.0;
+  std::printf("special_ok %d\n", (int)special_ok);
+  std::printf("rcp_ulp %.3f\nrsqrt_ulp %.3f\nsqrt_ulp %.3f\nsin_abs_ulp1 %.3f\ncos_abs_ulp1 %.3f\nsqrt_zero_ok %d\n",
+              e_rcp, e_rsqrt, e_sqrt, e_sin, e_cos, (int)zero_ok);
   std::printf("exp_ulp %.3f\nlog_ulp %.3f\npow_ulp %.3f\npow_ulp_per_unit_ylogx %.3f\npow_rel %.3e\n", e_exp, e_log,
               e_pow, e_pow_rel, e_powp);
   return 0;

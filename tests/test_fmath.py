@@ -1,18 +1,24 @@
 """Accuracy of the kernel's FP64 exp / log / pow (mytrim_amd/csrc/mt_fmath.h).
 
 The header is compiled for the host (the same source the device runs; the
-device build differs only in frexp/ldexp, which are exact either way) into
+device build differs in frexp/ldexp, which are exact either way, and in the
+seeds of rcp/rsqrt, the hardware estimates instead of single precision) into
 tests/native/fmath_check.cpp, which measures the worst error against long
 double references on random arguments:
 
 * exp on [-700, 700], log on [1e-300, 1e300] and on [0.5, 2] (near 1, where
   the result is small),
 * pow for x in [1e-8, 1e6], y in [-2, 2], the ranges the physics uses
-  (pmax's eeg^0.1, the Kinchin-Pease pows, the stopping fits).
+  (pmax's eeg^0.1, the Kinchin-Pease pows, the stopping fits),
+* rcp, rsqrt and sqrt on +-[1e-30, 1e30] from a single-precision seed (the
+  device's hardware estimate is of the same order), sqrt(0) = 0,
+* sin and cos of 2 pi u on the grid of 32-bit draws u in (0, 1], where the
+  error is absolute (in units of 2^-53) since both cross zero.
 
-Bounds: exp <= 1 ulp, log <= 2 ulp, pow <= 2 ulp per unit of |y ln x| + 1
-and 1e-14 relative. A dropped series term, a wrong ln 2 split or a missing
-reduction step each raise one of these by orders of magnitude.
+Bounds: exp, rcp, rsqrt, sqrt <= 1 ulp, log <= 2 ulp, pow <= 2 ulp per unit
+of |y ln x| + 1 and 1e-14 relative, sin/cos <= 2 units of 2^-53. A dropped
+series term, a wrong ln 2 or 2 pi split, a missing Newton step or a wrong
+quadrant each raise one of these by orders of magnitude.
 """
 import os
 import shutil
@@ -36,6 +42,10 @@ def fmath_report(tmp_path_factory):
     return {k: float(v) for k, v in (line.split() for line in out.strip().splitlines())}
 
 
+def test_special_arguments(fmath_report):
+    assert fmath_report["special_ok"] == 1
+
+
 def test_exp_within_one_ulp(fmath_report):
     assert fmath_report["exp_ulp"] <= 1.0
 
@@ -47,3 +57,15 @@ def test_log_within_two_ulp(fmath_report):
 def test_pow_error_tracks_y_log_x(fmath_report):
     assert fmath_report["pow_ulp_per_unit_ylogx"] <= 2.0
     assert fmath_report["pow_rel"] <= 1e-14
+
+
+def test_reciprocal_and_square_roots(fmath_report):
+    assert fmath_report["rcp_ulp"] <= 1.0
+    assert fmath_report["rsqrt_ulp"] <= 1.0
+    assert fmath_report["sqrt_ulp"] <= 1.0
+    assert fmath_report["sqrt_zero_ok"] == 1
+
+
+def test_sincos2pi(fmath_report):
+    assert fmath_report["sin_abs_ulp1"] <= 2.0
+    assert fmath_report["cos_abs_ulp1"] <= 2.0

@@ -5,10 +5,12 @@ Tolerances (FP64 path):
 * stopping (getrstop): relative 1e-12 against the reference's own values.
 * scattering: |d cos(theta)| <= 1e-12 and |d s2| <= 1e-12 s2 + 8 ulp(1); s2 =
   1 - co^2 is formed by a subtraction from 1, so glancing collisions carry an
-  absolute rounding floor whatever the libm.
+  absolute rounding floor whatever the libm; sin(theta) to 1e-12 where it is
+  >= 1e-2, below that through sin^2 = 1 - cos^2.
 * cascades: the device draw scheme is restated by the oracle (rng=1), so
-  histories are compared one by one; device and host libm differ in the last
-  ulp, which can flip a rare threshold decision deep in a long cascade, so
+  histories are compared one by one; the device's elementary functions and
+  the host libm differ in the last ulps, which can flip a rare threshold
+  decision deep in a long cascade, so
   >= 97 % of histories must agree exactly in every integer tally and totals
   within 1 %.
 * against the reference (std::mt19937 streams): statistics within 2.5 sigma.
@@ -73,7 +75,12 @@ def test_scatter_matches_oracle(eng, potential):
     print("max |d ct| %.3g, max |d s2| - tol %.3g" % (d_ct.max(), (d_s2 - 1e-12 * ref[:, 0]).max()))
     assert np.all(d_ct <= 1e-12)
     assert np.all(d_s2 <= 1e-12 * ref[:, 0] + 8 * 2.0 ** -52)
-    assert np.all(np.abs(st - ref[:, 2]) <= 1e-12 + 1e-12 * ref[:, 2])
+    # st = sqrt(1 - ct^2) (trim.C:178, 271) amplifies a difference in ct by ct/st:
+    # where st >= 1e-2 it must match to 1e-12; everywhere st^2 = 1 - ct^2 must
+    # differ by no more than the ct difference explains, 2 |d ct| + rounding
+    good = ref[:, 2] >= 1e-2
+    assert np.all(np.abs(st - ref[:, 2])[good] <= 1e-12 + 1e-12 * ref[:, 2][good])
+    assert np.all(np.abs(st * st - ref[:, 2] ** 2) <= 2 * d_ct + 8 * 2.0 ** -52)
     assert np.mean(it == ref[:, 3]) >= 0.999
 
 
