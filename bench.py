@@ -96,6 +96,15 @@ def tally_errors(ref, r):
     return out
 
 
+def _tuned(eng, sample, arith, n_sample):
+    """Autotune the launch shape for one precision (kept for the next runs)."""
+    if n_sample <= 0:
+        return None
+    best, rate = eng.autotune(sample, sample=sample.n_hist, arith=arith)
+    return {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
+            "sample_rate": rate}
+
+
 def timed(eng, prob, arith, n_hist, steps):
     bounds = [n_hist * i // steps for i in range(steps + 1)]
     coll, kms, res = 0, 0.0, []
@@ -161,15 +170,20 @@ def main():
     if args.variant_histories > 0:
         nv = min(args.variant_histories, args.histories)
         c64, w64, k64, r64 = timed(eng, prob, gpu.MT_FP64, nv, 1)
+        # each precision runs at its own autotuned launch shape (the FP32
+        # kernels need fewer registers, so more waves fit)
+        sample = prob.with_(hist0=2 * args.histories, n_hist=max(args.autotune_sample, 2000))
+        t32 = _tuned(eng, sample, gpu.MT_FP32, args.autotune_sample)
         c32, w32, k32, r32 = timed(eng, prob, gpu.MT_FP32, nv, 1)
         sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)
+        tsur = _tuned(eng, sample, gpu.MT_FP32_SURROGATE, args.autotune_sample)
         cs, ws, ks, rs = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
         variants = {
             "histories": nv,
             "fp64": {"collisions_per_s": c64 / w64, "kernel_ms": k64},
             "fp32": {"collisions_per_s": c32 / w32, "kernel_ms": k32, "speedup_vs_fp64": (c32 / w32) / (c64 / w64),
-                     "tally_rel_err_vs_fp64": tally_errors(r64[0], r32[0])},
-            "fp32_surrogate": {"collisions_per_s": cs / ws, "kernel_ms": ks,
+                     "tuning": t32, "tally_rel_err_vs_fp64": tally_errors(r64[0], r32[0])},
+            "fp32_surrogate": {"collisions_per_s": cs / ws, "kernel_ms": ks, "tuning": tsur,
                                "speedup_vs_fp64": (cs / ws) / (c64 / w64),
                                "surrogate_share": rs[0]["surrogate"] / max(rs[0]["n_coll"], 1),
                                "tally_rel_err_vs_fp64": tally_errors(r64[0], rs[0]),

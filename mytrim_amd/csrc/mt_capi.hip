@@ -756,10 +756,13 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   const mt_tuning keep = ctx->tune;
   // candidate shapes (threads per block, resident blocks per CU, LDS stack
   // entries per wave) around the region measured best on MI355X: one or two
-  // waves per SIMD with deep stacks; queue capacity stays at its default
+  // waves per SIMD with deep stacks, three for the single-precision kernels
+  // (fewer registers; a kernel whose registers do not allow the requested
+  // blocks per CU runs at the most that fit); queue capacity stays at its
+  // default
   static const int kCand[][3] = {{256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {128, 4, 192}, {128, 4, 128},
-                                 {64, 8, 160}, {64, 8, 128}, {256, 1, 384}, {256, 1, 256}, {128, 3, 256},
-                                 {128, 2, 384}, {64, 6, 192}};
+                                 {64, 8, 160},  {64, 8, 128},  {256, 1, 384}, {256, 1, 256}, {128, 3, 256},
+                                 {128, 2, 384}, {64, 6, 192},  {256, 3, 128}, {256, 3, 96},  {128, 6, 128}};
   std::vector<mt_tuning> cand;
   for (const auto& c : kCand) {
     mt_tuning t{};

@@ -7,19 +7,27 @@
 //
 // Work distribution
 //   * recoils pushed by a wave go to its LDS stack (LIFO, depth-first); what
-//     does not fit goes to an HBM ring of structure-of-arrays slots with a
-//     lap tag each (one atomic per wave push/pop; release/acquire fences at
-//     agent scope so any XCD can take them)
-//   * primaries are claimed in wave batches from a counter, and only while
-//     the HBM queue holds fewer than admit_limit recoils
+//     does not fit goes to one of up to 16 HBM queue shards (structure-of-
+//     arrays rings, head/tail words on their own cache lines; a wave pushes
+//     to the home shard of its XCD and steals from others when that is empty)
+//   * an entry is handed over without cache write-back/invalidate fences:
+//     its fields are stored and loaded at agent scope and a ready word
+//     (lap << 32 | checksum of the fields) tells a reader that all of them
+//     have landed
+//   * primaries are claimed in wave batches from a counter under a cascade
+//     budget: live cascades x expected flights per cascade stay within half
+//     the queue capacity
 //   * termination: a wave that finds no work registers idle in a packed
 //     (epoch | live | idle) word; every wave leaves once all live waves are
-//     idle, the queue is empty and the primaries are used up, observed twice
-//     with the same epoch.  Waits are bounded by a wall-clock limit.
+//     idle, every shard is empty and the primaries are used up, observed
+//     twice with the same epoch.  Waits are bounded by a wall-clock limit.
 //
-// Tallies: wave-uniform counters in scalar registers, per-lane compensated
-// FP64 energy sums, histogram bins in LDS (or HBM atomics) flushed at exit.
+// Tallies: per-lane 32-bit event counters summed over the wave at exit,
+// per-lane compensated FP64 energy sums, histogram bins in LDS (or HBM
+// atomics) flushed at exit.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "mt_cascade.h"
 #include "mt_geometry.h"
@@ -161,15 +169,15 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     const float Ef = (float)E;
     float eps = Ef * (float)mc.f;
     const float eeg = m_sqrt(eps * (float)mc.epsdg);
-    const float pmax = (float)mc.a / (eeg + m_sqrt(eeg) + 0.125f * m_pow(eeg, 0.1f));
-    float lsf = 1.0f / (3.14159265358979f * (pmax * pmax) * (float)mc.arho);
+    const float pmax = (float)mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125f * m_pow(eeg, 0.1f));
+    float lsf = m_rcp(3.14159265358979f * (pmax * pmax) * (float)mc.arho);
     if (ic == 1) {
       const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
       lsf = u24f(b0.y) * mn(lsf, (float)cw);
     }
     const float p = pmax * m_sqrt(u24f(blk.x));
     eps = (float)ec.fi * Ef;
-    const float b = p / (float)ec.ai;
+    const float b = p * m_rcp((float)ec.ai);
     ls = (double)lsf;
     dee = ls * (double)getrstop<float>(T, sp, k, Ef);
     float s2f = -1.0f, ctf, stf;
@@ -251,12 +259,16 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     }
   };
   unsigned long long idle_since = 0;
-  unsigned long long c_coll = 0, c_flight = 0, c_vac = 0, c_repl = 0, c_subst = 0, c_inter = 0,
-                     c_lost = 0, c_vacuum = 0, c_ruth = 0, c_disp = 0, c_rangen = 0, c_surr = 0;
   unsigned long long c_qops = 0;
   // per-lane accumulators
   // (32-bit per lane: a lane runs at most a few 1e7 collisions per launch)
-  unsigned l_newton32 = 0, l_kpvac32 = 0;
+  // Event counts are kept per lane (VGPRs) rather than per wave: a wave-wide
+  // count of every event in every iteration would hold a dozen 64-bit scalar
+  // registers and cost a ballot and a population count each.  Collisions are
+  // counted when the flight ends (its collision index), displacements are
+  // vacancies + replacements + substitutions.
+  unsigned l_newton32 = 0, l_kpvac32 = 0, l_coll32 = 0, l_ruth32 = 0, l_surr32 = 0, l_vac32 = 0;
+  unsigned l_flight32 = 0, l_fate32[4] = {0u, 0u, 0u, 0u}, l_vacuum32 = 0, l_rangen32 = 0;
   double eel_s = 0.0, eel_e = 0.0, enuc_s = 0.0, enuc_e = 0.0;
   unsigned err = 0;
 
@@ -500,6 +512,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         ruth = (it == 0);
         surr = (it < 0);
         l_newton32 += it > 0 ? (unsigned)it : 0u;
+        l_ruth32 += ruth ? 1u : 0u;
+        if constexpr (ARITH == 2) l_surr32 += surr ? 1u : 0u;
         if (it >= kMaxNewton) err |= kErrNewton;
         // energy transfer and momenta (trim.C:275-297)
         double den = ec.ec * s2 * F.E;
@@ -507,11 +521,14 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         F.E -= dee;
         kadd(eel_s, eel_e, dee);
         if (PERHIST) F.eel += dee;
-        const double m1 = H->species[F.sp].m;
-        const double p1 = m_sqrt(2.0 * m1 * F.E);
+        // The momenta and the deflection run in RD: FP64 on the FP64 path,
+        // FP32 on the single-precision paths (positions and energies stay FP64)
+        using RD = typename std::conditional<ARITH == 0, double, float>::type;
+        const RD m1 = RD(H->species[F.sp].m);
+        const RD p1 = m_sqrt(RD(2.0) * m1 * RD(F.E));
         if (den > F.E) den = F.E;
         F.E -= den;
-        const double p2 = m_sqrt(2.0 * m1 * F.E);
+        const RD p2 = m_sqrt(RD(2.0) * m1 * RD(F.E));
         // recoil at the collision site, ion moves on (trim.C:306-318)
         rx = F.px;
         ry = F.py;
@@ -520,44 +537,49 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         F.px += F.dx * step;
         F.py += F.dy * step;
         F.pz += F.dz * step;
-        rdx = F.dx * p1;
-        rdy = F.dy * p1;
-        rdz = F.dz * p1;
+        const RD dx = RD(F.dx), dy = RD(F.dy), dz = RD(F.dz);
+        RD rdxr = dx * p1, rdyr = dy * p1, rdzr = dz * p1;
         rE = den - ec.Elbind;
         // azimuth about the flight direction (replaces trim.C:320-333)
         // u: unit vector perpendicular to d in the plane of its two largest
         // components (x smallest: (0, dz, -dy); y: (-dz, 0, dx); z: (dy, -dx, 0)),
         // chosen by selects so that a wave evaluates one reciprocal square root
-        double ux, uy, uz;
+        RD ux, uy, uz;
         {
-          const double ax = fabs(F.dx), ay = fabs(F.dy), az = fabs(F.dz);
+          const RD ax = m_abs(dx), ay = m_abs(dy), az = m_abs(dz);
           const bool xs = ax <= ay && ax <= az, ys = !xs && ay <= az;
-          const double c0 = xs ? F.dy : F.dx, c1 = (xs || ys) ? F.dz : F.dy;
-          const double n = fx_rsqrt(c0 * c0 + c1 * c1);
-          ux = xs ? 0.0 : (ys ? -F.dz : F.dy) * n;
-          uy = xs ? F.dz * n : (ys ? 0.0 : -F.dx * n);
-          uz = xs ? -F.dy * n : (ys ? F.dx * n : 0.0);
+          const RD c0 = xs ? dy : dx, c1 = (xs || ys) ? dz : dy;
+          const RD n = m_rsqrt(c0 * c0 + c1 * c1);
+          ux = xs ? RD(0.0) : (ys ? -dz : dy) * n;
+          uy = xs ? dz * n : (ys ? RD(0.0) : -dx * n);
+          uz = xs ? -dy * n : (ys ? dx * n : RD(0.0));
         }
-        const double vx = F.dy * uz - F.dz * uy;
-        const double vy = F.dz * ux - F.dx * uz;
-        const double vz = F.dx * uy - F.dy * ux;
-        double sphi, cphi;
-        fx_sincos2pi(u32d(blk.w), sphi, cphi);
-        const double qx = cphi * ux + sphi * vx;
-        const double qy = cphi * uy + sphi * vy;
-        const double qz = cphi * uz + sphi * vz;
+        const RD vx = dy * uz - dz * uy;
+        const RD vy = dz * ux - dx * uz;
+        const RD vz = dx * uy - dy * ux;
+        RD sphi, cphi;
+        m_sincos2pi(u32d(blk.w), sphi, cphi);
+        const RD qx = cphi * ux + sphi * vx;
+        const RD qy = cphi * uy + sphi * vy;
+        const RD qz = cphi * uz + sphi * vz;
         // new direction (trim.C:336-341): psi = atan2(st, ct + my), so
         // cos psi and sin psi are the normalised arguments (st >= 0)
-        const double xpsi = ct + ec.my;
-        const double ipsi = fx_rsqrt(xpsi * xpsi + st * st);
-        const double cpsi = xpsi * ipsi;
-        const double spsi = st * ipsi;
-        F.dx = F.dx * cpsi + qx * spsi;
-        F.dy = F.dy * cpsi + qy * spsi;
-        F.dz = F.dz * cpsi + qz * spsi;
-        rdx -= F.dx * p2;
-        rdy -= F.dy * p2;
-        rdz -= F.dz * p2;
+        const RD xpsi = RD(ct) + RD(ec.my);
+        const RD ipsi = m_rsqrt(xpsi * xpsi + RD(st) * RD(st));
+        const RD cpsi = xpsi * ipsi;
+        const RD spsi = RD(st) * ipsi;
+        const RD ndx = dx * cpsi + qx * spsi;
+        const RD ndy = dy * cpsi + qy * spsi;
+        const RD ndz = dz * cpsi + qz * spsi;
+        F.dx = ndx;
+        F.dy = ndy;
+        F.dz = ndz;
+        rdxr -= ndx * p2;
+        rdyr -= ndy * p2;
+        rdzr -= ndz * p2;
+        rdx = rdxr;
+        rdy = rdyr;
+        rdz = rdzr;
         // boundaries (trim.C:344-352)
         bool lost = false;
         if (G.bc[0] == 2 && (F.px > G.w[0] || F.px < 0.0)) lost = true;
@@ -623,16 +645,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         }
       }
     }
-    // wave-uniform event counts
-    c_coll += __popcll(__ballot(active && !vacuum));
-    c_ruth += __popcll(__ballot(active && !vacuum && ruth));
-    c_surr += __popcll(__ballot(active && !vacuum && surr));
-    c_vac += __popcll(__ballot(active && state == -1));
-    c_repl += __popcll(__ballot(active && state == 1));
-    c_subst += __popcll(__ballot(active && state == 2));
-    c_inter += __popcll(__ballot(active && state == 3));
-    c_lost += __popcll(__ballot(active && state == 4));
-    c_disp += __popcll(__ballot(active && (state == -1 || state == 1 || state == 2)));
+    l_vac32 += (active && state == -1) ? 1u : 0u;
 
     // ------------------------------------------------------------ push recoils
     const uint64_t push_m = __ballot(push);
@@ -714,7 +727,6 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     const bool fin = active && (ended || vacuum);
     const uint64_t fin_m = __ballot(fin);
     if (fin_m) {
-      c_flight += __popcll(fin_m);
       // the decrement's old value is read one iteration later (close_pending)
       // so its latency overlaps the next collision
       close_pending();
@@ -724,8 +736,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       }
       fe_pending += __popcll(fin_m);
       if (lane == 0 && fe_pending >= 256ull) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
-      c_vacuum += __popcll(__ballot(fin && vacuum));
       if (fin) {
+        ++l_flight32;
+        l_coll32 += (unsigned)(vacuum ? F.ic - 1 : F.ic);
+        l_vacuum32 += vacuum ? 1u : 0u;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) l_fate32[f] += (state == f + 1) ? 1u : 0u;
+        l_rangen32 += (F.gen == 0 && !vacuum && state != 4) ? 1u : 0u;
         if (vacuum) atomicAdd(&P.energy[eEvac], F.E);
         const uint32_t hl = F.hist - (uint32_t)P.hist0;
         if (PERHIST) {
@@ -752,15 +769,22 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         active = false;
       }
       if (fe_pending >= 256ull) fe_pending = 0;
-      c_rangen += __popcll(__ballot(fin && F.gen == 0 && !vacuum && state != 4));
     }
   }
 
   // ------------------------------------------------------------ flush
-  long long l_newton = (long long)l_newton32, l_kpvac = (long long)l_kpvac32;
+  // wave sums of the per-lane counters (each 64-bit; lane sums are 32-bit)
+  auto wsum = [&](unsigned v) {
+    unsigned long long t = v;
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    return t;
+  };
+  const unsigned long long n_coll = wsum(l_coll32), n_flight = wsum(l_flight32), n_vac = wsum(l_vac32),
+                           n_repl = wsum(l_fate32[0]), n_subst = wsum(l_fate32[1]), n_inter = wsum(l_fate32[2]),
+                           n_lost = wsum(l_fate32[3]), n_vacuum = wsum(l_vacuum32), n_ruth = wsum(l_ruth32),
+                           n_newton = wsum(l_newton32), n_kpvac = wsum(l_kpvac32), n_rangen = wsum(l_rangen32),
+                           n_surr = wsum(l_surr32);
   for (int off = 32; off > 0; off >>= 1) {
-    l_newton += __shfl_xor(l_newton, off);
-    l_kpvac += __shfl_xor(l_kpvac, off);
     const double s_a = __shfl_xor(eel_s, off), e_a = __shfl_xor(eel_e, off);
     kadd(eel_s, eel_e, s_a);
     eel_e += e_a;
@@ -770,19 +794,19 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
   }
   if (lane == 0) {
     unsigned long long* C = P.counters;
-    atomicAdd(&C[cColl], c_coll);
-    atomicAdd(&C[cFlight], c_flight);
-    atomicAdd(&C[cVac], c_vac + (unsigned long long)l_kpvac);
-    atomicAdd(&C[cRepl], c_repl);
-    atomicAdd(&C[cSubst], c_subst);
-    atomicAdd(&C[cInter], c_inter);
-    atomicAdd(&C[cLost], c_lost);
-    atomicAdd(&C[cVacuum], c_vacuum);
-    atomicAdd(&C[cRuth], c_ruth);
-    atomicAdd(&C[cNewton], (unsigned long long)l_newton);
-    atomicAdd(&C[cDisp], c_disp);
-    atomicAdd(&C[cRangeN], c_rangen);
-    atomicAdd(&C[cSurr], c_surr);
+    atomicAdd(&C[cColl], n_coll);
+    atomicAdd(&C[cFlight], n_flight);
+    atomicAdd(&C[cVac], n_vac + n_kpvac);
+    atomicAdd(&C[cRepl], n_repl);
+    atomicAdd(&C[cSubst], n_subst);
+    atomicAdd(&C[cInter], n_inter);
+    atomicAdd(&C[cLost], n_lost);
+    atomicAdd(&C[cVacuum], n_vacuum);
+    atomicAdd(&C[cRuth], n_ruth);
+    atomicAdd(&C[cNewton], n_newton);
+    atomicAdd(&C[cDisp], n_vac + n_repl + n_subst);
+    atomicAdd(&C[cRangeN], n_rangen);
+    atomicAdd(&C[cSurr], n_surr);
     atomicAdd(&ctrl[kQueueOps], c_qops);
     if (fe_pending) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
     const int gw = blockIdx.x * nwb + wid;

@@ -36,10 +36,21 @@ __device__ __forceinline__ double m_rcp(double x) { return fx_rcp(x); }
 __device__ __forceinline__ float m_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double m_rsqrt(double x) { return fx_rsqrt(x); }
 __device__ __forceinline__ float m_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+// sin and cos of 2 pi u, u in [0, 1]; in single precision the hardware
+// sine / cosine, whose argument is in revolutions
+__device__ __forceinline__ void m_sincos2pi(double u, double& s, double& c) { fx_sincos2pi(u, s, c); }
+__device__ __forceinline__ void m_sincos2pi(double u, float& s, float& c) {
+  s = __builtin_amdgcn_sinf((float)u);
+  c = __builtin_amdgcn_cosf((float)u);
+}
 __device__ __forceinline__ double m_abs(double x) { return fabs(x); }
+// single precision: the hardware exp2 / log2 (about 1 ulp for normal
+// arguments; the physics' arguments are normal), pow as exp2(y log2 x)
 __device__ __forceinline__ float m_exp(float x) { return __expf(x); }
-__device__ __forceinline__ float m_log(float x) { return __logf(x); }
-__device__ __forceinline__ float m_pow(float x, float y) { return __powf(x, y); }
+__device__ __forceinline__ float m_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+__device__ __forceinline__ float m_pow(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 __device__ __forceinline__ float m_sqrt(float x) { return __fsqrt_rn(x); }
 __device__ __forceinline__ float m_abs(float x) { return fabsf(x); }
 
