@@ -235,8 +235,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         n_cpu = args.cpu_histories
         if n_cpu <= 0:
-            # about 15 s of CPU work at ~1.2e6 collisions/s per core
-            n_cpu = max(16, int(15.0 * 1.2e6 * threads / max(coll / args.histories, 1.0)))
+            # about 15 s of CPU work at ~2.6e6 collisions/s per core
+            n_cpu = max(16, int(15.0 * 2.6e6 * threads / max(coll / args.histories, 1.0)))
         out["cpu_baseline"] = cpu_baseline(prob, n_cpu, threads)
     print(json.dumps(out), flush=True)
 
