@@ -47,8 +47,14 @@ enum {
 
 /* boundary conditions (SampleBase::sampleBoundary, sample.h:49-54) */
 enum { MT_PBC = 0, MT_INF = 1, MT_CUT = 2 };
-/* geometry kinds */
-enum { MT_GEO_SOLID = 0, MT_GEO_LAYERS = 1, MT_GEO_CLUSTERS = 2 };
+/* geometry kinds: SampleSolid (sample_solid.C:25-29), SampleLayers
+ * (sample_layers.C:26-49), sampleClusters (sample_clusters.C:43-133),
+ * SampleWire (a cylinder along z inscribed in the w[0] x w[1] box, material 0
+ * inside, vacuum outside; sample_wire.C:37-46) and SampleBurriedWire (the
+ * wire under a 250 A cover of material 1 at -250 <= z < 0, material 1
+ * around the wire, vacuum above z = w[2] and below z = -250;
+ * sample_burried_wire.C:34-55) */
+enum { MT_GEO_SOLID = 0, MT_GEO_LAYERS = 1, MT_GEO_CLUSTERS = 2, MT_GEO_WIRE = 3, MT_GEO_BURRIED_WIRE = 4 };
 /* interatomic potentials (TrimBase::Potential, trim.h:63-68) */
 enum { MT_UNIVERSAL = 0, MT_MOLIERE = 1, MT_CKR = 2 };
 /* arithmetic of the cascade kernel */

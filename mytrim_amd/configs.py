@@ -1,7 +1,7 @@
 """The benchmark and validation workloads, as :class:`Problem` objects.
 
-C1..C5 are the five cascade workloads this engine is measured on; the
-``vacancy_*`` problems restate the infinite-medium set-up of the reference's
+C1..C5 are the five cascade workloads this engine is measured on; the wire
+problems follow apps/mytrim_wire2.C; the ``vacancy_*`` problems restate the infinite-medium set-up of the reference's
 vacancy validation inputs (validation/vacancy_count/*_vac.json run through
 apps/runmytrim.C: SampleLayers(thickness, 100, 100) with the default all-PBC
 boundaries and primaries started at (0, 50, 50) along +x).
@@ -10,8 +10,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from .problem import (CUT, GEO_CLUSTERS, GEO_LAYERS, GEO_SOLID, PBC, Bins, Element, Geometry,
-                      Material, Primaries, Problem, log_uniform)
+from .problem import (CUT, GEO_BURRIED_WIRE, GEO_CLUSTERS, GEO_LAYERS, GEO_SOLID, GEO_WIRE, INF, PBC,
+                      Bins, Element, Geometry, Material, Primaries, Problem, log_uniform)
 
 
 def c1_he_fe(n_hist: int = 2000, seed: int = 2344) -> Problem:
@@ -157,3 +157,33 @@ def vacancy_case(ion: str, energy_eV: float, n_hist: int, seed: int = 2344) -> P
 
 
 ALL = {"c1": c1_he_fe, "c2": c2_cu_cu, "c3": c3_ag_layers, "c4": c4_mixed_uo2, "c5": c5_c_w}
+
+
+def wire_b_si(n_hist: int = 100000, seed: int = 2344, d: float = 500.0, length: float = 2000.0) -> Problem:
+    """320 keV B onto the top face of a Si nanowire (SampleWire: a cylinder of
+    diameter d along z, length `length`), as apps/mytrim_wire2.C:70, 92-93, 120-128
+    at theta = 0: CUT on every axis, the ion enters at the centre of the top."""
+    return Problem(
+        name="wire_b_si_320keV",
+        materials=[Material(2.329, [Element(14, 28.0, 1.0)])],
+        geometry=Geometry(GEO_WIRE, (d, d, length), (CUT, CUT, CUT)),
+        primaries=Primaries.single(5, 11.0, 320.0e3, (d / 2, d / 2, 0.0), (0.0, 0.0, 1.0)),
+        n_hist=n_hist, seed=seed,
+        range_bins=Bins(0.0, 10.0, 400), vac_bins=Bins(0.0, 10.0, 400),
+        lat_bw=10.0, lat_nb=100, lat_y0=d / 2, lat_z0=0.0)
+
+
+def burried_wire_p_si(n_hist: int = 100000, seed: int = 2344, d: float = 500.0,
+                      length: float = 2000.0) -> Problem:
+    """250 keV P through the 250 A SiO2 cover onto a buried Si nanowire
+    (SampleBurriedWire), as apps/mytrim_wire2.C:74, 89, 120-142, 182-189 with the
+    ion started above the wire axis; every axis INF."""
+    return Problem(
+        name="burried_wire_p_si_250keV",
+        materials=[Material(2.329, [Element(14, 28.0, 1.0)]),
+                   Material(2.634, [Element(14, 28.0, 1.0), Element(8, 16.0, 2.0)])],
+        geometry=Geometry(GEO_BURRIED_WIRE, (d, d, length), (INF, INF, INF)),
+        primaries=Primaries.single(15, 31.0, 250.0e3, (d / 2, d / 2, -250.0), (0.0, 0.0, 1.0)),
+        n_hist=n_hist, seed=seed,
+        range_bins=Bins(-250.0, 10.0, 400), vac_bins=Bins(-250.0, 10.0, 400),
+        lat_bw=10.0, lat_nb=100, lat_y0=d / 2, lat_z0=-250.0)

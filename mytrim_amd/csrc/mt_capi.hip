@@ -398,7 +398,9 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
     g.cy = ctx->cy.as<double>();
     g.cz = ctx->cz.as<double>();
     g.cr = ctx->cr.as<double>();
-  } else if (g.kind != MT_GEO_SOLID) {
+  } else if (g.kind == MT_GEO_BURRIED_WIRE) {
+    if (t->n_mat < 2) return fail(MT_E_ARG, "a buried wire needs two materials (wire, cover)");
+  } else if (g.kind != MT_GEO_SOLID && g.kind != MT_GEO_WIRE) {
     return fail(MT_E_ARG, "unknown geometry kind");
   }
   if (t->potential < 0 || t->potential > 2) return fail(MT_E_ARG, "unknown potential");

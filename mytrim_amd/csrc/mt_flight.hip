@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(64) flight_kernel(FlightParams P) {
     ++ic;
     int mtag;
     const double pos[3] = {ion.pos[0], ion.pos[1], ion.pos[2]};
-    const int k = G.kind == 0 ? lookup_material<0>(G, pos, mtag)
+    const int k = (G.kind == 0 || G.kind >= 3) ? lookup_material<0>(G, pos, mtag)
                   : G.kind == 1 ? lookup_material<1>(G, pos, mtag)
                                 : lookup_material<2>(G, pos, mtag);
     if (k < 0) break;  // vacuum: state stays MOVING (trim.C:80-82)

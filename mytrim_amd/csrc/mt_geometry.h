@@ -5,8 +5,13 @@
 //   sampleClusters::lookupCluster   sample_clusters.C:43-133 (cell hash with a
 //                                   ks-cell neighbourhood and minimum-image
 //                                   distances along periodic axes)
-// GEO is the sample kind (0 solid, 1 layers, 2 clusters).  Returns the material index, or -1 for vacuum (a CUT axis left by a
-// clusters sample); `tag` receives the cluster index when inside one.
+//   SampleWire::lookupMaterial      sample_wire.C:37-46 (inside the cylinder
+//                                   inscribed in the w[0] x w[1] box)
+//   SampleBurriedWire::lookupMaterial  sample_burried_wire.C:34-55
+// GEO is the kernel's sample template (0 solid and the two wire kinds, told
+// apart by Geometry::kind at run time; 1 layers; 2 clusters).  Returns the
+// material index, or -1 for vacuum (a CUT axis left by a clusters sample,
+// outside a wire); `tag` receives the cluster index when inside one.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -58,10 +63,24 @@ __device__ __forceinline__ int lookup_cluster(const Geometry& g, const double po
   return -1;
 }
 
+// inside the wire: ((x / w0) 2 - 1)^2 + ((y / w1) 2 - 1)^2 <= 1
+__device__ __forceinline__ bool in_wire(const Geometry& g, const double pos[3]) {
+  const double x = (pos[0] / g.w[0]) * 2.0 - 1.0;
+  const double y = (pos[1] / g.w[1]) * 2.0 - 1.0;
+  return !((x * x + y * y) > 1.0);
+}
+
+__device__ __forceinline__ int lookup_wire(const Geometry& g, const double pos[3]) {
+  if (g.kind == 3) return in_wire(g, pos) ? 0 : -1;
+  if (pos[2] < 0.0 && pos[2] >= -250.0) return 1;  // cover layer
+  if (pos[2] > g.w[2] || pos[2] < -250.0) return -1;
+  return in_wire(g, pos) ? 0 : 1;
+}
+
 template <int GEO>
 __device__ __forceinline__ int lookup_material(const Geometry& g, const double pos[3], int& tag) {
   tag = -1;
-  if (GEO == 0) return 0;
+  if (GEO == 0) return g.kind >= 3 ? lookup_wire(g, pos) : 0;
   if (GEO == 1) {
     // first layer with x < its running end (the ends are non-decreasing, so
     // this counts a prefix); uniform loop bound keeps layer_end in scalars

@@ -77,7 +77,7 @@ struct TableView {
 };
 
 struct Geometry {
-  int kind;                     // 0 solid, 1 layers, 2 clusters
+  int kind;                     // 0 solid, 1 layers, 2 clusters, 3 wire, 4 buried wire
   int bc[3];
   double w[3];
   int n_layers, n_mat;

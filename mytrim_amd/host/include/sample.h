@@ -1,6 +1,6 @@
-// SampleBase of the MyTRIM API (sample.h).  The device runs the three
-// sample kinds below (SampleSolid, SampleLayers, sampleClusters); geometry()
-// reports which one a sample is.
+// SampleBase of the MyTRIM API (sample.h).  The device runs the sample kinds
+// SampleSolid, SampleLayers, sampleClusters, SampleWire and
+// SampleBurriedWire; geometry() reports which one a sample is.
 #ifndef MYTRIM_AMD_SAMPLE_H
 #define MYTRIM_AMD_SAMPLE_H
 
@@ -25,7 +25,8 @@ class SampleBase {
   enum sampleBoundary { PBC, INF, CUT };
   sampleBoundary bc[3];
 
-  // device geometry kind: 0 solid, 1 layers, 2 clusters, -1 not supported
+  // device geometry kind: 0 solid, 1 layers, 2 clusters, 3 wire, 4 buried
+  // wire, -1 not supported
   virtual int geometry() const { return -1; }
 };
 

@@ -108,7 +108,10 @@ std::vector<double> target_signature(SampleBase* s, SimconfType* simconf, int po
 
 void upload_target(mt_ctx* ctx, SampleBase* s, SimconfType* simconf, int potential) {
   const int kind = s->geometry();
-  if (kind < 0) throw std::runtime_error("sample kind has no device geometry (use SampleSolid, SampleLayers or sampleClusters)");
+  if (kind < 0)
+    throw std::runtime_error(
+        "sample kind has no device geometry (use SampleSolid, SampleLayers, sampleClusters, SampleWire or "
+        "SampleBurriedWire)");
   std::vector<double> rho;
   std::vector<int> off{0}, Z;
   std::vector<double> mm, t, ed, eb;

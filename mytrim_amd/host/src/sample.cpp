@@ -1,6 +1,7 @@
-// SampleBase, SampleLayers and sampleClusters geometry on the host
-// (sample.C, sample_layers.C, sample_clusters.C).  The device has its own
-// copies of the lookups; these serve sample construction and host queries.
+// SampleBase, SampleLayers, sampleClusters, SampleWire and SampleBurriedWire
+// geometry on the host (sample.C, sample_layers.C, sample_clusters.C,
+// sample_wire.C, sample_burried_wire.C).  The device has its own copies of
+// the lookups; these serve sample construction and host queries.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -8,7 +9,9 @@
 #include "functions.h"
 #include "sample.h"
 #include "sample_clusters.h"
+#include "sample_burried_wire.h"
 #include "sample_layers.h"
+#include "sample_wire.h"
 
 namespace MyTRIM_NS {
 
@@ -24,6 +27,30 @@ void SampleBase::averages(const IonBase* pka) {
 }
 
 Real SampleBase::rangeMaterial(Point&, Point&) { return 100000.0; }
+
+SampleWire::SampleWire(Real x, Real y, Real z) : SampleBase(x, y, z) {  // sample_wire.C:30-34
+  bc[0] = CUT;
+  bc[1] = CUT;
+}
+
+MaterialBase* SampleWire::lookupMaterial(Point& pos) {  // sample_wire.C:37-46
+  const Real x = (pos(0) / w[0]) * 2.0 - 1.0;
+  const Real y = (pos(1) / w[1]) * 2.0 - 1.0;
+  return (x * x + y * y) > 1.0 ? nullptr : material[0];
+}
+
+SampleBurriedWire::SampleBurriedWire(Real x, Real y, Real z) : SampleWire(x, y, z) {  // sample_burried_wire.C:29-34
+  bc[0] = INF;
+  bc[1] = INF;
+  bc[2] = INF;
+}
+
+MaterialBase* SampleBurriedWire::lookupMaterial(Point& pos) {  // sample_burried_wire.C:37-55
+  if (pos(2) < 0.0 && pos(2) >= -250.0) return material[1];  // cover layer
+  if (pos(2) > w[2] || pos(2) < -250.0) return nullptr;      // above the sample or below the cover
+  MaterialBase* m = SampleWire::lookupMaterial(pos);
+  return m ? m : material[1];
+}
 
 int SampleLayers::lookupLayer(Point& pos) {  // sample_layers.C:26-43
   unsigned int i;

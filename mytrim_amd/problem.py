@@ -15,7 +15,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 PBC, INF, CUT = 0, 1, 2
-GEO_SOLID, GEO_LAYERS, GEO_CLUSTERS = 0, 1, 2
+GEO_SOLID, GEO_LAYERS, GEO_CLUSTERS, GEO_WIRE, GEO_BURRIED_WIRE = 0, 1, 2, 3, 4
 UNIVERSAL, MOLIERE, CKR = 0, 1, 2
 
 
@@ -118,7 +118,8 @@ class Problem:
     def spec_text(self) -> str:
         """Key/value text read by the reference fixture harness."""
         g = self.geometry
-        kind = {GEO_SOLID: "solid", GEO_LAYERS: "layers", GEO_CLUSTERS: "clusters"}[g.kind]
+        kind = {GEO_SOLID: "solid", GEO_LAYERS: "layers", GEO_CLUSTERS: "clusters", GEO_WIRE: "wire",
+                GEO_BURRIED_WIRE: "burriedwire"}[g.kind]
         out = ["geo " + kind,
                "w %r %r %r" % tuple(float(x) for x in g.w),
                "bc %d %d %d" % tuple(g.bc)]

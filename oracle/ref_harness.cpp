@@ -26,6 +26,8 @@
 #include "material.h"
 #include "sample_clusters.h"
 #include "sample_layers.h"
+#include "sample_burried_wire.h"
+#include "sample_wire.h"
 #include "sample_solid.h"
 #include "simconf.h"
 #include "trim.h"
@@ -130,6 +132,10 @@ SampleBase* build(const Spec& s, SimconfType* sc) {
     SampleLayers* sl = new SampleLayers(s.w[0], s.w[1], s.w[2]);
     for (double t : s.layers) sl->layerThickness.push_back(t);
     smp = sl;
+  } else if (s.geo == "wire") {
+    smp = new SampleWire(s.w[0], s.w[1], s.w[2]);
+  } else if (s.geo == "burriedwire") {
+    smp = new SampleBurriedWire(s.w[0], s.w[1], s.w[2]);
   } else {
     sampleClusters* c = new sampleClusters(s.w[0], s.w[1], s.w[2]);
     for (int i = 0; i < 3; ++i) c->bc[i] = SampleBase::sampleBoundary(s.bc[i]);

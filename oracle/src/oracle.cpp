@@ -225,6 +225,20 @@ int SampleLayers::lookupLayer(const double pos[3]) const {  // sample_layers.C:2
   return int(i);
 }
 
+Material* SampleWire::lookupMaterial(const double pos[3]) {  // sample_wire.C:37-46
+  const double x = (pos[0] / w[0]) * 2.0 - 1.0;
+  const double y = (pos[1] / w[1]) * 2.0 - 1.0;
+  if ((x * x + y * y) > 1.0) return nullptr;
+  return material[0];
+}
+
+Material* SampleBurriedWire::lookupMaterial(const double pos[3]) {  // sample_burried_wire.C:34-55
+  if (pos[2] < 0.0 && pos[2] >= -250.0) return material[1];  // cover
+  if (pos[2] > w[2] || pos[2] < -250.0) return nullptr;
+  Material* m = SampleWire::lookupMaterial(pos);
+  return m ? m : material[1];
+}
+
 Material* SampleClusters::lookupMaterial(const double pos[3]) {  // sample_clusters.C:43-55
   const int l = lookupCluster(pos, 0.0);
   if (l == -2) return nullptr;

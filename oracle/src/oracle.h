@@ -101,6 +101,19 @@ struct SampleLayers : Sample {  // sample_layers.C:26-49
   Material* lookupMaterial(const double pos[3]) override { return material[lookupLayer(pos)]; }
 };
 
+struct SampleWire : Sample {  // sample_wire.C:30-46
+  SampleWire(double x, double y, double z) : Sample(x, y, z) {
+    bc[0] = CUT;
+    bc[1] = CUT;
+  }
+  Material* lookupMaterial(const double pos[3]) override;
+};
+
+struct SampleBurriedWire : SampleWire {  // sample_burried_wire.C:29-55
+  SampleBurriedWire(double x, double y, double z) : SampleWire(x, y, z) { bc[0] = bc[1] = bc[2] = INF; }
+  Material* lookupMaterial(const double pos[3]) override;
+};
+
 struct SampleClusters : Sample {  // sample_clusters.C:31-252
   double sd = 0.0, kd[3] = {0, 0, 0};
   int kn[3] = {0, 0, 0};

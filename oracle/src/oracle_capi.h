@@ -18,7 +18,8 @@ typedef struct {
   const double* el_t;
   const double* el_Edisp;
   const double* el_Elbind;
-  /* geometry: 0 SampleSolid, 1 SampleLayers, 2 sampleClusters */
+  /* geometry: 0 SampleSolid, 1 SampleLayers, 2 sampleClusters, 3 SampleWire,
+     4 SampleBurriedWire */
   int geo;
   double w[3];
   int bc[3]; /* 0 PBC, 1 INF, 2 CUT */

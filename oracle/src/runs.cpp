@@ -49,6 +49,10 @@ std::unique_ptr<Sample> build_sample(const orc_run_spec& s, std::vector<std::uni
     SampleLayers* sl = new SampleLayers(s.w[0], s.w[1], s.w[2]);
     for (int i = 0; i < s.n_layers; ++i) sl->layerThickness.push_back(s.layer_thick[i]);
     smp.reset(sl);
+  } else if (s.geo == 3) {
+    smp.reset(new SampleWire(s.w[0], s.w[1], s.w[2]));
+  } else if (s.geo == 4) {
+    smp.reset(new SampleBurriedWire(s.w[0], s.w[1], s.w[2]));
   } else {
     SampleClusters* sc = new SampleClusters(s.w[0], s.w[1], s.w[2]);
     for (int i = 0; i < 3; ++i) sc->bc[i] = s.bc[i];

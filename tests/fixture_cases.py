@@ -29,7 +29,8 @@ def trace_cases():
     return {"c1": configs.c1_he_fe(4), "c2": configs.c2_cu_cu(4), "c3": configs.c3_ag_layers(4),
             "c4": c4_uniform(2000.0, 4), "c5": configs.c5_c_w(4),
             "moliere": configs.c2_cu_cu(4).with_(potential=1),
-            "ckr": configs.c5_c_w(4).with_(potential=2)}
+            "ckr": configs.c5_c_w(4).with_(potential=2),
+            "wire": configs.wire_b_si(4), "bwire": configs.burried_wire_p_si(4)}
 
 
 def tally_cases(quick=False):
@@ -47,4 +48,6 @@ def tally_cases(quick=False):
         "vac_cu_100keV": (configs.vacancy_case("cu", 1e5, int(800 * s)), 8),
         "vac_si_10keV": (configs.vacancy_case("si", 1e4, 2000), 2),
         "vac_xe_10keV": (configs.vacancy_case("xe", 1e4, 2000), 2),
+        "wire": (configs.wire_b_si(int(2000 * s)), 8),
+        "bwire": (configs.burried_wire_p_si(int(400 * s)), 8),
     }

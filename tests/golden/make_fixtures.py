@@ -17,7 +17,7 @@ computed by the reference library; the harness only drives its API.
                       runmytrim loop writes for it (oracle/_ref/runmytrim_ref:
                       the reference's driver loop and output modules; jsoncpp is
                       absent, so the deck is handed over as a flat spec)
-Usage:  python tests/golden/make_fixtures.py [--quick] [--tally=NAME] [--runmytrim]
+Usage:  python tests/golden/make_fixtures.py [--quick] [--tally=NAME] [--trace=NAME] [--runmytrim]
 """
 import json
 import os
@@ -82,8 +82,10 @@ def spec_file(prob):
     return f.name
 
 
-def traces(max_rows=1500):
+def traces(max_rows=1500, only=None):
     for name, prob in trace_cases().items():
+        if only and name not in only:
+            continue
         path = spec_file(prob)
         try:
             for h in (0, 1):
@@ -286,8 +288,12 @@ def main():
         runstopping_fixtures()
         return
     only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--tally=")]
-    if only:  # regenerate the named tally fixtures only
-        tallies(quick, only)
+    only_tr = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--trace=")]
+    if only or only_tr:  # regenerate the named tally / trace fixtures only
+        if only_tr:
+            traces(only=only_tr)
+        if only:
+            tallies(quick, only)
         return
     stopping_grid()
     average_table()

@@ -95,6 +95,8 @@ CASES = {
     "kp": lambda: configs.c2_cu_cu(200, seed=5).with_(follow_max_gen=1, kp=True),
     "moliere": lambda: configs.c1_he_fe(100, seed=5).with_(potential=1),
     "ckr": lambda: configs.c5_c_w(10, seed=5).with_(potential=2),
+    "wire": lambda: configs.wire_b_si(400, seed=5),
+    "bwire": lambda: configs.burried_wire_p_si(24, seed=5),
 }
 
 
@@ -149,6 +151,8 @@ def test_history_offsets_select_the_same_streams(eng):
     ("vac_cu_10keV", lambda n: configs.vacancy_case("cu", 1e4, n, seed=31), 20000),
     ("c5", lambda n: configs.c5_c_w(n, seed=31), 2000),
     ("c2", lambda n: configs.c2_cu_cu(n, seed=31), 3000),
+    ("wire", lambda n: configs.wire_b_si(n, seed=31), 2000),
+    ("bwire", lambda n: configs.burried_wire_p_si(n, seed=31), 400),
 ])
 def test_cascade_statistics_vs_reference(eng, name, prob, n):
     ref = load_tally(name)
@@ -158,14 +162,24 @@ def test_cascade_statistics_vs_reference(eng, name, prob, n):
     print("%s vac/ion gpu %.3f ref %.3f (sigma %.3f), %.1f ms" %
           (name, g["vac"] / n, rv.mean(), sig, g["kernel_ms"]))
     assert abs(g["vac"] / n - rv.mean()) < 2.5 * sig
-    m_ref = ref["range_sum"] / ref["range_n"]
-    v_ref = ref["range_sum2"] / ref["range_n"] - m_ref ** 2
-    m = g["range_sum"] / g["range_n"]
-    v = g["range_sum2"] / g["range_n"] - m ** 2
-    assert abs(m - m_ref) < 2.5 * math.sqrt(v_ref / ref["range_n"] + v / g["range_n"])
-    # collisions per history (electronic loss ledger share) within 2.5 sigma of the mean
+    if ref["range_n"] == 0:  # no primary came to rest inside (a thin wire)
+        assert g["range_n"] <= max(3, n // 1000)
+    else:
+        m_ref = ref["range_sum"] / ref["range_n"]
+        v_ref = ref["range_sum2"] / ref["range_n"] - m_ref ** 2
+        m = g["range_sum"] / g["range_n"]
+        v = g["range_sum2"] / g["range_n"] - m ** 2
+        assert abs(m - m_ref) < 2.5 * math.sqrt(v_ref / ref["range_n"] + v / g["range_n"])
+    # collisions per history scale with the cascade size: within 5 % or three
+    # times the relative sigma of vacancies per history, whichever is larger
     cpi = ref["n_coll"] / ref["n_hist"]
-    assert g["n_coll"] / n == pytest.approx(cpi, rel=0.05)
+    assert g["n_coll"] / n == pytest.approx(cpi, rel=max(0.05, 3 * sig / rv.mean()))
+    if name == "wire":
+        # flights leaving through a CUT face or out of the wire's side: mostly
+        # the primaries themselves, so nearly Poisson counts (4 sigma)
+        for k in ("lost", "vacuum"):
+            a, b = g[k] / n, ref[k] / ref["n_hist"]
+            assert abs(a - b) <= 4 * math.sqrt(g[k] / n ** 2 + ref[k] / ref["n_hist"] ** 2), k
 
 
 def test_vacancy_table_on_gpu(eng):
@@ -212,7 +226,7 @@ def replay_history(eng, prob, h, max_rows):
     return np.array(rows[:max_rows])
 
 
-@pytest.mark.parametrize("case", ["c1", "c2", "c3", "c4", "c5", "moliere", "ckr"])
+@pytest.mark.parametrize("case", ["c1", "c2", "c3", "c4", "c5", "moliere", "ckr", "wire", "bwire"])
 def test_flight_path_reproduces_reference_trace(eng, case):
     ref = np.load(os.path.join(GOLD, "trace_%s_h0.npz" % case))["rows"][:600]
     prob = trace_cases()[case]

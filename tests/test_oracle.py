@@ -98,7 +98,7 @@ def test_average_constants_bitwise():
             k += 1
 
 
-@pytest.mark.parametrize("case", ["c1", "c2", "c3", "c4", "c5", "moliere", "ckr"])
+@pytest.mark.parametrize("case", ["c1", "c2", "c3", "c4", "c5", "moliere", "ckr", "wire", "bwire"])
 @pytest.mark.parametrize("h", [0, 1])
 def test_trace_bitwise(case, h):
     from fixture_cases import trace_cases
@@ -115,6 +115,8 @@ def test_trace_bitwise(case, h):
     ("vac_si_10keV", lambda: configs.vacancy_case("si", 1e4, 2000)),
     ("vac_xe_10keV", lambda: configs.vacancy_case("xe", 1e4, 2000)),
     ("c2_primaries_kp", lambda: configs.c2_cu_cu(4000).with_(follow_max_gen=1, kp=True)),
+    ("wire", lambda: configs.wire_b_si(2000)),
+    ("bwire", lambda: configs.burried_wire_p_si(400)),
 ])
 def test_tally_bitwise(name, prob):
     ref = load_tally(name)
