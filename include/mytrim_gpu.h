@@ -173,6 +173,37 @@ typedef struct {
   int mat_tag;                           /* material tag at this collision (cluster index) */
 } mt_event;
 
+/* Event streams of the batched path: one record per line the reference's
+ * stream-writing TrimBase subclasses print, and TrimVacMap's map.
+ *   kind 1      TrimDefectLog "V" + the recoil at vacancyCreation (trim.C:466-469)
+ *   kind 2/3/4  TrimDefectLog "I" / "S" / "R" + the ion at rest (trim.C:471-477)
+ *   kind 5      TrimHistory: the ion when a recoil is offered to followRecoil (trim.h:139-155)
+ *   kind 6      TrimPhononOut::followRecoil: value Elbind + the recoil (trim.C:520-526)
+ *   kind 7      TrimPhononOut::dissipateRecoilEnergy: value recoil E + Elbind + the recoil (trim.C:512-518)
+ *   kind 8      TrimPhononOut::checkPKAState: value the ion's E + the ion at rest (trim.C:503-510)
+ * The ion fields are those IonBase's operator<< prints (ion.C:74-79); id is
+ * the flight key (unique per flight; the reference numbers ions in creation
+ * order from a global counter).  Records of one history are in trim.C order;
+ * histories interleave.  Recoils in kinds 6 and 7 carry id 0 and tag -1, as
+ * they have not been queued when the reference prints them. */
+enum { MT_LOG_DEFECTS = 1, MT_LOG_HISTORY = 2, MT_LOG_PHONONS = 4, MT_LOG_VACMAP = 8 };
+typedef struct {
+  double pos[3];
+  double E, value, m;
+  unsigned long long id;
+  uint32_t hist;
+  int Z, gen, tag, kind, pad;
+} mt_record;
+typedef struct {
+  int mask;                 /* MT_LOG_* */
+  mt_record* records;       /* caller-allocated, max_records entries (may be NULL) */
+  long long max_records;
+  long long n_records;      /* out: records produced; those past max_records are dropped */
+  int vacmap_Z[3];          /* TrimVacMap z1 z2 z3 (trim.h:176-201) */
+  int* vacmap;              /* caller-allocated int[20][20][3] (x cell, y cell, constituent):
+                               the run's counts are added (MT_LOG_VACMAP) */
+} mt_log;
+
 /* Scattering surrogate (no counterpart in the reference): an MLP
  * 3 -> 64 -> 64 -> 1 (tanh) from (log10 eps, log10 b, b) to ln sin^2(theta/2),
  * trained on the device on the engine's own universal-potential scattering
@@ -209,6 +240,12 @@ int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* tuning);
 
 int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts,
                     mt_tallies* out);
+
+/* mt_run_cascades with event streams (TrimDefectLog, TrimHistory,
+ * TrimPhononOut, TrimVacMap; trim.C:466-527, trim.h:139-227) written to
+ * `log`.  Runs the per-history kernel. */
+int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, mt_log* log,
+                        mt_tallies* out);
 
 /* Autotuner (no counterpart in the reference: the reference's tuning knob is
  * its thread count, apps/runmytrim.C:59-68).  Runs the first `sample`

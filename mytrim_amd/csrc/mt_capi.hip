@@ -186,11 +186,13 @@ struct mt_ctx {
   bool mlp_ready = false;
   int mlp_bf16 = 0;
   DBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
+  DBuf log_rec, log_map;  // event records and TrimVacMap map
 
   ~mt_ctx() {
     graph.release();
     DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
-                   &wave_energy, &hist_all, &h_out, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e};
+                   &wave_energy, &hist_all, &h_out, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e,
+                   &log_rec, &log_map};
     for (DBuf* b : all) b->release();
     stage_res.release();
     stage_hist.release();
@@ -417,7 +419,25 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
   return MT_OK;
 }
 
+static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_log* lg, mt_tallies* out);
+
 int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_tallies* out) {
+  return run_cascades(ctx, pk, o, nullptr, out);
+}
+
+int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_log* lg, mt_tallies* out) {
+  if (!lg) return fail(MT_E_ARG, "NULL log");
+  if (lg->mask & ~(MT_LOG_DEFECTS | MT_LOG_HISTORY | MT_LOG_PHONONS | MT_LOG_VACMAP))
+    return fail(MT_E_ARG, "unknown log bits");
+  if (lg->max_records < 0 || (lg->max_records > 0 && !lg->records))
+    return fail(MT_E_ARG, "records buffer missing");
+  if ((lg->mask & MT_LOG_VACMAP) && !lg->vacmap) return fail(MT_E_ARG, "vacmap buffer missing");
+  return run_cascades(ctx, pk, o, lg, out);
+}
+
+static_assert(sizeof(LogRecord) == sizeof(mt_record) && sizeof(mt_record) == 80, "record layout");
+
+static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_log* lg, mt_tallies* out) {
   if (!ctx || !pk || !o || !out) return fail(MT_E_ARG, "NULL argument");
   if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
   if (pk->n_hist < 0 || pk->n_hist > (1ll << 31)) return fail(MT_E_LIMIT, "n_hist out of range");
@@ -472,7 +492,9 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   if (lds_hist) lds += lds_hist_bytes;
   if (lds > lds_max) return fail(MT_E_LIMIT, "tables and stacks exceed the LDS of a workgroup");
   const int pot = ctx->tgt.potential;
-  const bool perhist = o->per_history != 0;
+  // event streams run in the per-history kernel
+  const bool log_on = lg && lg->mask != 0;
+  const bool perhist = o->per_history != 0 || log_on;
   int bpc = 0;
   HIPCHECK(cascade_occupancy(o->arith, pot, ctx->tgt.geo.kind, perhist, block, lds, &bpc));
   bpc = std::min(bpc, ctx->tune.blocks_per_cu > 0 ? ctx->tune.blocks_per_cu : kDefaultWavesPerCu * 64 / block);
@@ -616,6 +638,20 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     P.h_state = ctx->per[6].as<int>();
     P.h_Eel = ctx->per[7].as<double>();
   }
+  if (log_on) {
+    P.log.mask = lg->mask;
+    if (lg->max_records > 0) {
+      HIPCHECK(ctx->log_rec.ensure(size_t(lg->max_records) * sizeof(LogRecord)));
+      P.log.rec = ctx->log_rec.as<LogRecord>();
+      P.log.cap = (unsigned long long)lg->max_records;
+    }
+    for (int i = 0; i < 3; ++i) P.log.vz[i] = lg->vacmap_Z[i];
+    if (lg->mask & MT_LOG_VACMAP) {
+      HIPCHECK(ctx->log_map.ensure(sizeof(int) * kVacMapN * kVacMapN * 3));
+      P.log.vacmap = ctx->log_map.as<int>();
+      span(ctx->log_map.p, sizeof(int) * kVacMapN * kVacMapN * 3);
+    }
+  }
   if (o->arith == MT_FP32_SURROGATE) {
     P.mlp.p = ctx->mlp_p.as<float>();
     P.mlp.w2bf = ctx->mlp_w2bf.as<uint32_t>();
@@ -626,7 +662,7 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
   // reset -> [event] -> persistent kernel -> [event] -> energy flush -> copies
   // to pinned staging: replayed from a cached graph, or launched one by one
   double* red = reinterpret_cast<double*>(res + kResReduced);
-  if (ctx->tune.graph >= 0) {
+  if (ctx->tune.graph >= 0 && !log_on) {
     const void* kfn = cascade_function(o->arith, pot, ctx->tgt.geo.kind, perhist);
     const RunGraphKey key{kfn, grid, block, lds, ctx->res.p, ctx->hist_all.p, hist_bytes, ctx->stage_res.p,
                           ctx->stage_hist.p};
@@ -721,7 +757,19 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
     if (hdst[i])
       std::memcpy(hdst[i], static_cast<const unsigned long long*>(ctx->stage_hist.p) + hoff[i],
                   nb[i] * sizeof(long long));
-  if (perhist) {
+  if (log_on) {
+    const long long n = (long long)ctrl[kLogCount];
+    lg->n_records = n;
+    const long long keep = std::min(n, lg->max_records);
+    if (keep > 0)
+      HIPCHECK(hipMemcpy(lg->records, ctx->log_rec.p, size_t(keep) * sizeof(LogRecord), hipMemcpyDeviceToHost));
+    if (lg->mask & MT_LOG_VACMAP) {
+      std::vector<int> m(kVacMapN * kVacMapN * 3);
+      HIPCHECK(hipMemcpy(m.data(), ctx->log_map.p, m.size() * sizeof(int), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < m.size(); ++i) lg->vacmap[i] += m[i];
+    }
+  }
+  if (perhist && o->per_history) {
     void* pdst[8] = {out->h_vac, out->h_coll, out->h_flights, out->h_x,
                      out->h_y,   out->h_z,    out->h_state,   out->h_Eel};
     for (int i = 0; i < 8; ++i)

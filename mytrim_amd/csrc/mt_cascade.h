@@ -33,6 +33,7 @@ constexpr int kQueueReady = 10;  // index of the ready words among the arrays
 enum Ctrl {
   kState = 0,           // epoch << 32 | live waves << 16 | idle waves
   kPkaNext = 16,        // next primary to start
+  kLogCount = 32,       // event records produced (diagnostic kernels)
   kError = 64,          // error bits
   kQueueOps = 65,       // wave-level queue transfers
   kQueuePeak = 80,      // largest tail - head seen at a push
@@ -59,6 +60,28 @@ enum Counter {
 };
 
 enum Energy { eEel = 0, eEnuc, eEunf, eElost, eEdrop, eEvac, eRangeSum, eRangeSum2, kNumEnergy = 8 };
+
+// Event records of the diagnostic (per-history) kernels, one per line the
+// reference's stream-writing TrimBase subclasses print (the layout of
+// mt_record): kinds 1 V (recoil at vacancyCreation), 2 I / 3 S / 4 R (ion at
+// rest), 5 TrimHistory (ion when a recoil is followed), 6/7/8 TrimPhononOut
+// (spawned recoil: Elbind; dissipated recoil: E + Elbind; ion at rest: E).
+struct LogRecord {
+  double pos[3];
+  double E, value, m;
+  unsigned long long id;
+  uint32_t hist;
+  int Z, gen, tag, kind, pad;
+};
+constexpr int kLogDefects = 1, kLogHistory = 2, kLogPhonons = 4, kLogVacmap = 8;
+constexpr int kVacMapN = 20;  // TrimVacMap cells per axis (trim.h:178)
+struct LogDev {
+  int mask;                // kLog* bits; 0: no records
+  LogRecord* rec;          // cap records; the count runs on past cap
+  unsigned long long cap;
+  int vz[3];               // TrimVacMap z1 z2 z3
+  int* vacmap;             // [20][20][3]
+};
 
 struct Params {
   const unsigned char* tables;  // table blob (TableHeader + arrays)
@@ -109,6 +132,8 @@ struct Params {
   double* h_Eel;
   // scattering surrogate (FP32 + surrogate mode)
   MlpDev mlp;
+  // event records and vacancy map (per-history kernels only)
+  LogDev log;
 };
 
 // launch bound of the persistent kernel (threads per workgroup)

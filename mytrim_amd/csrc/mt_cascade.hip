@@ -130,6 +130,23 @@ __device__ __forceinline__ uint64_t sum_add(uint64_t h, uint64_t w) {
 }
 __device__ __forceinline__ uint32_t sum_end(uint64_t h) { return (uint32_t)(h >> 32); }
 
+// Appends one record per lane where `pred` holds (called by the whole wave:
+// one counter atomic per wave; records past the capacity are counted, not
+// stored)
+__device__ __forceinline__ void log_put(const LogDev& L, unsigned long long* count, bool pred, const LogRecord& r) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  uint64_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+  base = bcast64(base, leader);
+  if (pred) {
+    const uint64_t i = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (i < L.cap) L.rec[i] = r;
+  }
+}
+
 // Free path, impact parameter, target atom, electronic loss and scattering
 // of one collision (trim.C:88-272) in precision R.  FP64 follows the
 // reference's expressions; FP32 evaluates the same quantities in single
@@ -486,6 +503,11 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     int state = 0;
     double rx = 0.0, ry = 0.0, rz = 0.0, rdx = 0.0, rdy = 0.0, rdz = 0.0, rE = 0.0;
     int rsp = 0, rtag = -1;
+    // event-record inputs (per-history kernels): recoil above threshold and
+    // not lost, followed, its element (Z, m, Elbind)
+    bool lg_above = false, lg_follow = false;
+    int lg_rZ = 0;
+    double lg_rm = 0.0, lg_lb = 0.0;
     if (active) {
       ++F.ic;
       int mtag;
@@ -585,6 +607,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         if (G.bc[0] == 2 && (F.px > G.w[0] || F.px < 0.0)) lost = true;
         if (G.bc[1] == 2 && (F.py > G.w[1] || F.py < 0.0)) lost = true;
         if (G.bc[2] == 2 && (F.pz > G.w[2] || F.pz < 0.0)) lost = true;
+        if (PERHIST) {
+          lg_rZ = ec.Z;
+          lg_rm = H->species[ec.recoil_species].m;
+          lg_lb = ec.Elbind;
+          lg_above = !lost && rE > ec.Edisp - ec.Elbind;
+          lg_follow = lg_above && (P.follow_max_gen < 0 || F.gen + 1 < P.follow_max_gen);
+        }
         // fate of recoil and ion (trim.C:357-411)
         if (lost) {
           state = 4;
@@ -646,6 +675,84 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       }
     }
     l_vac32 += (active && state == -1) ? 1u : 0u;
+
+    // ------------------------------------------------------------ event records
+    // the lines TrimHistory, TrimPhononOut and TrimDefectLog write for this
+    // collision, in trim.C order: followRecoil (history, spawn), vacancy,
+    // dissipation, ion at rest; and TrimVacMap's cell (trim.C:479-501)
+    if constexpr (PERHIST) {
+      if (P.log.mask) {
+        const LogDev& L = P.log;
+        unsigned long long* lc = &ctrl[kLogCount];
+        const bool coll = active && !vacuum;
+        LogRecord ion, rec;
+        ion.pos[0] = F.px;
+        ion.pos[1] = F.py;
+        ion.pos[2] = F.pz;
+        ion.E = F.E;
+        ion.value = 0.0;
+        ion.m = active ? H->species[F.sp].m : 0.0;
+        ion.id = F.key;
+        ion.hist = F.hist;
+        ion.Z = active ? H->species[F.sp].Z : 0;
+        ion.gen = F.gen;
+        ion.tag = F.tag;
+        ion.pad = 0;
+        rec.pos[0] = rx;
+        rec.pos[1] = ry;
+        rec.pos[2] = rz;
+        rec.E = rE;
+        rec.m = lg_rm;
+        rec.id = 0ull;
+        rec.hist = F.hist;
+        rec.Z = lg_rZ;
+        rec.gen = F.gen + 1;
+        rec.tag = -1;
+        rec.pad = 0;
+        if (L.mask & kLogHistory) {
+          ion.kind = 5;
+          log_put(L, lc, coll && lg_above, ion);
+        }
+        if (L.mask & kLogPhonons) {
+          rec.kind = 6;
+          rec.value = lg_lb;
+          log_put(L, lc, coll && lg_follow, rec);
+        }
+        if (L.mask & kLogDefects) {
+          rec.kind = 1;
+          rec.value = 0.0;
+          rec.id = lg_follow ? child_key(F.key, (uint32_t)F.ic) : 0ull;
+          rec.tag = lg_follow ? rtag : -1;
+          log_put(L, lc, coll && state == -1, rec);
+          rec.id = 0ull;
+          rec.tag = -1;
+        }
+        if ((L.mask & kLogVacmap) && coll && state == -1) {
+          int x = (int)((rx * kVacMapN) / G.w[0]);
+          int y = (int)((ry * kVacMapN) / G.w[1]);
+          x -= (x / kVacMapN) * kVacMapN;
+          y -= (y / kVacMapN) * kVacMapN;
+          const int c = lg_rZ == L.vz[0] ? 0 : lg_rZ == L.vz[1] ? 1 : lg_rZ == L.vz[2] ? 2 : -1;
+          // the reference indexes out of its map for x or y < 0; those are skipped
+          if (c >= 0 && x >= 0 && y >= 0) atomicAdd(&L.vacmap[(x * kVacMapN + y) * 3 + c], 1);
+        }
+        if (L.mask & kLogPhonons) {
+          rec.kind = 7;
+          rec.value = rE + lg_lb;
+          log_put(L, lc, coll && !lg_above && state != 4, rec);
+        }
+        const bool rest = coll && state >= 1 && state <= 3;
+        if (L.mask & kLogDefects) {
+          ion.kind = state == 3 ? 2 : state == 2 ? 3 : 4;
+          log_put(L, lc, rest, ion);
+        }
+        if (L.mask & kLogPhonons) {
+          ion.kind = 8;
+          ion.value = F.E;
+          log_put(L, lc, rest, ion);
+        }
+      }
+    }
 
     // ------------------------------------------------------------ push recoils
     const uint64_t push_m = __ballot(push);

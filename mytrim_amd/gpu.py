@@ -23,11 +23,18 @@ ip = C.POINTER(C.c_int)
 lp = C.POINTER(C.c_longlong)
 
 MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
+MT_LOG_DEFECTS, MT_LOG_HISTORY, MT_LOG_PHONONS, MT_LOG_VACMAP = 1, 2, 4, 8
+# one event record (mt_record); kinds: 1 V, 2 I, 3 S, 4 R (TrimDefectLog),
+# 5 TrimHistory, 6 / 7 / 8 TrimPhononOut (spawn, dissipation, ion at rest)
+RECORD = np.dtype([("pos", "<f8", 3), ("E", "<f8"), ("value", "<f8"), ("m", "<f8"), ("id", "<u8"),
+                   ("hist", "<u4"), ("Z", "<i4"), ("gen", "<i4"), ("tag", "<i4"), ("kind", "<i4"),
+                   ("pad", "<i4")])
 
 # every symbol include/mytrim_gpu.h declares
 EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt_ctx_set_tables",
            "mt_default_tables",
-           "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_autotune", "mt_trim_flight",
+           "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_run_cascades_log", "mt_autotune",
+           "mt_trim_flight",
            "mt_stopping_batch", "mt_scatter_batch", "mt_surrogate_norm", "mt_surrogate_train", "mt_surrogate_install",
            "mt_surrogate_eval", "mt_surrogate_grad"]
 
@@ -58,6 +65,11 @@ class RunOpts(C.Structure):
 COUNTERS = ["n_coll", "n_flight", "vac", "repl", "subst", "inter", "lost", "vacuum",
             "rutherford", "newton", "disp"]
 ENERGIES = ["E0", "Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac"]
+
+
+class LogSpec(C.Structure):
+    _fields_ = [("mask", C.c_int), ("records", C.c_void_p), ("max_records", C.c_longlong),
+                ("n_records", C.c_longlong), ("vacmap_Z", C.c_int * 3), ("vacmap", ip)]
 
 
 class Tallies(C.Structure):
@@ -119,6 +131,8 @@ def lib():
         L.mt_ctx_set_tables.argtypes = [C.c_void_p, dp, C.c_int, C.c_int]
         L.mt_ctx_set_target.argtypes = [C.c_void_p, C.POINTER(Target)]
         L.mt_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.mt_run_cascades_log.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
+                                          C.POINTER(LogSpec), C.POINTER(Tallies)]
         L.mt_run_cascades.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
                                       C.POINTER(Tallies)]
         L.mt_autotune.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts), C.c_longlong,
@@ -229,8 +243,10 @@ class Engine:
         del keep
         return best, rate.value
 
-    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64) -> dict:
-        """Full cascades for every history of `prob` (sets the target if needed)."""
+    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64, log=None) -> dict:
+        """Full cascades for every history of `prob` (sets the target if needed).
+        `log` (optional): dict(mask=MT_LOG_*, max_records=n, vacmap_Z=(z1, z2, z3));
+        the result then holds "records" (RECORD array), "n_records" and "vacmap"."""
         pk, o, keep = self._launch_args(prob, per_history, arith)
         out = Tallies()
         hists = {"range_hist": np.zeros(prob.range_bins.nb + 2, np.int64),
@@ -248,9 +264,25 @@ class Engine:
                    "h_z": np.zeros(n), "h_state": np.zeros(n, np.int32), "h_Eel": np.zeros(n)}
             for k, a in per.items():
                 setattr(out, k, _p(a, {np.int32: ip, np.int64: lp, np.float64: dp}[a.dtype.type]))
-        _check(lib().mt_run_cascades(self.h, C.byref(pk), C.byref(o), C.byref(out)),
-               "mt_run_cascades")
+        if log is None:
+            _check(lib().mt_run_cascades(self.h, C.byref(pk), C.byref(o), C.byref(out)),
+                   "mt_run_cascades")
+        else:
+            lg = LogSpec()
+            lg.mask = int(log.get("mask", 0))
+            recs = np.zeros(int(log.get("max_records", 0)), RECORD)
+            lg.records = recs.ctypes.data_as(C.c_void_p) if len(recs) else None
+            lg.max_records = len(recs)
+            vz = log.get("vacmap_Z", (0, 0, 0))
+            for i in range(3):
+                lg.vacmap_Z[i] = int(vz[i])
+            vmap = np.zeros((20, 20, 3), np.int32)
+            lg.vacmap = _p(vmap, ip)
+            _check(lib().mt_run_cascades_log(self.h, C.byref(pk), C.byref(o), C.byref(lg), C.byref(out)),
+                   "mt_run_cascades_log")
         res = {k: getattr(out, k) for k in COUNTERS + ENERGIES}
+        if log is not None:
+            res.update(records=recs[:min(lg.n_records, len(recs))], n_records=lg.n_records, vacmap=vmap)
         res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
                    kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
                    queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate)

@@ -11,6 +11,7 @@
 #define MYTRIM_AMD_TRIM_GPU_H
 
 #include <cstdint>
+#include <iosfwd>
 #include <vector>
 
 #include "trim.h"
@@ -34,6 +35,23 @@ class TrimGPU : public TrimBase {
     // FP64 everywhere; FP32 physics with an FP64 ledger and positions; FP32
     // with the learned scattering surrogate (installed on the device context)
     enum Arith { FP64 = 0, FP32 = 1, FP32_SURROGATE = 2 } arith = FP64;
+    // event streams (the records TrimDefectLog, TrimHistory and
+    // TrimPhononOut write, trim.C:466-527): bits DEFECTS 1, HISTORY 2,
+    // PHONONS 4, VACMAP 8; at most max_records records are kept
+    int log_mask = 0;
+    long long max_records = 1 << 20;
+    int vacmap_Z[3] = {-1, -1, -1};  // TrimVacMap z1 z2 z3
+  };
+
+  // one event record (mt_record of include/mytrim_gpu.h): kind 1 V, 2 I,
+  // 3 S, 4 R (TrimDefectLog), 5 TrimHistory, 6/7/8 TrimPhononOut (recoil
+  // spawned, recoil dissipated, ion at rest) with `value` the energy it prints
+  struct Record {
+    Real pos[3];
+    Real E, value, m;
+    unsigned long long id;
+    unsigned int hist;
+    int Z, gen, tag, kind, pad;
   };
 
   struct Tallies {
@@ -47,7 +65,14 @@ class TrimGPU : public TrimBase {
     std::vector<long long> h_coll;
     std::vector<Real> h_x, h_y, h_z, h_Eel;
     double kernel_ms = 0;
+    std::vector<Record> records;  // log_mask != 0
+    long long n_records = 0;      // produced (records past max_records are dropped)
+    std::vector<int> vacmap;      // [20][20][3] (LOG VACMAP)
   };
+
+  // the lines TrimDefectLog (kinds 1-4) or TrimPhononOut (kinds 6-8) would
+  // print for `records` (IonBase's operator<<, ion.C:74-79), in record order
+  static void writeRecords(std::ostream& os, const std::vector<Record>& records, bool phonons);
 
   // n copies of `pka` (histories hist0 .. hist0+n-1 of stream `seed`)
   Tallies runCascades(const IonBase& pka, long long n, uint32_t seed, const Options& opt,

@@ -1,6 +1,7 @@
 // TrimGPU::runCascades: the batched device path (mt_run_cascades).
 #include "trim_gpu.h"
 
+#include <ostream>
 #include <stdexcept>
 
 #include "device.h"
@@ -85,7 +86,22 @@ TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<i
     out.h_z = t.h_z.data();
     out.h_Eel = t.h_Eel.data();
   }
-  device::check(mt_run_cascades(ctx, &L.pk, &L.ro, &out), "mt_run_cascades");
+  if (o.log_mask) {
+    static_assert(sizeof(TrimGPU::Record) == sizeof(mt_record), "record layout");
+    mt_log lg = {};
+    lg.mask = o.log_mask;
+    t.records.resize(size_t(std::max<long long>(o.max_records, 0)));
+    lg.records = reinterpret_cast<mt_record*>(t.records.data());
+    lg.max_records = (long long)t.records.size();
+    for (int i = 0; i < 3; ++i) lg.vacmap_Z[i] = o.vacmap_Z[i];
+    t.vacmap.assign(20 * 20 * 3, 0);
+    lg.vacmap = t.vacmap.data();
+    device::check(mt_run_cascades_log(ctx, &L.pk, &L.ro, &lg, &out), "mt_run_cascades_log");
+    t.n_records = lg.n_records;
+    t.records.resize(size_t(std::min<long long>(lg.n_records, lg.max_records)));
+  } else {
+    device::check(mt_run_cascades(ctx, &L.pk, &L.ro, &out), "mt_run_cascades");
+  }
   t.collisions = out.n_coll;
   t.flights = out.n_flight;
   t.vacancies = out.vac;
@@ -142,6 +158,19 @@ TrimGPU::Tallies TrimGPU::runCascades(const std::vector<IonBase*>& pkas, uint32_
   _simconf->EelTotal += t.Eel;
   _simconf->EnucTotal += t.Enuc;
   return t;
+}
+
+void TrimGPU::writeRecords(std::ostream& os, const std::vector<Record>& records, bool phonons) {
+  static const char kDefect[] = " VISR";
+  for (const Record& r : records) {
+    if (phonons ? (r.kind < 6 || r.kind > 8) : (r.kind < 1 || r.kind > 4)) continue;
+    if (phonons)
+      os << r.value << ' ';
+    else
+      os << kDefect[r.kind] << ' ';
+    os << r.pos[0] << ' ' << r.pos[1] << ' ' << r.pos[2] << ' ' << r.Z << ' ' << r.m << ' ' << r.E << ' '
+       << r.id << ' ' << r.gen << ' ' << r.tag << ' ' << '\n';
+  }
 }
 
 double TrimGPU::autotune(const IonBase& p, long long sample, const Options& o, int reps) {

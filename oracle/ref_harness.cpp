@@ -10,6 +10,10 @@
 //   ref_harness avg  Z1 m1 tmin rho nel Z m t ...                   -> average()
 //   ref_harness trace SPECFILE h max_rows                           -> per collision
 //   ref_harness tally SPECFILE                                      -> tallies
+//   ref_harness log SPECFILE defects|phonons|history|vacmap [z1 z2 z3]
+//       -> the stream of TrimDefectLog / TrimPhononOut (17 digits), the
+//          positions TrimHistory stores, or TrimVacMap's map; "H h" before
+//          each history
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +24,8 @@
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <iomanip>
 
 #include "element.h"
 #include "ion.h"
@@ -380,6 +386,64 @@ int cmd_trace(const Spec& s, long long h, int max_rows) {
 
 }  // namespace
 
+// the stream-writing and recording TrimBase subclasses of trim.h:139-227
+int cmd_log(const Spec& s, const std::string& mode, int z1, int z2, int z3) {
+  SimconfType sc;
+  sc.tmin = s.tmin;
+  sc.tau = s.tau;
+  sc.cw = s.cw;
+  sc.setLengthScale(s.scale);
+  SampleBase* smp = build(s, &sc);
+  std::ostringstream os;
+  os << std::setprecision(17);
+  TrimBase* trim;
+  TrimHistory* th = nullptr;
+  TrimVacMap* tv = nullptr;
+  if (mode == "defects") {
+    trim = new TrimDefectLog(&sc, smp, os);
+  } else if (mode == "phonons") {
+    trim = new TrimPhononOut(&sc, smp, os);
+  } else if (mode == "history") {
+    trim = th = new TrimHistory(&sc, smp);
+  } else if (mode == "vacmap") {
+    trim = tv = new TrimVacMap(&sc, smp, z1, z2, z3);
+  } else {
+    fprintf(stderr, "unknown log mode %s\n", mode.c_str());
+    return 2;
+  }
+  trim->_potential = TrimBase::Potential(s.potential);
+  size_t nh = 0;
+  for (long long h = s.hist0; h < s.hist0 + s.nhist; ++h) {
+    sc.seed(history_seed(s.seed, h));
+    std::queue<IonBase*> q;
+    IonBase* p = primary(s);
+    p->_id = 0;
+    q.push(p);
+    while (!q.empty()) {
+      IonBase* ion = q.front();
+      q.pop();
+      smp->averages(ion);
+      trim->trim(ion, q);
+      delete ion;
+    }
+    printf("H %lld\n", h);
+    fputs(os.str().c_str(), stdout);
+    os.str("");
+    if (th) {
+      const std::vector<Point>& v = th->getHistory();
+      for (; nh < v.size(); ++nh) printf("%.17g %.17g %.17g\n", v[nh](0), v[nh](1), v[nh](2));
+    }
+  }
+  if (tv) {
+    printf("M");
+    for (int x = 0; x < 20; ++x)
+      for (int y = 0; y < 20; ++y)
+        for (int e = 0; e < 3; ++e) printf(" %d", tv->vmap[x][y][e]);
+    printf("\n");
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) {
     fprintf(stderr, "usage: ref_harness stop|avg|trace|tally ...\n");
@@ -423,6 +487,9 @@ int main(int argc, char** argv) {
   }
   if (cmd == "trace") return cmd_trace(read_spec(argv[2]), atoll(argv[3]), atoi(argv[4]));
   if (cmd == "tally") return cmd_tally(read_spec(argv[2]));
+  if (cmd == "log")
+    return cmd_log(read_spec(argv[2]), argv[3], argc > 4 ? atoi(argv[4]) : -1, argc > 5 ? atoi(argv[5]) : -1,
+                   argc > 6 ? atoi(argv[6]) : -1);
   fprintf(stderr, "unknown command %s\n", argv[1]);
   return 2;
 }

@@ -79,6 +79,16 @@ typedef struct {
   double seconds;
 } orc_run_out;
 
+/* One line of the reference's stream-writing TrimBase subclasses (kinds in
+ * runs.cpp, LogHooks); 80 bytes, the layout of the device's mt_record. */
+typedef struct {
+  double pos[3];
+  double E, value, m;
+  unsigned long long id;
+  unsigned hist;
+  int Z, gen, tag, kind, pad;
+} orc_record;
+
 int orc_load_tables(const char* path);
 double orc_rpstop(int z2, double e);
 double orc_rstop(int z1, double m1, double E, int z2);
@@ -99,6 +109,12 @@ int orc_run(const orc_run_spec* spec, orc_run_out* out);
 int orc_workcount(const orc_run_spec* spec, double* out);
 /* (eps, b, E, target Z) of every collision of the spec's histories */
 long long orc_scatter_inputs(const orc_run_spec* spec, double* rows, long long max_rows);
+/* event records (mask: 1 defects, 2 history, 4 phonons) and the vacancy map
+ * (int[20][20][3], accumulated; NULL: none) of the spec's histories, run in
+ * order on one thread; returns the number of records (those past
+ * max_records are not stored) */
+long long orc_run_log(const orc_run_spec* spec, int mask, const int* vacmap_Z, orc_record* records,
+                      long long max_records, int* vacmap);
 /* per-collision trace of history `h` (rows of 21 doubles, see runs.cpp) */
 int orc_trace(const orc_run_spec* spec, long long h, double* rows, int max_rows);
 

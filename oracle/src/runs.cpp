@@ -170,6 +170,73 @@ struct TallyHooks : Hooks {
   }
 };
 
+// Event records of the reference's stream-writing TrimBase subclasses, one
+// per line they print (trim.C:466-527, trim.h:139-201):
+//   1 V  TrimDefectLog::vacancyCreation, the recoil     (trim.C:466-469)
+//   2 I, 3 S, 4 R  TrimDefectLog::checkPKAState, the ion at rest (trim.C:471-477)
+//   5    TrimHistory::followRecoil, the ion's position  (trim.h:139-155)
+//   6    TrimPhononOut::followRecoil: Elbind, the recoil (trim.C:520-526)
+//   7    TrimPhononOut::dissipateRecoilEnergy: recoil E + Elbind, the recoil (trim.C:512-518)
+//   8    TrimPhononOut::checkPKAState: the ion's E, the ion at rest (trim.C:503-510)
+// and TrimVacMap's 20 x 20 map of vacancies per (x, y) cell for three recoil
+// Z (trim.C:479-501).  Records are written in the reference's order; the
+// phonon records of a recoil carry the id and tag it has when they are
+// printed (not yet assigned: 0 here, uninitialised in the reference, and -1).
+struct LogHooks : TallyHooks {
+  int mask;
+  const int* vz;
+  std::vector<orc_record>& rec;
+  int* vacmap;
+  long long hist = 0;
+  LogHooks(const orc_run_spec& spec, Partial& part, const orc_run_out* o, int m, const int* z,
+           std::vector<orc_record>& r, int* vm)
+      : TallyHooks(spec, part, o), mask(m), vz(z), rec(r), vacmap(vm) {}
+  void put(int kind, const Ion& i, double value, int id, int tag) {
+    orc_record r{};
+    for (int k = 0; k < 3; ++k) r.pos[k] = i.pos[k];
+    r.E = i.E;
+    r.value = value;
+    r.m = i.m;
+    r.id = (unsigned long long)(unsigned)id;
+    r.hist = unsigned(hist);
+    r.Z = i.Z;
+    r.gen = i.gen;
+    r.tag = tag;
+    r.kind = kind;
+    rec.push_back(r);
+  }
+  bool followRecoil() override {
+    const bool f = TallyHooks::followRecoil();
+    if (mask & 2) put(5, *pka, 0.0, pka->id, pka->tag);
+    if ((mask & 4) && f) put(6, *recoil, element->Elbind, 0, -1);
+    return f;
+  }
+  void vacancyCreation() override {
+    TallyHooks::vacancyCreation();
+    if (mask & 1) put(1, *recoil, 0.0, recoil->id, recoil->tag);
+    if (vacmap) {  // trim.C:481-500
+      int x = int((recoil->pos[0] * 20) / sample_w[0]);
+      int y = int((recoil->pos[1] * 20) / sample_w[1]);
+      x -= int(x / 20) * 20;
+      y -= int(y / 20) * 20;
+      const int c = recoil->Z == vz[0] ? 0 : recoil->Z == vz[1] ? 1 : recoil->Z == vz[2] ? 2 : -1;
+      if (c >= 0 && x >= 0 && y >= 0) vacmap[(x * 20 + y) * 3 + c]++;
+    }
+  }
+  void dissipateRecoilEnergy() override {
+    TallyHooks::dissipateRecoilEnergy();
+    if (mask & 4) put(7, *recoil, recoil->E + element->Elbind, 0, -1);
+  }
+  void checkPKAState() override {
+    TallyHooks::checkPKAState();
+    const int st = pka->state;
+    if ((mask & 1) && (st == INTERSTITIAL || st == SUBSTITUTIONAL || st == REPLACEMENT))
+      put(st == INTERSTITIAL ? 2 : st == SUBSTITUTIONAL ? 3 : 4, *pka, 0.0, pka->id, pka->tag);
+    if ((mask & 4) && st != MOVING && st != LOST) put(8, *pka, pka->E, pka->id, pka->tag);
+  }
+  double sample_w[3] = {1.0, 1.0, 1.0};
+};
+
 Ion* make_primary(const orc_run_spec& s, long long h) {
   const long long j = s.pka_uniform ? 0 : (h - s.hist0);
   Ion* ion = new Ion;
@@ -264,8 +331,8 @@ uint32_t mt_history_seed(uint32_t master, long long h) {
   return uint32_t(flight_key_child(master, uint32_t(h)));
 }
 
-template <class Draws>
-static void run_history(const orc_run_spec& s, long long h, Sample& smp, Draws& draws, TallyHooks& hk,
+template <class Draws, class H>
+static void run_history(const orc_run_spec& s, long long h, Sample& smp, Draws& draws, H& hk,
                         Partial& p) {
   const Config cfg{s.tmin, s.tau, s.cw, s.scale, s.potential};
   hk.hidx = h - s.hist0;
@@ -527,6 +594,39 @@ struct TraceHooks : Hooks {
   }
 };
 }  // namespace
+
+long long orc_run_log(const orc_run_spec* sp, int mask, const int* vacmap_Z, orc_record* records,
+                      long long max_records, int* vacmap) {
+  const orc_run_spec& s = *sp;
+  Partial p;
+  p.range_hist.assign(s.range_nb + 2, 0);
+  p.lat_y.assign(s.lat_nb + 2, 0);
+  p.lat_z.assign(s.lat_nb + 2, 0);
+  p.vac_hist.assign(s.vac_nb + 2, 0);
+  p.repl_hist.assign(s.vac_nb + 2, 0);
+  std::vector<std::unique_ptr<Material>> mats;
+  std::unique_ptr<Sample> smp = build_sample(s, mats);
+  orc_run_out none{};
+  std::vector<orc_record> rec;
+  const int zero[3] = {0, 0, 0};
+  LogHooks hk(s, p, &none, mask, vacmap_Z ? vacmap_Z : zero, rec, vacmap);
+  for (int i = 0; i < 3; ++i) hk.sample_w[i] = s.w[i];
+  MtRng rng;
+  MtDraws mtd(rng);
+  PhiloxDraws phd(s.seed);
+  for (long long h = s.hist0; h < s.hist0 + s.n_hist; ++h) {  // in order, one thread
+    hk.hist = h;
+    if (s.rng == 0) {
+      rng.seed(mt_history_seed(s.seed, h));
+      run_history(s, h, *smp, mtd, hk, p);
+    } else {
+      run_history(s, h, *smp, phd, hk, p);
+    }
+  }
+  const long long n = (long long)rec.size();
+  for (long long i = 0; i < n && i < max_records; ++i) records[i] = rec[size_t(i)];
+  return n;
+}
 
 int orc_trace(const orc_run_spec* sp, long long h, double* rows, int max_rows) {
   const orc_run_spec& s = *sp;

@@ -33,6 +33,14 @@ def trace_cases():
             "wire": configs.wire_b_si(4), "bwire": configs.burried_wire_p_si(4)}
 
 
+def log_cases():
+    """name -> (problem, TrimVacMap z1 z2 z3) for the event-stream fixtures"""
+    return {"vac_cu_10keV": (configs.vacancy_case("cu", 1e4, 6), (29, -1, -1)),
+            "wire": (configs.wire_b_si(12), (14, 5, -1)),
+            "c4": (c4_uniform(2000.0, 3).with_(primaries=Primaries.single(
+                54, 131.29, 3.0e4, (1000.0, 1000.0, 1000.0), (0.6, 0.8, 0.0))), (92, 8, 54))}
+
+
 def tally_cases(quick=False):
     """name -> (problem, processes used to write it)"""
     s = 0.1 if quick else 1.0

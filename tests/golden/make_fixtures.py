@@ -12,12 +12,15 @@ computed by the reference library; the harness only drives its API.
                       the first collisions of one history, std::mt19937 stream
   tally_<case>.json   full-cascade tallies (FIFO drain of apps/runmytrim.C:71-93),
                       one std::mt19937 seed per history
+  log_<case>.npz      what TrimDefectLog and TrimPhononOut write (17 digits), the
+                      positions TrimHistory stores and TrimVacMap's map
+                      (trim.C:466-527, trim.h:139-227), histories in order
   runmytrim/<case>/   a JSON deck of the reference (tests/json, validation/) with
                       fewer ions, and the output file and totals the reference's
                       runmytrim loop writes for it (oracle/_ref/runmytrim_ref:
                       the reference's driver loop and output modules; jsoncpp is
                       absent, so the deck is handed over as a flat spec)
-Usage:  python tests/golden/make_fixtures.py [--quick] [--tally=NAME] [--trace=NAME] [--runmytrim]
+Usage:  python tests/golden/make_fixtures.py [--quick] [--tally=NAME] [--trace=NAME] [--logs] [--runmytrim]
 """
 import json
 import os
@@ -37,7 +40,7 @@ import oracle_lib as O  # noqa: E402
 
 HARNESS = O.REF_HARNESS
 
-from fixture_cases import ENERGIES, IONS, TARGETS, tally_cases, trace_cases  # noqa: E402
+from fixture_cases import ENERGIES, IONS, TARGETS, log_cases, tally_cases, trace_cases  # noqa: E402
 
 
 def run(args, stdin=None):
@@ -94,6 +97,43 @@ def traces(max_rows=1500, only=None):
                 np.savez_compressed(os.path.join(HERE, "trace_%s_h%d.npz" % (name, h)), rows=rows)
         finally:
             os.unlink(path)
+
+
+def parse_log(txt, kinds):
+    """Lines of one log mode -> rows [hist, kind, numbers...]; `kinds` maps a
+    leading letter to a kind code (None: no letter)."""
+    rows, h = [], -1
+    for line in txt.splitlines():
+        f = line.split()
+        if not f:
+            continue
+        if f[0] == "H":
+            h = int(f[1])
+            continue
+        if kinds:
+            rows.append([h, kinds[f[0]]] + [float(x) for x in f[1:]])
+        else:
+            rows.append([h, 0] + [float(x) for x in f])
+    return np.array(rows, np.float64).reshape(-1, 2 + (len(rows[0]) - 2 if rows else 0))
+
+
+def logs():
+    for name, (prob, vz) in log_cases().items():
+        path = spec_file(prob)
+        try:
+            z = [str(v) for v in vz]
+            d = parse_log(run(["log", path, "defects"]), {"V": 1, "I": 2, "S": 3, "R": 4})
+            # TrimPhononOut prints recoils before their id is set (an
+            # uninitialised int in the reference): the id column is dropped
+            p = np.delete(parse_log(run(["log", path, "phonons"]), None), 9, axis=1)
+            h = parse_log(run(["log", path, "history"]), None)
+            m = run(["log", path, "vacmap"] + z).split("M", 1)[1].split()
+        finally:
+            os.unlink(path)
+        np.savez_compressed(os.path.join(HERE, "log_%s.npz" % name), defects=d, phonons=p, history=h,
+                            vacmap=np.array([int(v) for v in m], np.int32).reshape(20, 20, 3),
+                            vacmap_Z=np.array(vz, np.int32))
+        print(name, "defects", len(d), "phonons", len(p), "history", len(h), flush=True)
 
 
 def parse_tally(txt):
@@ -289,6 +329,9 @@ def main():
         return
     only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--tally=")]
     only_tr = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--trace=")]
+    if "--logs" in sys.argv:
+        logs()
+        return
     if only or only_tr:  # regenerate the named tally / trace fixtures only
         if only_tr:
             traces(only=only_tr)
@@ -298,6 +341,7 @@ def main():
     stopping_grid()
     average_table()
     traces()
+    logs()
     tallies(quick)
 
 

@@ -84,6 +84,8 @@ def lib():
         L.orc_run.argtypes = [C.POINTER(RunSpec), C.POINTER(RunOut)]
         L.orc_workcount.argtypes = [C.POINTER(RunSpec), dp]
         L.orc_trace.argtypes = [C.POINTER(RunSpec), C.c_longlong, dp, C.c_int]
+        L.orc_run_log.restype = C.c_longlong
+        L.orc_run_log.argtypes = [C.POINTER(RunSpec), C.c_int, ip, C.c_void_p, C.c_longlong, ip]
         _lib = L
     return _lib
 
@@ -189,6 +191,28 @@ def scatter_inputs(prob, max_rows: int) -> np.ndarray:
     n = lib().orc_scatter_inputs(C.byref(spec), _p(rows, dp), C.c_longlong(max_rows))
     del keep
     return rows[:n]
+
+
+# one line of the reference's stream-writing TrimBase subclasses (orc_record)
+RECORD = np.dtype([("pos", "<f8", 3), ("E", "<f8"), ("value", "<f8"), ("m", "<f8"), ("id", "<u8"),
+                   ("hist", "<u4"), ("Z", "<i4"), ("gen", "<i4"), ("tag", "<i4"), ("kind", "<i4"),
+                   ("pad", "<i4")])
+assert RECORD.itemsize == 80
+
+
+def run_log(prob, mask: int, vacmap_Z=None, rng: int = 0, max_records: int = 1 << 20):
+    """Event records (mask 1 defects, 2 history, 4 phonons) of `prob`'s histories
+    in the reference's order, and the 20 x 20 x 3 vacancy map when vacmap_Z is given."""
+    L = lib()
+    s, keep = make_spec(prob, rng, 1)
+    rec = np.zeros(max_records, RECORD)
+    vz = np.array(list(vacmap_Z) if vacmap_Z is not None else [0, 0, 0], np.int32)
+    vm = np.zeros((20, 20, 3), np.int32) if vacmap_Z is not None else None
+    n = L.orc_run_log(C.byref(s), mask, _p(vz, ip), rec.ctypes.data_as(C.c_void_p), max_records,
+                      _p(vm, ip) if vm is not None else None)
+    if n > max_records:
+        raise RuntimeError("record buffer too small: %d records" % n)
+    return rec[:n], vm
 
 
 def trace(prob, h: int, max_rows: int, rng: int = 0) -> np.ndarray:

@@ -240,3 +240,27 @@ def test_device_draw_scheme_matches_reference_statistically(name, prob, n):
     v = mine["range_sum2"] / mine["range_n"] - m ** 2
     sig = math.sqrt(v_ref / ref["range_n"] + v / mine["range_n"])
     assert abs(m - m_ref) < 2.5 * sig + 1e-9
+
+
+# ---------------------------------------------------------------- event streams
+@pytest.mark.parametrize("name", ["vac_cu_10keV", "wire", "c4"])
+def test_event_logs_bitwise(name):
+    """The oracle's event records equal what the reference's TrimDefectLog,
+    TrimPhononOut, TrimHistory and TrimVacMap write for the same histories."""
+    from fixture_cases import log_cases
+    prob, vz = log_cases()[name]
+    g = np.load(os.path.join(GOLD, "log_%s.npz" % name))
+    rec, vm = O.run_log(prob, 7, vz, rng=0)
+    d = rec[(rec["kind"] >= 1) & (rec["kind"] <= 4)]  # V I S R: [hist kind x y z Z m E id gen tag]
+    mine = np.column_stack([d["hist"], d["kind"], d["pos"], d["Z"], d["m"], d["E"], d["id"], d["gen"],
+                            d["tag"]]).astype(np.float64)
+    assert mine.shape == g["defects"].shape
+    assert np.array_equal(mine, g["defects"])
+    p = rec[rec["kind"] >= 6]  # phonons: [hist 0 value x y z Z m E gen tag]
+    mine = np.column_stack([p["hist"], np.zeros(len(p)), p["value"], p["pos"], p["Z"], p["m"], p["E"],
+                            p["gen"], p["tag"]]).astype(np.float64)
+    assert np.array_equal(mine, g["phonons"])
+    h = rec[rec["kind"] == 5]  # history: [hist 0 x y z]
+    assert np.array_equal(np.column_stack([h["hist"], np.zeros(len(h)), h["pos"]]), g["history"])
+    assert np.array_equal(vm, g["vacmap"])
+    assert vm.sum() > 0
