@@ -145,7 +145,7 @@ typedef struct {
 typedef struct {
   int block;          /* threads per workgroup (multiple of 64, at most 256) */
   int blocks_per_cu;  /* resident workgroups per CU (0: two waves per SIMD) */
-  int stack_cap;      /* per-wave LDS recoil stack entries (0: 192) */
+  int stack_cap;      /* per-wave LDS recoil stack entries (0: 128) */
   long long queue_cap;/* HBM queue entries (power of two) */
   int lds_hist;       /* bin vacancy/replacement depths in LDS first */
   int graph;          /* >= 0: replay the run sequence (reset, kernel, flush,

@@ -480,9 +480,9 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
 
   // launch shape
   // defaults: the shape measured best for Cu -> Cu cascades (two waves per
-  // SIMD, 192-entry LDS stacks)
+  // SIMD, 128-entry LDS stacks)
   const int block = ctx->tune.block ? ctx->tune.block : 256;
-  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 192;
+  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 128;
   const int nwb = block / 64;
   const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
   size_t lds = blob.size() + size_t(nwb) * cascade_stack_bytes(cap);
@@ -810,9 +810,9 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // (fewer registers; a kernel whose registers do not allow the requested
   // blocks per CU runs at the most that fit); queue capacity stays at its
   // default
-  static const int kCand[][3] = {{256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {128, 4, 192}, {128, 4, 128},
-                                 {64, 8, 160},  {64, 8, 128},  {256, 1, 384}, {256, 1, 256}, {128, 3, 256},
-                                 {128, 2, 384}, {64, 6, 192},  {256, 3, 128}, {256, 3, 96},  {128, 6, 128}};
+  static const int kCand[][3] = {{256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {256, 2, 96},  {256, 2, 64},
+                                 {128, 4, 128}, {128, 4, 96},  {64, 8, 128},  {64, 8, 96},   {256, 1, 256},
+                                 {128, 3, 256}, {64, 6, 192},  {256, 3, 128}, {256, 3, 96},  {128, 6, 96}};
   std::vector<mt_tuning> cand;
   for (const auto& c : kCand) {
     mt_tuning t{};

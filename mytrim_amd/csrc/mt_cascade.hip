@@ -130,10 +130,28 @@ __device__ __forceinline__ uint64_t sum_add(uint64_t h, uint64_t w) {
 }
 __device__ __forceinline__ uint32_t sum_end(uint64_t h) { return (uint32_t)(h >> 32); }
 
+// Kernel arguments used off the collision path are read from the argument
+// segment where they are used (scalar loads) rather than held in scalar
+// registers across the loop: the argument block (~150 dwords) does not fit
+// beside the loop state in the 102 scalar registers, and every use of a
+// spilled one costs a v_readlane.  The empty asm makes the pointer opaque,
+// so the loads stay at their uses instead of being hoisted out of the loop.
+typedef const __attribute__((address_space(4))) Params* ArgPtr;
+__device__ __forceinline__ ArgPtr cold_args() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t k = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return (ArgPtr)k;
+#else
+  return nullptr;
+#endif
+}
+
 // Appends one record per lane where `pred` holds (called by the whole wave:
 // one counter atomic per wave; records past the capacity are counted, not
 // stored)
-__device__ __forceinline__ void log_put(const LogDev& L, unsigned long long* count, bool pred, const LogRecord& r) {
+__device__ __forceinline__ void log_put(LogRecord* rec, unsigned long long cap, unsigned long long* count, bool pred,
+                                        const LogRecord& r) {
   const uint64_t m = __ballot(pred);
   if (m == 0ull) return;
   const int lane = threadIdx.x & 63;
@@ -143,7 +161,7 @@ __device__ __forceinline__ void log_put(const LogDev& L, unsigned long long* cou
   base = bcast64(base, leader);
   if (pred) {
     const uint64_t i = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (i < L.cap) L.rec[i] = r;
+    if (i < cap) rec[i] = r;
   }
 }
 
@@ -306,28 +324,29 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       // a wave whose last poll found nothing waits a few iterations before
       // polling again, unless all its lanes are idle
       if (scount < n_idle && (poll_skip == 0 || n_idle == 64)) {
+        const ArgPtr A = cold_args();
         // one lane loads one entry, so a batch holds at most 64
         const int want = min(64, min(cap - scount, max(n_idle, cap / 2) - scount));
         // the home shard first, then up to three others (rotating victim)
         int qs = home;
         uint64_t h0 = ld_relaxed(&ctrl[kQHead(qs)]);
         uint64_t t0 = ld_relaxed(&ctrl[kQTail(qs)]);
-        for (int v = 0; v < 3 && t0 <= h0 && P.q.nq > 1; ++v) {
-          victim = (victim + 1) & (P.q.nq - 1);
-          if (victim == home) victim = (victim + 1) & (P.q.nq - 1);
+        for (int v = 0; v < 3 && t0 <= h0 && A->q.nq > 1; ++v) {
+          victim = (victim + 1) & (A->q.nq - 1);
+          if (victim == home) victim = (victim + 1) & (A->q.nq - 1);
           qs = victim;
           h0 = ld_relaxed(&ctrl[kQHead(qs)]);
           t0 = ld_relaxed(&ctrl[kQTail(qs)]);
         }
         const bool may_pop = t0 > h0;
-        bool may_pka = !may_pop && ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)P.n_hist;
+        bool may_pka = !may_pop && ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
         long long n_adm = 0;
         if (may_pka) {  // cascade budget (see Params)
           const uint64_t live = ld_relaxed(&ctrl[kLive]);
           const uint64_t done = ld_relaxed(&ctrl[kCascDone]);
           const double fhat =
-              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : P.f_init;
-          const long long cmax = max((long long)(P.budget / fmax(fhat, 1.0)), P.c_floor);
+              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : A->f_init;
+          const long long cmax = max((long long)(A->budget / fmax(fhat, 1.0)), A->c_floor);
           n_adm = min((long long)want, cmax - (long long)live);
           may_pka = n_adm > 0;
         }
@@ -361,29 +380,29 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
             uint64_t slot = 0;
             if (mine) {
               const uint64_t pos = base + (uint64_t)lane;
-              slot = ((uint64_t)qs << P.q.sub_log2) | (pos & P.q.sub_mask);
-              const uint32_t lap = (uint32_t)(pos >> P.q.sub_log2) + 1u;
+              slot = ((uint64_t)qs << A->q.sub_log2) | (pos & A->q.sub_mask);
+              const uint32_t lap = (uint32_t)(pos >> A->q.sub_log2) + 1u;
               const unsigned long long t_start = (unsigned long long)wall_clock64();
               const int j = scount + lane;
               // wait for the lap, then for fields that match the checksum;
               // fields go straight to this lane's stack slot
               while (true) {
-                const uint64_t rw = ld_agent(&P.q.ready[slot]);
+                const uint64_t rw = ld_agent(&A->q.ready[slot]);
                 if ((uint32_t)(rw >> 32) == lap) {
                   uint64_t h = sum_init(lap), w;
-                  w = ld_agent(&P.q.px[slot]); S.px[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.py[slot]); S.py[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.pz[slot]); S.pz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.dx[slot]); S.dx[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.dy[slot]); S.dy[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.dz[slot]); S.dz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.E[slot]); S.E[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
-                  w = ld_agent(&P.q.key[slot]); S.key[j] = w; h = sum_add(h, w);
-                  w = ld_agent(&P.q.mt[slot]); h = sum_add(h, w);
+                  w = ld_agent(&A->q.px[slot]); S.px[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.py[slot]); S.py[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.pz[slot]); S.pz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.dx[slot]); S.dx[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.dy[slot]); S.dy[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.dz[slot]); S.dz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.E[slot]); S.E[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
+                  w = ld_agent(&A->q.key[slot]); S.key[j] = w; h = sum_add(h, w);
+                  w = ld_agent(&A->q.mt[slot]); h = sum_add(h, w);
                   const uint32_t m = (uint32_t)(w >> 32);
                   S.meta[j] = (int)(m & 0xffu) < T.n_species ? m : 0u;
                   S.tag[j] = (int)(uint32_t)w;
-                  const uint32_t wh = ld_agent(&P.q.hist[slot]);
+                  const uint32_t wh = ld_agent(&A->q.hist[slot]);
                   S.hist[j] = wh;
                   h = sum_add(h, wh);
                   if (sum_end(h) == (uint32_t)rw) {
@@ -392,7 +411,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
                   }
                 }
                 __builtin_amdgcn_s_sleep(1);
-                if ((unsigned long long)wall_clock64() - t_start > P.max_wait_ticks) {
+                if ((unsigned long long)wall_clock64() - t_start > A->max_wait_ticks) {
                   err |= kErrTimeout;
                   mine = false;
                   break;
@@ -415,24 +434,24 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           uint64_t first = 0;
           if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_adm);
           first = bcast64(first, 0);
-          const long long got = min(n_adm, max(0ll, P.n_hist - (long long)first));
+          const long long got = min(n_adm, max(0ll, A->n_hist - (long long)first));
           if (lane == 0 && got > 0) atomicAdd(&ctrl[kLive], (unsigned long long)got);
           const long long jh = (long long)first + lane;
           if (lane < got) {
-            __hip_atomic_store(&P.h_out[jh], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const long long e = P.pka_uniform ? 0 : jh;
+            __hip_atomic_store(&A->h_out[jh], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const long long e = A->pka_uniform ? 0 : jh;
             const int j = scount + lane;
-            S.px[j] = P.pka_pos[3 * e + 0];
-            S.py[j] = P.pka_pos[3 * e + 1];
-            S.pz[j] = P.pka_pos[3 * e + 2];
-            S.dx[j] = P.pka_dir[3 * e + 0];
-            S.dy[j] = P.pka_dir[3 * e + 1];
-            S.dz[j] = P.pka_dir[3 * e + 2];
-            S.E[j] = P.pka_E[e];
+            S.px[j] = A->pka_pos[3 * e + 0];
+            S.py[j] = A->pka_pos[3 * e + 1];
+            S.pz[j] = A->pka_pos[3 * e + 2];
+            S.dx[j] = A->pka_dir[3 * e + 0];
+            S.dy[j] = A->pka_dir[3 * e + 1];
+            S.dz[j] = A->pka_dir[3 * e + 2];
+            S.E[j] = A->pka_E[e];
             S.key[j] = 0ull;
-            S.meta[j] = (uint32_t)P.pka_species[e];
+            S.meta[j] = (uint32_t)A->pka_species[e];
             S.tag[j] = -1;
-            S.hist[j] = (uint32_t)(P.hist0 + jh);
+            S.hist[j] = (uint32_t)(A->hist0 + jh);
           }
           scount += (int)max(got, 0ll);
         }
@@ -472,6 +491,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       if (__ballot(active) != 0ull) backoff = 1;
       if (__ballot(active) == 0ull) {
         // nothing to run: idle protocol and termination test
+        const ArgPtr A = cold_args();
         if (!wave_idle) {
           if (lane == 0) atomicAdd(&ctrl[kState], 1ull);
           wave_idle = true;
@@ -481,11 +501,11 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         if ((s1 & 0xffffull) == ((s1 >> 16) & 0xffffull)) {
           const uint64_t pn = ld_relaxed(&ctrl[kPkaNext]);
           bool empty = true;
-          for (int q = 0; q < P.q.nq; ++q) empty = empty && ld_relaxed(&ctrl[kQHead(q)]) == ld_relaxed(&ctrl[kQTail(q)]);
+          for (int q = 0; q < A->q.nq; ++q) empty = empty && ld_relaxed(&ctrl[kQHead(q)]) == ld_relaxed(&ctrl[kQTail(q)]);
           const uint64_t s2 = ld_relaxed(&ctrl[kState]);
-          if (s1 == s2 && empty && pn >= (unsigned long long)P.n_hist) break;
+          if (s1 == s2 && empty && pn >= (unsigned long long)A->n_hist) break;
         }
-        if ((unsigned long long)wall_clock64() - idle_since > P.max_wait_ticks) {
+        if ((unsigned long long)wall_clock64() - idle_since > A->max_wait_ticks) {
           err |= kErrTimeout;
           break;
         }
@@ -617,8 +637,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         // fate of recoil and ion (trim.C:357-411)
         if (lost) {
           state = 4;
-          atomicAdd(&P.energy[eElost], F.E);
-          atomicAdd(&P.energy[eEdrop], den);
+          atomicAdd(&cold_args()->energy[eElost], F.E);
+          atomicAdd(&cold_args()->energy[eEdrop], den);
         } else if (rE > ec.Edisp - ec.Elbind) {
           const bool follow = P.follow_max_gen < 0 || F.gen + 1 < P.follow_max_gen;
           if (follow) {
@@ -631,7 +651,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
             rsp = ec.recoil_species;
             rtag = mtag;
           } else {
-            atomicAdd(&P.energy[eEunf], den);
+            atomicAdd(&cold_args()->energy[eEunf], den);
           }
           if (F.E > ec.Edisp) {
             // vacancyCreation (trim.C:439-443, TrimVacCount.C:31-42, KP trim.C:445-464)
@@ -639,7 +659,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
             if (P.lds_hist)
               atomicAdd(&lvac[bn], 1u);
             else
-              atomicAdd(&P.vac_hist[bn], 1ull);
+              atomicAdd(&cold_args()->vac_hist[bn], 1ull);
             int extra = 0;
             if (P.kp && F.gen + 1 == P.follow_max_gen) {
               const double ed = 0.0115 * m_pow(mc.az, -7.0 / 3.0) * rE;
@@ -649,14 +669,14 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
               extra = (int)(0.8 * Ev / (2.0 * ec.Edisp));
               l_kpvac32 += (unsigned)extra;
             }
-            if (PERHIST) atomicAdd(&P.h_vac[F.hist - (uint32_t)P.hist0], 1 + extra);
+            if (PERHIST) atomicAdd(&cold_args()->h_vac[F.hist - (uint32_t)P.hist0], 1 + extra);
             state = -1;  // vacancy marker, ion keeps moving
           } else {
             const int bn = bin_of(rx, P.vac_x0, P.vac_bw, P.vac_nb);
             if (P.lds_hist)
               atomicAdd(&lrepl[bn], 1u);
             else
-              atomicAdd(&P.repl_hist[bn], 1ull);
+              atomicAdd(&cold_args()->repl_hist[bn], 1ull);
             state = (H->species[F.sp].Z == ec.Z) ? 1 : 2;
             kadd(enuc_s, enuc_e, F.E);
           }
@@ -681,8 +701,11 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     // collision, in trim.C order: followRecoil (history, spawn), vacancy,
     // dissipation, ion at rest; and TrimVacMap's cell (trim.C:479-501)
     if constexpr (PERHIST) {
-      if (P.log.mask) {
-        const LogDev& L = P.log;
+      const ArgPtr A = cold_args();
+      const int lmask = A->log.mask;
+      if (lmask) {
+        LogRecord* const lrec = A->log.rec;
+        const unsigned long long lcap = A->log.cap;
         unsigned long long* lc = &ctrl[kLogCount];
         const bool coll = active && !vacuum;
         LogRecord ion, rec;
@@ -709,47 +732,47 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         rec.gen = F.gen + 1;
         rec.tag = -1;
         rec.pad = 0;
-        if (L.mask & kLogHistory) {
+        if (lmask & kLogHistory) {
           ion.kind = 5;
-          log_put(L, lc, coll && lg_above, ion);
+          log_put(lrec, lcap, lc, coll && lg_above, ion);
         }
-        if (L.mask & kLogPhonons) {
+        if (lmask & kLogPhonons) {
           rec.kind = 6;
           rec.value = lg_lb;
-          log_put(L, lc, coll && lg_follow, rec);
+          log_put(lrec, lcap, lc, coll && lg_follow, rec);
         }
-        if (L.mask & kLogDefects) {
+        if (lmask & kLogDefects) {
           rec.kind = 1;
           rec.value = 0.0;
           rec.id = lg_follow ? child_key(F.key, (uint32_t)F.ic) : 0ull;
           rec.tag = lg_follow ? rtag : -1;
-          log_put(L, lc, coll && state == -1, rec);
+          log_put(lrec, lcap, lc, coll && state == -1, rec);
           rec.id = 0ull;
           rec.tag = -1;
         }
-        if ((L.mask & kLogVacmap) && coll && state == -1) {
+        if ((lmask & kLogVacmap) && coll && state == -1) {
           int x = (int)((rx * kVacMapN) / G.w[0]);
           int y = (int)((ry * kVacMapN) / G.w[1]);
           x -= (x / kVacMapN) * kVacMapN;
           y -= (y / kVacMapN) * kVacMapN;
-          const int c = lg_rZ == L.vz[0] ? 0 : lg_rZ == L.vz[1] ? 1 : lg_rZ == L.vz[2] ? 2 : -1;
+          const int c = lg_rZ == A->log.vz[0] ? 0 : lg_rZ == A->log.vz[1] ? 1 : lg_rZ == A->log.vz[2] ? 2 : -1;
           // the reference indexes out of its map for x or y < 0; those are skipped
-          if (c >= 0 && x >= 0 && y >= 0) atomicAdd(&L.vacmap[(x * kVacMapN + y) * 3 + c], 1);
+          if (c >= 0 && x >= 0 && y >= 0) atomicAdd(&A->log.vacmap[(x * kVacMapN + y) * 3 + c], 1);
         }
-        if (L.mask & kLogPhonons) {
+        if (lmask & kLogPhonons) {
           rec.kind = 7;
           rec.value = rE + lg_lb;
-          log_put(L, lc, coll && !lg_above && state != 4, rec);
+          log_put(lrec, lcap, lc, coll && !lg_above && state != 4, rec);
         }
         const bool rest = coll && state >= 1 && state <= 3;
-        if (L.mask & kLogDefects) {
+        if (lmask & kLogDefects) {
           ion.kind = state == 3 ? 2 : state == 2 ? 3 : 4;
-          log_put(L, lc, rest, ion);
+          log_put(lrec, lcap, lc, rest, ion);
         }
-        if (L.mask & kLogPhonons) {
+        if (lmask & kLogPhonons) {
           ion.kind = 8;
           ion.value = F.E;
-          log_put(L, lc, rest, ion);
+          log_put(lrec, lcap, lc, rest, ion);
         }
       }
     }
@@ -780,6 +803,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       const int n_over = n_push - min(n_push, room);
       scount += min(n_push, room);
       if (n_over > 0) {
+        const ArgPtr A = cold_args();
         ++c_qops;
         uint64_t base = 0, head = 0;
         if (lane == 0) {
@@ -795,7 +819,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           peak_seen = fill;
           if (lane == 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
         }
-        if (fill > (long long)P.q.sub_mask) {  // abort: every wave leaves, the caller reports the error
+        if (fill > (long long)A->q.sub_mask) {  // abort: every wave leaves, the caller reports the error
           err |= kErrQueueFull;
           if (lane == 0) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
         }
@@ -804,29 +828,29 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         uint32_t lap = 0, sum = 0;
         if (over) {
           const uint64_t pos = base + (uint64_t)(r - room);
-          slot = ((uint64_t)home << P.q.sub_log2) | (pos & P.q.sub_mask);
-          lap = (uint32_t)(pos >> P.q.sub_log2) + 1u;
+          slot = ((uint64_t)home << A->q.sub_log2) | (pos & A->q.sub_mask);
+          lap = (uint32_t)(pos >> A->q.sub_log2) + 1u;
           uint64_t h = sum_init(lap), w;
-          w = (uint64_t)__double_as_longlong(rx); st_agent(&P.q.px[slot], w); h = sum_add(h, w);
-          w = (uint64_t)__double_as_longlong(ry); st_agent(&P.q.py[slot], w); h = sum_add(h, w);
-          w = (uint64_t)__double_as_longlong(rz); st_agent(&P.q.pz[slot], w); h = sum_add(h, w);
-          w = (uint64_t)__double_as_longlong(rdx); st_agent(&P.q.dx[slot], w); h = sum_add(h, w);
-          w = (uint64_t)__double_as_longlong(rdy); st_agent(&P.q.dy[slot], w); h = sum_add(h, w);
-          w = (uint64_t)__double_as_longlong(rdz); st_agent(&P.q.dz[slot], w); h = sum_add(h, w);
-          w = (uint64_t)__double_as_longlong(rE); st_agent(&P.q.E[slot], w); h = sum_add(h, w);
-          st_agent(&P.q.key[slot], rkey);
+          w = (uint64_t)__double_as_longlong(rx); st_agent(&A->q.px[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(ry); st_agent(&A->q.py[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rz); st_agent(&A->q.pz[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rdx); st_agent(&A->q.dx[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rdy); st_agent(&A->q.dy[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rdz); st_agent(&A->q.dz[slot], w); h = sum_add(h, w);
+          w = (uint64_t)__double_as_longlong(rE); st_agent(&A->q.E[slot], w); h = sum_add(h, w);
+          st_agent(&A->q.key[slot], rkey);
           h = sum_add(h, rkey);
           w = ((uint64_t)meta << 32) | (uint32_t)rtag;
-          st_agent(&P.q.mt[slot], w);
+          st_agent(&A->q.mt[slot], w);
           h = sum_add(h, w);
-          st_agent(&P.q.hist[slot], F.hist);
+          st_agent(&A->q.hist[slot], F.hist);
           sum = sum_end(sum_add(h, F.hist));
         }
         // this wave's outstanding memory operations (the entry and the
         // history's flight-count increment) complete before the ready word;
         // readers check the checksum in any case
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (over) st_agent(&P.q.ready[slot], ((uint64_t)lap << 32) | sum);
+        if (over) st_agent(&A->q.ready[slot], ((uint64_t)lap << 32) | sum);
       }
     }
 
@@ -850,27 +874,29 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
 #pragma unroll
         for (int f = 0; f < 4; ++f) l_fate32[f] += (state == f + 1) ? 1u : 0u;
         l_rangen32 += (F.gen == 0 && !vacuum && state != 4) ? 1u : 0u;
-        if (vacuum) atomicAdd(&P.energy[eEvac], F.E);
+        if (vacuum) atomicAdd(&cold_args()->energy[eEvac], F.E);
         const uint32_t hl = F.hist - (uint32_t)P.hist0;
         if (PERHIST) {
-          atomicAdd(&P.h_coll[hl], (unsigned long long)(vacuum ? F.ic - 1 : F.ic));
-          atomicAdd(&P.h_flights[hl], 1);
-          atomicAdd(&P.h_Eel[hl], F.eel);
+          const ArgPtr A = cold_args();
+          atomicAdd(&A->h_coll[hl], (unsigned long long)(vacuum ? F.ic - 1 : F.ic));
+          atomicAdd(&A->h_flights[hl], 1);
+          atomicAdd(&A->h_Eel[hl], F.eel);
         }
         if (F.gen == 0) {
+          const ArgPtr A = cold_args();
           if (PERHIST) {
-            P.h_x[hl] = F.px;
-            P.h_y[hl] = F.py;
-            P.h_z[hl] = F.pz;
-            P.h_state[hl] = vacuum ? -1 : (state > 0 ? state : 0);
+            A->h_x[hl] = F.px;
+            A->h_y[hl] = F.py;
+            A->h_z[hl] = F.pz;
+            A->h_state[hl] = vacuum ? -1 : (state > 0 ? state : 0);
           }
           if (!vacuum && state != 4) {
-            atomicAdd(&P.range_hist[bin_of(F.px, P.range_x0, P.range_bw, P.range_nb)], 1ull);
-            const double lo = -0.5 * P.lat_nb * P.lat_bw;
-            atomicAdd(&P.lat_y_hist[bin_of(F.py - P.lat_y0, lo, P.lat_bw, P.lat_nb)], 1ull);
-            atomicAdd(&P.lat_z_hist[bin_of(F.pz - P.lat_z0, lo, P.lat_bw, P.lat_nb)], 1ull);
-            atomicAdd(&P.energy[eRangeSum], F.px);
-            atomicAdd(&P.energy[eRangeSum2], F.px * F.px);
+            atomicAdd(&A->range_hist[bin_of(F.px, A->range_x0, A->range_bw, A->range_nb)], 1ull);
+            const double lo = -0.5 * A->lat_nb * A->lat_bw;
+            atomicAdd(&A->lat_y_hist[bin_of(F.py - A->lat_y0, lo, A->lat_bw, A->lat_nb)], 1ull);
+            atomicAdd(&A->lat_z_hist[bin_of(F.pz - A->lat_z0, lo, A->lat_bw, A->lat_nb)], 1ull);
+            atomicAdd(&A->energy[eRangeSum], F.px);
+            atomicAdd(&A->energy[eRangeSum2], F.px * F.px);
           }
         }
         active = false;
@@ -880,6 +906,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
   }
 
   // ------------------------------------------------------------ flush
+  const ArgPtr A = cold_args();
   // wave sums of the per-lane counters (each 64-bit; lane sums are 32-bit)
   auto wsum = [&](unsigned v) {
     unsigned long long t = v;
@@ -900,7 +927,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     enuc_e += e_b;
   }
   if (lane == 0) {
-    unsigned long long* C = P.counters;
+    unsigned long long* C = A->counters;
     atomicAdd(&C[cColl], n_coll);
     atomicAdd(&C[cFlight], n_flight);
     atomicAdd(&C[cVac], n_vac + n_kpvac);
@@ -917,7 +944,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     atomicAdd(&ctrl[kQueueOps], c_qops);
     if (fe_pending) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
     const int gw = blockIdx.x * nwb + wid;
-    double* we = P.wave_energy + (size_t)gw * (2 * kNumEnergy);
+    double* we = A->wave_energy + (size_t)gw * (2 * kNumEnergy);
     we[2 * eEel] = eel_s;
     we[2 * eEel + 1] = eel_e;
     we[2 * eEnuc] = enuc_s;
@@ -927,8 +954,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
   if (P.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < P.vac_nb + 2; i += blockDim.x) {
-      if (lvac[i]) atomicAdd(&P.vac_hist[i], (unsigned long long)lvac[i]);
-      if (lrepl[i]) atomicAdd(&P.repl_hist[i], (unsigned long long)lrepl[i]);
+      if (lvac[i]) atomicAdd(&A->vac_hist[i], (unsigned long long)lvac[i]);
+      if (lrepl[i]) atomicAdd(&A->repl_hist[i], (unsigned long long)lrepl[i]);
     }
   }
 }
