@@ -122,20 +122,21 @@ __host__ __device__ __forceinline__ void fx_sincos2pi(double u, double& s, doubl
   c = ((q + 1) & 2) ? -cv : cv;
 }
 
-// exp(r) for |r| <= ln2 / 2: Taylor series to r^13 (next term < 4e-18)
+// exp(r) for |r| <= ln2 / 2: degree-11 polynomial fitted to exp on
+// [-ln2/2, ln2/2] by Chebyshev interpolation (maximum error 3.2e-18 before
+// the coefficients are rounded to double; the Taylor series needs degree 13
+// for the same accuracy)
 __host__ __device__ __forceinline__ double fx_exp_core(double r) {
-  double p = 1.0 / 6227020800.0;  // 1/13!
-  p = fma(p, r, 1.0 / 479001600.0);
-  p = fma(p, r, 1.0 / 39916800.0);
-  p = fma(p, r, 1.0 / 3628800.0);
-  p = fma(p, r, 1.0 / 362880.0);
-  p = fma(p, r, 1.0 / 40320.0);
-  p = fma(p, r, 1.0 / 5040.0);
-  p = fma(p, r, 1.0 / 720.0);
-  p = fma(p, r, 1.0 / 120.0);
-  p = fma(p, r, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
-  p = fma(p, r, 0.5);
+  double p = 2.5110037605963777e-08;
+  p = fma(p, r, 2.763263963904103e-07);
+  p = fma(p, r, 2.755724091857897e-06);
+  p = fma(p, r, 2.4801485482328494e-05);
+  p = fma(p, r, 0.00019841269890047113);
+  p = fma(p, r, 0.0013888888952314775);
+  p = fma(p, r, 0.008333333333319601);
+  p = fma(p, r, 0.0416666666664881);
+  p = fma(p, r, 0.1666666666666668);
+  p = fma(p, r, 0.5000000000000019);
   p = fma(p, r, 1.0);
   return fma(p, r, 1.0);
 }
@@ -152,8 +153,8 @@ __host__ __device__ __forceinline__ double fx_exp(double x) {
 }
 
 // log(x), x = 2^e m with m in [sqrt(1/2), sqrt(2)): log m = 2 atanh(s),
-// s = (m - 1) / (m + 1), |s| <= 0.1716, series in s^2 to s^21 (next term
-// < 1e-18 relative).  Subnormal x is exact through frexp; zero, negative,
+// s = (m - 1) / (m + 1), |s| <= 0.1716, series in s^2 to s^19 (the next
+// term is below 2.3e-17 relative).  Subnormal x is exact through frexp; zero, negative,
 // infinite and NaN arguments are selected at the end (-inf, NaN, inf, NaN).
 __host__ __device__ __forceinline__ double fx_log(double x) {
   int e;
@@ -168,8 +169,7 @@ __host__ __device__ __forceinline__ double fx_log(double x) {
   const double s0 = f * y;
   const double s = fma(fma(-d, s0, f), y, s0);
   const double z = s * s;
-  double p = 1.0 / 21.0;
-  p = fma(p, z, 1.0 / 19.0);
+  double p = 1.0 / 19.0;
   p = fma(p, z, 1.0 / 17.0);
   p = fma(p, z, 1.0 / 15.0);
   p = fma(p, z, 1.0 / 13.0);
@@ -187,6 +187,45 @@ __host__ __device__ __forceinline__ double fx_log(double x) {
   v = (x == HUGE_VAL) ? HUGE_VAL : v;
   return (x >= 0.0) ? v : __builtin_nan("");  // negative or NaN: NaN
 }
+
+// The physics only takes exp of finite arguments and logarithms of finite
+// positive ones; these variants leave out the clamping and the special-value
+// selects (5 to 8 instructions each).  exp_fin needs |x| < 2^30; NaN still
+// gives NaN.
+__host__ __device__ __forceinline__ double fx_exp_fin(double x) {
+  const double k = rint(x * kInvLn2);
+  const double r = fma(-k, kLn2Lo, fma(-k, kLn2Hi, x));
+  return fx_ldexp(fx_exp_core(r), (int)k);
+}
+
+__host__ __device__ __forceinline__ double fx_log_pos(double x) {
+  int e;
+  double m = fx_frexp(x, e);
+  if (m < kSqrtHalf) {
+    m *= 2.0;
+    --e;
+  }
+  const double f = m - 1.0;
+  const double d = m + 1.0, y = fx_rcp(d);
+  const double s0 = f * y;
+  const double s = fma(fma(-d, s0, f), y, s0);
+  const double z = s * s;
+  double p = 1.0 / 19.0;
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  const double two_s = 2.0 * s;
+  const double de = (double)e;
+  return fma(de, kLn2Hi, fma(de, kLn2Lo, fma(two_s * z, p, two_s)));
+}
+
+// pow for x > 0 finite and y != 0 (the physics' constant exponents)
+__host__ __device__ __forceinline__ double fx_pow_pos(double x, double y) { return fx_exp_fin(y * fx_log_pos(x)); }
 
 // pow(x, y) = exp(y log x) for x >= 0 (error grows with |y log x|: about
 // 1.7 ulp per unit of |y log x|, 7e-15 relative at |y log x| = 64); x = 0

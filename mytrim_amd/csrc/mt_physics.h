@@ -26,11 +26,12 @@ template <typename R>
 __device__ __forceinline__ R sq(R x) { return x * x; }
 
 // precision-dispatched math: FP64 from mt_fmath.h (within a few ulp, a
-// third to a half of the library's instructions), single precision the
+// third to a half of the library's instructions; the physics passes finite
+// arguments, positive ones to log and pow), single precision the
 // hardware-assisted variants
-__device__ __forceinline__ double m_exp(double x) { return fx_exp(x); }
-__device__ __forceinline__ double m_log(double x) { return fx_log(x); }
-__device__ __forceinline__ double m_pow(double x, double y) { return fx_pow(x, y); }
+__device__ __forceinline__ double m_exp(double x) { return fx_exp_fin(x); }
+__device__ __forceinline__ double m_log(double x) { return fx_log_pos(x); }
+__device__ __forceinline__ double m_pow(double x, double y) { return fx_pow_pos(x, y); }
 __device__ __forceinline__ double m_sqrt(double x) { return fx_sqrt(x); }
 __device__ __forceinline__ double m_rcp(double x) { return fx_rcp(x); }
 __device__ __forceinline__ float m_rcp(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -27,14 +27,17 @@ int main(int argc, char** argv) {
   for (long i = 0; i < n; ++i) {
     const double xe = -700.0 + 1400.0 * u(g);
     e_exp = std::fmax(e_exp, ulp_err(mt::fx_exp(xe), std::exp((long double)xe)));
+    e_exp = std::fmax(e_exp, ulp_err(mt::fx_exp_fin(xe), std::exp((long double)xe)));
     const double xl = std::pow(10.0, -300.0 + 600.0 * u(g));
     e_log = std::fmax(e_log, ulp_err(mt::fx_log(xl), std::log((long double)xl)));
+    e_log = std::fmax(e_log, ulp_err(mt::fx_log_pos(xl), std::log((long double)xl)));
     const double xl2 = 0.5 + 1.5 * u(g);  // around 1, where log is small
     e_log = std::fmax(e_log, ulp_err(mt::fx_log(xl2), std::log((long double)xl2)));
+    e_log = std::fmax(e_log, ulp_err(mt::fx_log_pos(xl2), std::log((long double)xl2)));
     // pow over the physics' ranges: x in [1e-8, 1e6], y in [-2, 2]
     const double xp = std::pow(10.0, -8.0 + 14.0 * u(g)), yp = -2.0 + 4.0 * u(g);
     const long double wp = std::pow((long double)xp, (long double)yp);
-    const double ep = ulp_err(mt::fx_pow(xp, yp), wp);
+    const double ep = std::fmax(ulp_err(mt::fx_pow(xp, yp), wp), ulp_err(mt::fx_pow_pos(xp, yp), wp));
     e_pow = std::fmax(e_pow, ep);
     const double t = std::fabs(yp * std::log(xp));
     e_pow_rel = std::fmax(e_pow_rel, ep / (t + 1.0));

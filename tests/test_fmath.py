@@ -7,7 +7,8 @@ tests/native/fmath_check.cpp, which measures the worst error against long
 double references on random arguments:
 
 * exp on [-700, 700], log on [1e-300, 1e300] and on [0.5, 2] (near 1, where
-  the result is small),
+  the result is small), for the general versions and the kernel's variants
+  without special-value handling (fx_exp_fin, fx_log_pos, fx_pow_pos),
 * pow for x in [1e-8, 1e6], y in [-2, 2], the ranges the physics uses
   (pmax's eeg^0.1, the Kinchin-Pease pows, the stopping fits),
 * rcp, rsqrt and sqrt on +-[1e-30, 1e30] from a single-precision seed (the
