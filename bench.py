@@ -124,7 +124,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=int, default=1000000)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--autotune-sample", type=int, default=10000, help="0 = default or given launch shape")
+    ap.add_argument("--autotune-sample", type=int, default=10000,
+                    help="least histories in the tuning sample (0 = default or given launch shape)")
     ap.add_argument("--block", type=int, default=0, help="fixed launch shape (with --autotune-sample 0)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--stack-cap", type=int, default=0)
@@ -157,10 +158,15 @@ def main():
         tuning = {"block": args.block, "blocks_per_cu": args.blocks_per_cu, "stack_cap": args.stack_cap,
                   "fixed": True}
     elif args.autotune_sample > 0:
-        best, rate = eng.autotune(prob.with_(hist0=2 * args.histories, n_hist=args.autotune_sample),
-                                  sample=args.autotune_sample)
+        # the tuning sample holds at least --autotune-sample histories and about
+        # 1e9 collisions, at most one timed step: small cascades need more
+        # histories to reach the queue occupancy of a full launch
+        pilot = eng.run(prob.with_(hist0=3 * args.histories, n_hist=2000))
+        per_hist = max(pilot["n_coll"] / 2000.0, 1.0)
+        n_sample = int(min(max(args.autotune_sample, 1e9 / per_hist), max(args.histories // max(args.steps, 1), 1)))
+        best, rate = eng.autotune(prob.with_(hist0=2 * args.histories, n_hist=n_sample), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
-                  "sample_histories": args.autotune_sample, "sample_rate": rate}
+                  "sample_histories": n_sample, "sample_rate": rate}
     wsize = max(1, min(20000, args.histories // max(args.steps, 1)))
     for i in range(args.warmup):
         eng.run(prob.with_(hist0=args.histories + i * wsize, n_hist=wsize))
@@ -172,7 +178,9 @@ def main():
         c64, w64, k64, r64 = timed(eng, prob, gpu.MT_FP64, nv, 1)
         # each precision runs at its own autotuned launch shape (the FP32
         # kernels need fewer registers, so more waves fit)
-        sample = prob.with_(hist0=2 * args.histories, n_hist=max(args.autotune_sample, 2000))
+        sample = prob.with_(hist0=2 * args.histories,
+                            n_hist=tuning["sample_histories"] if tuning and "sample_histories" in tuning
+                            else max(args.autotune_sample, 2000))
         t32 = _tuned(eng, sample, gpu.MT_FP32, args.autotune_sample)
         c32, w32, k32, r32 = timed(eng, prob, gpu.MT_FP32, nv, 1)
         sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)

@@ -169,7 +169,7 @@ struct mt_ctx {
   HostTarget tgt;
   bool has_target = false;
   mt_tuning tune{};
-  DBuf tables, sh, cl, cx, cy, cz, cr;
+  DBuf tables, sh, cl, cx, cy, cz, cr, occ;
   DBuf pka_sp, pka_E, pka_pos, pka_dir;
   DBuf res, wave_energy, hist_all, h_out;
   HBuf stage_res, stage_hist;
@@ -190,7 +190,7 @@ struct mt_ctx {
 
   ~mt_ctx() {
     graph.release();
-    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
+    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &occ, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
                    &wave_energy, &hist_all, &h_out, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e,
                    &log_rec, &log_map};
     for (DBuf* b : all) b->release();
@@ -393,6 +393,59 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
       DBuf* dst[4] = {&ctx->cx, &ctx->cy, &ctx->cz, &ctx->cr};
       for (int i = 0; i < 4; ++i)
         HIPCHECK(hipMemcpy(dst[i]->p, v[i].data(), v[i].size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // occupancy grid for the lookup's early exit: cells of about half the
+    // largest radius (at most 1024 per axis, 2^30 in all); a cell's bit is set
+    // when a sphere, radius padded by 1e-9 of the sample size, meets the cell
+    // (at its periodic images along PBC axes)
+    {
+      const double cs = std::max(0.5 * g.cmr, 1e-9 * std::max(g.w[0], std::max(g.w[1], g.w[2])));
+      long long on[3];
+      for (int i = 0; i < 3; ++i) on[i] = std::min(1024ll, std::max(1ll, (long long)std::ceil(g.w[i] / cs)));
+      while (on[0] * on[1] * on[2] > (1ll << 30)) {
+        int m = 0;
+        for (int i = 1; i < 3; ++i)
+          if (on[i] > on[m]) m = i;
+        on[m] = (on[m] + 1) / 2;
+      }
+      for (int i = 0; i < 3; ++i) g.on[i] = int(on[i]);
+      const long long nbits = on[0] * on[1] * on[2];
+      std::vector<uint32_t> bits(size_t((nbits + 31) / 32), 0u);
+      const double pad = 1e-9 * std::max(g.w[0], std::max(g.w[1], g.w[2]));
+      for (const HostCluster& c : tg.clusters) {
+        const double cc[3] = {c.x, c.y, c.z}, rr = c.r + pad;
+        long long lo[3], hi[3];
+        for (int i = 0; i < 3; ++i) {
+          lo[i] = (long long)std::floor((cc[i] - rr) * double(on[i]) / g.w[i]);
+          hi[i] = (long long)std::floor((cc[i] + rr) * double(on[i]) / g.w[i]);
+          if (g.bc[i] != MT_PBC) {
+            lo[i] = std::max(lo[i], 0ll);
+            hi[i] = std::min(hi[i], on[i] - 1);
+          } else if (hi[i] - lo[i] + 1 > on[i]) {  // spans the axis
+            lo[i] = 0;
+            hi[i] = on[i] - 1;
+          }
+        }
+        for (long long j0 = lo[0]; j0 <= hi[0]; ++j0)
+          for (long long j1 = lo[1]; j1 <= hi[1]; ++j1)
+            for (long long j2 = lo[2]; j2 <= hi[2]; ++j2) {
+              const long long j[3] = {j0, j1, j2};
+              double d2 = 0.0;
+              long long b = 0, mul = 1;
+              for (int i = 0; i < 3; ++i) {
+                const double a = double(j[i]) * g.w[i] / double(on[i]);
+                const double e = double(j[i] + 1) * g.w[i] / double(on[i]);
+                const double d = std::max(std::max(a - cc[i], cc[i] - e), 0.0);
+                d2 += d * d;
+                b += (((j[i] % on[i]) + on[i]) % on[i]) * mul;
+                mul *= on[i];
+              }
+              if (d2 <= rr * rr) bits[size_t(b >> 5)] |= 1u << (b & 31);
+            }
+      }
+      HIPCHECK(ctx->occ.ensure(bits.size() * sizeof(uint32_t)));
+      HIPCHECK(hipMemcpy(ctx->occ.p, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      g.occ = ctx->occ.as<uint32_t>();
     }
     g.sh = ctx->sh.as<int>();
     g.cl = ctx->cl.as<int>();
@@ -810,9 +863,9 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // (fewer registers; a kernel whose registers do not allow the requested
   // blocks per CU runs at the most that fit); queue capacity stays at its
   // default
-  static const int kCand[][3] = {{256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {256, 2, 96},  {256, 2, 64},
-                                 {128, 4, 128}, {128, 4, 96},  {64, 8, 128},  {64, 8, 96},   {256, 1, 256},
-                                 {128, 3, 256}, {64, 6, 192},  {256, 3, 128}, {256, 3, 96},  {128, 6, 96}};
+  static const int kCand[][3] = {{256, 2, 256}, {256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {256, 2, 96},
+                                 {256, 2, 64},  {128, 4, 128}, {128, 4, 96},  {64, 8, 128},  {256, 1, 384},
+                                 {256, 1, 256}, {128, 3, 256}, {256, 3, 128}, {256, 3, 96},  {128, 6, 96}};
   std::vector<mt_tuning> cand;
   for (const auto& c : kCand) {
     mt_tuning t{};

@@ -92,6 +92,12 @@ struct Geometry {
   const double* cy;
   const double* cz;
   const double* cr;
+  // occupancy bits of a grid of on[0] x on[1] x on[2] cells over the sample:
+  // a cell's bit is set when some cluster sphere (radius padded by
+  // 1e-9 of the sample size) meets it, so a position in a clear cell is in
+  // no cluster and needs no hash scan
+  int on[3];
+  const uint32_t* occ;
 };
 
 }  // namespace mt
