@@ -184,10 +184,12 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
   if (nn >= mc.n_el) nn = mc.n_el - 1;
   const ElemConst& ec = T.el[eb + nn];
   if constexpr (ARITH == 0) {
-    // free path and maximum impact parameter (trim.C:88-95)
+    // free path and maximum impact parameter (trim.C:88-95); log E serves
+    // eeg^0.1, the stopping fits and the apsis' starting point
+    const double lE = m_log(E);
     double eps = E * mc.f;
     const double eeg = m_sqrt(eps * mc.epsdg);
-    const double pmax = mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125 * m_pow(eeg, 0.1));
+    const double pmax = mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125 * m_exp(0.05 * (lE + mc.ln_f_epsdg)));
     ls = m_rcp(M_PI * (pmax * pmax) * mc.arho);
     if (ic == 1) {
       const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
@@ -198,13 +200,14 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     eps = ec.fi * E;
     const double b = p * m_rcp(ec.ai);
     // electronic loss along ls (trim.C:166-170) and scattering (trim.C:172-272)
-    dee = ls * getrstop<double>(T, sp, k, E);
-    scatter<double, POT>(eps, b, s2, ct, st, it);
+    dee = ls * getrstop<double>(T, sp, k, E, lE);
+    scatter<double, POT>(eps, b, lE + ec.ln_fi, s2, ct, st, it);
   } else {
     const float Ef = (float)E;
+    const float lEf = m_log(Ef);
     float eps = Ef * (float)mc.f;
     const float eeg = m_sqrt(eps * (float)mc.epsdg);
-    const float pmax = (float)mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125f * m_pow(eeg, 0.1f));
+    const float pmax = (float)mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125f * m_exp(0.05f * (lEf + (float)mc.ln_f_epsdg)));
     float lsf = m_rcp(3.14159265358979f * (pmax * pmax) * (float)mc.arho);
     if (ic == 1) {
       const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
@@ -214,7 +217,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     eps = (float)ec.fi * Ef;
     const float b = p * m_rcp((float)ec.ai);
     ls = (double)lsf;
-    dee = ls * (double)getrstop<float>(T, sp, k, Ef);
+    dee = ls * (double)getrstop<float>(T, sp, k, Ef, lEf);
     float s2f = -1.0f, ctf, stf;
     if constexpr (ARITH == 2) {
       if (eps <= 10.0f) s2f = mlp_s2(mlp, eps, b);  // the closed form is cheaper above 10
@@ -224,7 +227,7 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
       stf = 2.0f * m_sqrt(fmaxf(0.0f, s2f * (1.0f - s2f)));
       it = -1;
     } else {
-      scatter_f32<POT>(eps, b, s2f, ctf, stf, it);
+      scatter_f32<POT>(eps, b, lEf + (float)ec.ln_fi, s2f, ctf, stf, it);
     }
     s2 = s2f;
     ct = ctf;

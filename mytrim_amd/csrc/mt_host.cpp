@@ -100,6 +100,7 @@ static StopConst make_stop(const std::vector<CoefRow>& coef, int z1, double m_io
   c.inv_d1 = c.d1 > 0.0 ? 1.0 / c.d1 : 0.0;
   c.inv_d2 = c.d2 > 0.0 ? 1.0 / c.d2 : 0.0;
   c.ln_eee = std::log(c.eee);
+  c.ln_emul = std::log(0.001 * c.inv_m1);
   return c;
 }
 
@@ -167,6 +168,7 @@ std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt
       mc.f = f;
       mc.epsdg = epsdg;
       mc.a = a;
+      mc.ln_f_epsdg = std::log(f * epsdg);
       mc.arho = mat.arho;
       mc.az = mat.az;
       mc.am = mat.am;
@@ -180,6 +182,7 @@ std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt
         ec.ec = 4.0 * ec.my / ((1.0 + ec.my) * (1.0 + ec.my));
         ec.ai = .5292 * .8853 / (fZ023 + std::pow(double(he.Z), 0.23));
         ec.fi = ec.ai * he.m / (fZ * double(he.Z) * 14.4 * (m + he.m));
+        ec.ln_fi = std::log(ec.fi);
         ec.Edisp = he.Edisp;
         ec.Elbind = he.Elbind;
         ec.Z = he.Z;

@@ -60,28 +60,30 @@ __device__ __forceinline__ float m_abs(float x) { return fabsf(x); }
 // The stopping fits below keep the reference's formulas; divisions by
 // per-(ion, element) constants become multiplications by their reciprocals
 // (StopConst.inv_*), powers share one logarithm of their base, and
-// (e / eee)^power = exp(power (log e - log eee)) reuses log e.
+// (e / eee)^power = exp(power (log e - log eee)) reuses log e.  The caller
+// passes log E (shared with the free path): log e = log E + log(0.001 / m1).
 template <typename R>
-__device__ __forceinline__ R rpstop(const StopConst& c, R e) {  // material.C:134-158
+__device__ __forceinline__ R rpstop(const StopConst& c, R e, R le) {  // material.C:134-158
   const R pe = mx(K(25.0), e);
-  const R lpe = m_log(pe);
+  const R lpe = (e > K(25.0)) ? le : K(3.2188758248682007492);  // log 25
   const R sl = (R(c.pc[0]) * m_exp(R(c.pc[1]) * lpe)) + (R(c.pc[2]) * m_exp(R(c.pc[3]) * lpe));
   const R sh = R(c.pc[4]) * m_exp(-R(c.pc[5]) * lpe) * m_log(R(c.pc[6]) * m_rcp(pe) + R(c.pc[7]) * pe);
   R sp = sl * sh * m_rcp(sl + sh);
-  if (e <= K(25.0)) sp *= m_pow(e * K(0.04), R(c.velpwr));
+  if (e <= K(25.0)) sp *= m_exp(R(c.velpwr) * (le - K(3.2188758248682007492)));  // (e / 25)^velpwr
   return sp;
 }
 
 template <typename R>
-__device__ __forceinline__ R rstop(const StopConst& c, R E) {  // material.C:161-282
+__device__ __forceinline__ R rstop(const StopConst& c, R E, R lE) {  // material.C:161-282
   const R e = (K(0.001) * E) * R(c.inv_m1);
+  const R le = lE + R(c.ln_emul);
   R se;
   if (c.kind == 1) {
-    se = rpstop(c, e);
+    se = rpstop(c, e, le);
   } else if (c.kind == 2) {
     const R he0 = K(1.0);
     const R he = mx(he0, e);
-    const R b = m_log(he);
+    const R b = (e > he0) ? le : K(0.0);  // log he
     const R b2 = b * b;
     const R b4 = b2 * b2;
     R a = K(0.2865) + K(0.1266) * b - K(0.001429) * b2 + K(0.02402) * b * b2 - K(0.01135) * b4 +
@@ -89,7 +91,7 @@ __device__ __forceinline__ R rstop(const StopConst& c, R E) {  // material.C:161
     R heh = K(1.0) - m_exp(-mn(K(30.0), a));
     a = K(1.0) + R(c.he_k) * m_exp(-sq(K(7.6) - b));
     heh *= a * a;
-    se = rpstop(c, he) * heh * K(4.0);
+    se = rpstop(c, he, b) * heh * K(4.0);
     if (e <= he0) se *= m_sqrt(e);
   } else {
     const R vf = R(c.vfermi);
@@ -117,27 +119,31 @@ __device__ __forceinline__ R rstop(const StopConst& c, R E) {  // material.C:161
       l1 = R(c.bq) * (K(1.0) - q) * R(c.inv_d2);
     const R l = mx(l1, l0 * R(c.lfctr));
     R zeta = q + R(c.zk) * (K(1.0) - q) * m_log(K(1.0) + sq(l * vf * K(4.0 / 1.919)));
-    const R le = m_log(e);
     a = -sq(K(7.6) - mx(K(0.0), le));
     zeta *= K(1.0) + R(c.z3) * m_exp(a);
     if (yr <= R(c.yr_branch))
       se = R(c.sp_low) * sq(zeta * R(c.fz1)) * m_exp(R(c.power) * (le - R(c.ln_eee)));
     else
-      se = rpstop(c, e) * sq(zeta * R(c.fz1));
+      se = rpstop(c, e, le) * sq(zeta * R(c.fz1));
   }
   return se * K(10.0);
 }
 
 // MaterialBase::getrstop for species s in material k [eV/A]
 template <typename R>
-__device__ __forceinline__ R getrstop(const TableView& T, int s, int k, R E) {
+__device__ __forceinline__ R getrstop(const TableView& T, int s, int k, R E, R lE) {
   const MatConst& mc = T.mat[s * T.n_mat + k];
   R se = K(0.0);
   for (int i = 0; i < mc.n_el; ++i) {
     const int e = s * T.n_el + mc.el_begin + i;
-    se += rstop(T.stop[e], E) * R(T.el[e].t);
+    se += rstop(T.stop[e], E, lE) * R(T.el[e].t);
   }
   return se * R(mc.arho);
+}
+
+template <typename R>
+__device__ __forceinline__ R getrstop(const TableView& T, int s, int k, R E) {
+  return getrstop<R>(T, s, k, E, m_log(E));
 }
 
 // Screening sums of the interatomic potential V(r) = S(r) / r with
@@ -197,30 +203,31 @@ __device__ __forceinline__ int apsis(R eps, R b, R& r, R& v, R& v1) {
 
 // MAGIC coefficients (trim.C:236-262): cc, aa = 2 eps (1 + c/sqrt(eps)) b^cc
 // and ff for the three screenings; 1/sqrt(eps) comes from one reciprocal
-// square root and the quotients from reciprocals
+// square root, the quotients from reciprocals, b^cc = exp(cc log b) from the
+// log b of the apsis' starting point
 template <typename R, int POT>
-__device__ __forceinline__ void magic(R eps, R b, R& aa, R& ff) {
+__device__ __forceinline__ void magic(R eps, R lb, R& aa, R& ff) {  // lb = log b
   const R ise = m_rsqrt(eps);
   const R sqe = eps * ise;
   R cc;
   if (POT == 1) {
     cc = (K(0.009611) + sqe) * m_rcp(K(0.005175) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + K(0.6743) * ise) * m_pow(b, cc);
+    aa = K(2.0) * eps * (K(1.0) + K(0.6743) * ise) * m_exp(cc * lb);
     ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6.314) + eps) * m_rcp(K(10.0) + eps));
   } else if (POT == 2) {
     cc = (K(0.235809) + sqe) * m_rcp(K(0.126000) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + K(1.0144) * ise) * m_pow(b, cc);
+    aa = K(2.0) * eps * (K(1.0) + K(1.0144) * ise) * m_exp(cc * lb);
     ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6935.0) + eps) * m_rcp(K(83550.0) + eps));
   } else {
     cc = (K(0.011615) + sqe) * m_rcp(K(0.0071222) + sqe);
-    aa = K(2.0) * eps * (K(1.0) + K(0.99229) * ise) * m_pow(b, cc);
+    aa = K(2.0) * eps * (K(1.0) + K(0.99229) * ise) * m_exp(cc * lb);
     ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(9.3066) + eps) * m_rcp(K(14.813) + eps));
   }
 }
 
 // trim.C:172-272
 template <typename R, int POT>
-__device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& iters) {
+__device__ __forceinline__ void scatter(R eps, R b, R leps, R& s2, R& ct, R& st, int& iters) {
   if (eps > K(10.0)) {
     s2 = m_rcp(K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
     const R c2 = K(1.0) - s2;
@@ -230,7 +237,8 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
     return;
   }
   R r = b;
-  R rr = K(-2.7) * m_log(eps * b);
+  const R lb = m_log(b);
+  R rr = K(-2.7) * (leps + lb);  // log(eps b)
   if (rr >= b) {
     rr = K(-2.7) * m_log(eps * rr);
     if (rr >= b) r = rr;
@@ -239,7 +247,7 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
   const int it = apsis<R, POT>(eps, b, r, v, v1);
   const R roc = K(-2.0) * (eps - v) * m_rcp(v1);
   R aa, ff;
-  magic<R, POT>(eps, b, aa, ff);
+  magic<R, POT>(eps, lb, aa, ff);
   const R delta = (r - b) * aa * ff * m_rcp(ff + K(1.0));
   const R co = (b + delta + roc) * m_rcp(r + roc);
   const R c2 = co * co;
@@ -260,7 +268,7 @@ __device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& it
 // s2 = 1 - co^2 but keeps full relative precision when s2 << 1 (glancing
 // collisions transfer s2 ~ tmin / (ec E), below single-precision epsilon).
 template <typename R, int POT>
-__device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, int& iters) {
+__device__ __forceinline__ void scatter_stable(R eps, R b, R leps, R& s2, R& ct, R& st, int& iters) {
   if (eps > K(10.0)) {
     s2 = m_rcp(K(1.0) + (K(1.0) + b * (K(1.0) + b)) * sq(K(2.0) * eps * b));
     ct = K(1.0) - K(2.0) * s2;
@@ -269,7 +277,8 @@ __device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, 
     return;
   }
   R r = b;
-  R rr = K(-2.7) * m_log(eps * b);
+  const R lb = m_log(b);
+  R rr = K(-2.7) * (leps + lb);  // log(eps b)
   if (rr >= b) {
     rr = K(-2.7) * m_log(eps * rr);
     if (rr >= b) r = rr;
@@ -278,7 +287,7 @@ __device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, 
   const int it = apsis<R, POT>(eps, b, r, v, v1);
   const R roc = K(-2.0) * (eps - v) * m_rcp(v1);
   R aa, ff;
-  magic<R, POT>(eps, b, aa, ff);
+  magic<R, POT>(eps, lb, aa, ff);
   R Sr, Dr;
   screening<R, POT>(r, Sr, Dr);
   const R rmb = Sr * r / (eps * (r + b));  // v(r) r^2 = S(r) r
@@ -290,9 +299,20 @@ __device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, 
   iters = it;
 }
 
+template <typename R, int POT>
+__device__ __forceinline__ void scatter(R eps, R b, R& s2, R& ct, R& st, int& iters) {
+  scatter<R, POT>(eps, b, m_log(eps), s2, ct, st, iters);
+}
+
+template <typename R, int POT>
+__device__ __forceinline__ void scatter_stable(R eps, R b, R& s2, R& ct, R& st, int& iters) {
+  scatter_stable<R, POT>(eps, b, m_log(eps), s2, ct, st, iters);
+}
+
 template <int POT>
-__device__ __forceinline__ void scatter_f32(float eps, float b, float& s2, float& ct, float& st, int& iters) {
-  scatter_stable<float, POT>(eps, b, s2, ct, st, iters);
+__device__ __forceinline__ void scatter_f32(float eps, float b, float leps, float& s2, float& ct, float& st,
+                                            int& iters) {
+  scatter_stable<float, POT>(eps, b, leps, s2, ct, st, iters);
 }
 
 #undef K

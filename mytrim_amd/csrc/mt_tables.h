@@ -19,6 +19,7 @@ constexpr int kMaxLayers = 16;
 
 struct MatConst {   // per (species, material)
   double f, epsdg, a, arho;
+  double ln_f_epsdg;  // log(f epsdg): log eeg = (log E + ln_f_epsdg) / 2
   double az, am;    // prepare() averages, for Kinchin-Pease
   int el_begin, n_el;
 };
@@ -26,6 +27,7 @@ struct MatConst {   // per (species, material)
 struct ElemConst {  // per (species, element)
   double t;         // normalised stoichiometric fraction
   double fi, ai, ec, my;
+  double ln_fi;     // log fi: log eps = log E + ln_fi
   double Edisp, Elbind;
   int Z;
   int recoil_species;
@@ -51,6 +53,7 @@ struct StopConst {  // per (species, element): rstop(ion, Z2)
   // multiplies where the reference divides
   double inv_m1, inv_vfermi, inv_cbrt_fz1, inv_cbrt2_fz1, inv_d1, inv_d2;
   double ln_eee;                // log(eee)
+  double ln_emul;               // log(0.001 / m1): log e = log E + ln_emul
   int kind;                     // 1 proton, 2 helium, 3 heavy ion
   int pad;
 };
