@@ -101,6 +101,25 @@ static StopConst make_stop(const std::vector<CoefRow>& coef, int z1, double m_io
   c.inv_d2 = c.d2 > 0.0 ? 1.0 / c.d2 : 0.0;
   c.ln_eee = std::log(c.eee);
   c.ln_emul = std::log(0.001 * c.inv_m1);
+  {  // material.C:231-257 at yr = yr_branch, where the low-velocity branch has it
+    const double yr = c.yr_branch;
+    const double yr03 = std::pow(yr, 0.3);
+    const double a = -0.803 * yr03 + 1.3167 * yr03 * yr03 + 0.38157 * yr + 0.008983 * yr * yr;
+    const double q = std::min(1.0, std::max(0.0, 1.0 - std::exp(-std::min(a, 50.0))));
+    const double l0 = (0.8 - q * std::min(1.2, 0.6 + fz1 / 30.0)) / c.cbrt_fz1;
+    double l1;
+    if (q < 0.2)
+      l1 = 0.0;
+    else if (q < std::max(0.0, 0.9 - 0.025 * fz1))
+      l1 = c.bq * (q - 0.2) / std::abs(std::max(0.0, 0.9 - 0.025 * fz1) - 0.2000001);
+    else if (q < std::max(0.0, 1.0 - 0.025 * std::min(16.0, fz1)))
+      l1 = c.bq;
+    else
+      l1 = c.bq * (1.0 - q) / (0.025 * std::min(16.0, fz1));
+    const double l = std::max(l1, l0 * c.lfctr);
+    const double s = 4.0 * l * vfermi / 1.919;
+    c.zeta_lo = q + (1.0 / (2.0 * vfermi * vfermi)) * (1.0 - q) * std::log(1.0 + s * s);
+  }
   return c;
 }
 

@@ -104,24 +104,33 @@ __device__ __forceinline__ R rstop(const StopConst& c, R E, R lE) {  // material
       vr = R(c.vk) * (K(1.0) + v2 * K(2.0 / 3.0) - v2 * v2 * K(1.0 / 15.0));
     R yr = mx(K(0.13), vr * R(c.inv_cbrt2_fz1));
     yr = mx(yr, R(c.yr_lo));
-    const R yr03 = m_exp(K(0.3) * m_log(yr));
-    R a = K(-0.803) * yr03 + K(1.3167) * yr03 * yr03 + K(0.38157) * yr + K(0.008983) * yr * yr;
-    const R q = mn(K(1.0), mx(K(0.0), K(1.0) - m_exp(-mn(a, K(50.0)))));
-    const R l0 = (K(0.8) - q * R(c.l0k)) * R(c.inv_cbrt_fz1);
-    R l1;
-    if (q < K(0.2))
-      l1 = K(0.0);
-    else if (q < R(c.q1))
-      l1 = R(c.bq) * (q - K(0.2)) * R(c.inv_d1);
-    else if (q < R(c.q2))
-      l1 = R(c.bq);
-    else
-      l1 = R(c.bq) * (K(1.0) - q) * R(c.inv_d2);
-    const R l = mx(l1, l0 * R(c.lfctr));
-    R zeta = q + R(c.zk) * (K(1.0) - q) * m_log(K(1.0) + sq(l * vf * K(4.0 / 1.919)));
-    a = -sq(K(7.6) - mx(K(0.0), le));
+    // yr is at least yr_branch = max(yrmin, vrmin / cbrt2 fz1); the
+    // low-velocity branch (yr <= yr_branch) has yr = yr_branch, where q, l
+    // and the effective-charge factor are per-(ion, element) constants
+    const bool low = yr <= R(c.yr_branch);
+    R zeta;
+    if (low) {
+      zeta = R(c.zeta_lo);
+    } else {
+      const R yr03 = m_exp(K(0.3) * m_log(yr));
+      const R a = K(-0.803) * yr03 + K(1.3167) * yr03 * yr03 + K(0.38157) * yr + K(0.008983) * yr * yr;
+      const R q = mn(K(1.0), mx(K(0.0), K(1.0) - m_exp(-mn(a, K(50.0)))));
+      const R l0 = (K(0.8) - q * R(c.l0k)) * R(c.inv_cbrt_fz1);
+      R l1;
+      if (q < K(0.2))
+        l1 = K(0.0);
+      else if (q < R(c.q1))
+        l1 = R(c.bq) * (q - K(0.2)) * R(c.inv_d1);
+      else if (q < R(c.q2))
+        l1 = R(c.bq);
+      else
+        l1 = R(c.bq) * (K(1.0) - q) * R(c.inv_d2);
+      const R l = mx(l1, l0 * R(c.lfctr));
+      zeta = q + R(c.zk) * (K(1.0) - q) * m_log(K(1.0) + sq(l * vf * K(4.0 / 1.919)));
+    }
+    const R a = -sq(K(7.6) - mx(K(0.0), le));
     zeta *= K(1.0) + R(c.z3) * m_exp(a);
-    if (yr <= R(c.yr_branch))
+    if (low)
       se = R(c.sp_low) * sq(zeta * R(c.fz1)) * m_exp(R(c.power) * (le - R(c.ln_eee)));
     else
       se = rpstop(c, e, le) * sq(zeta * R(c.fz1));

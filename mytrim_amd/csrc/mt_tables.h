@@ -54,6 +54,8 @@ struct StopConst {  // per (species, element): rstop(ion, Z2)
   double inv_m1, inv_vfermi, inv_cbrt_fz1, inv_cbrt2_fz1, inv_d1, inv_d2;
   double ln_eee;                // log(eee)
   double ln_emul;               // log(0.001 / m1): log e = log E + ln_emul
+  double zeta_lo;               // heavy ions, low-velocity branch (yr = yr_branch): the
+                                // effective-charge factor before its z1^3 term
   int kind;                     // 1 proton, 2 helium, 3 heavy ion
   int pad;
 };
