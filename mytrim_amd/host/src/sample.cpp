@@ -5,11 +5,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <iostream>
 
 #include "functions.h"
 #include "sample.h"
 #include "sample_clusters.h"
 #include "sample_burried_wire.h"
+#include "sample_dynamic.h"
 #include "sample_layers.h"
 #include "sample_wire.h"
 
@@ -50,6 +52,56 @@ MaterialBase* SampleBurriedWire::lookupMaterial(Point& pos) {  // sample_burried
   if (pos(2) > w[2] || pos(2) < -250.0) return nullptr;      // above the sample or below the cover
   MaterialBase* m = SampleWire::lookupMaterial(pos);
   return m ? m : material[1];
+}
+
+SampleDynamic::SampleDynamic(SimconfType* simconf, Real x, Real y, Real z)  // sample_dynamic.C:28-34
+    : SampleLayers(x, y, z), pka(nullptr), _simconf(simconf) {
+  bc[0] = CUT;
+  bc[1] = PBC;
+  bc[2] = PBC;
+}
+
+void SampleDynamic::averages(const IonBase* p) {  // sample_dynamic.C:36-46: averages on demand
+  pka = p;
+  for (MaterialBase* m : material) m->_dirty = true;
+}
+
+MaterialBase* SampleDynamic::lookupMaterial(Point& pos) {  // sample_dynamic.C:48-62
+  MaterialBase* m = material[lookupLayer(pos)];
+  if (m->_dirty && pka) m->average(pka);
+  return m;
+}
+
+void SampleDynamic::addAtomsToLayer(int layer, int n, int Z) {  // sample_dynamic.C:64-119
+  MaterialBase* mat = material[layer];
+  std::vector<Element>& el = mat->_element;
+  // the element's index in the layer, and the volume of one atom of the
+  // layer's composition at natural density (weighted 1/atrho)
+  unsigned ne = unsigned(el.size());
+  Real rnorm = 0.0;
+  for (unsigned i = 0; i < el.size(); ++i) {
+    if (el[i]._Z == Z) ne = i;
+    rnorm += el[i]._t / _simconf->scoef[el[i]._Z - 1].atrho;
+  }
+  (void)rnorm;  // computed, not used, as in the reference
+  if (ne == el.size()) {  // a new element of the layer
+    Element e;
+    e._Z = Z;
+    e._m = _simconf->scoef[Z - 1].m1;
+    e._t = 0.0;
+    el.push_back(e);
+  }
+  const int nl = int(mat->_arho * w[1] * w[2] * layerThickness[layer]);  // atoms in the layer
+  const Real ma = 0.6022 * el[ne]._m * Real(n);                         // mass change
+  layerThickness[layer] += ma / (mat->_rho * w[1] * w[2]);               // density kept
+  if (el[ne]._t * nl + Real(n) < 0.0) {
+    std::cout << "Crap, t*nl=" << el[ne]._t * nl << ", but n=" << n << std::endl;
+    el[ne]._t = 0.0;
+  } else {
+    el[ne]._t += Real(n) / nl;  // prepare() renormalises the stoichiometry
+  }
+  mat->_dirty = true;
+  mat->prepare();
 }
 
 int SampleLayers::lookupLayer(Point& pos) {  // sample_layers.C:26-43

@@ -14,6 +14,8 @@
 //       -> the stream of TrimDefectLog / TrimPhononOut (17 digits), the
 //          positions TrimHistory stores, or TrimVacMap's map; "H h" before
 //          each history
+//   ref_harness dynamic  -> SampleDynamic::addAtomsToLayer on a fixed sequence
+//          (layer thicknesses and compositions after each step)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -34,6 +36,7 @@
 #include "sample_layers.h"
 #include "sample_burried_wire.h"
 #include "sample_wire.h"
+#include "sample_dynamic.h"
 #include "sample_solid.h"
 #include "simconf.h"
 #include "trim.h"
@@ -386,6 +389,43 @@ int cmd_trace(const Spec& s, long long h, int max_rows) {
 
 }  // namespace
 
+// SampleDynamic::addAtomsToLayer (sample_dynamic.C:64-119) on a fixed sequence
+// of additions and removals; the same sequence runs through the C++ API in
+// tests/native/dynamic_check.cpp
+int cmd_dynamic() {
+  SimconfType sc;
+  SampleDynamic smp(&sc, 1000.0, 100.0, 100.0);
+  MaterialBase* si = new MaterialBase(&sc, 2.33);
+  Element e;
+  e._Z = 14;
+  e._m = 28.086;
+  e._t = 1.0;
+  si->_element.push_back(e);
+  si->prepare();
+  MaterialBase* ox = new MaterialBase(&sc, 2.2);
+  ox->_element.push_back(e);
+  e._Z = 8;
+  e._m = 15.999;
+  e._t = 2.0;
+  ox->_element.push_back(e);
+  ox->prepare();
+  smp.material.push_back(si);
+  smp.material.push_back(ox);
+  smp.layerThickness.push_back(500.0);
+  smp.layerThickness.push_back(10000.0);
+  const int steps[][3] = {{0, 1000, 5}, {0, -10, 14}, {1, 500, 5}, {0, 2000, 32}, {1, -100000000, 8}, {1, 7, 8}};
+  for (const auto& st : steps) {
+    smp.addAtomsToLayer(st[0], st[1], st[2]);
+    printf("T %.17g %.17g\n", smp.layerThickness[0], smp.layerThickness[1]);
+    for (MaterialBase* m : smp.material) {
+      printf("M %.17g %.17g %.17g %.17g", m->_rho, m->_arho, m->_am, m->_az);
+      for (const Element& x : m->_element) printf(" %d %.17g %.17g", x._Z, x._m, x._t);
+      printf("\n");
+    }
+  }
+  return 0;
+}
+
 // the stream-writing and recording TrimBase subclasses of trim.h:139-227
 int cmd_log(const Spec& s, const std::string& mode, int z1, int z2, int z3) {
   SimconfType sc;
@@ -487,6 +527,7 @@ int main(int argc, char** argv) {
   }
   if (cmd == "trace") return cmd_trace(read_spec(argv[2]), atoll(argv[3]), atoi(argv[4]));
   if (cmd == "tally") return cmd_tally(read_spec(argv[2]));
+  if (cmd == "dynamic") return cmd_dynamic();
   if (cmd == "log")
     return cmd_log(read_spec(argv[2]), argv[3], argc > 4 ? atoi(argv[4]) : -1, argc > 5 ? atoi(argv[5]) : -1,
                    argc > 6 ? atoi(argv[6]) : -1);

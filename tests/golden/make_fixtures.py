@@ -319,6 +319,20 @@ def runstopping_fixtures():
         print(name, len(out.splitlines()), "lines", flush=True)
 
 
+def dynamic():
+    """SampleDynamic: the layer updates of addAtomsToLayer (ref_harness
+    dynamic) and the drain loop of tests/native/dynamic_trim.cpp built against
+    the reference (oracle/_ref/dynamic_trim), 6 Xe primaries."""
+    out = subprocess.run([HARNESS, "dynamic"], capture_output=True, text=True, check=True).stdout
+    with open(os.path.join(HERE, "dynamic_layers.txt"), "w") as f:
+        f.write(out)
+    drv = os.path.join(os.path.dirname(HARNESS), "dynamic_trim")
+    out = subprocess.run([drv, "6"], capture_output=True, text=True, check=True).stdout
+    with open(os.path.join(HERE, "dynamic_trim.txt"), "w") as f:
+        f.write(out)
+    print("dynamic", len(out.splitlines()), "lines", flush=True)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -332,6 +346,9 @@ def main():
     if "--logs" in sys.argv:
         logs()
         return
+    if "--dynamic" in sys.argv:
+        dynamic()
+        return
     if only or only_tr:  # regenerate the named tally / trace fixtures only
         if only_tr:
             traces(only=only_tr)
@@ -342,6 +359,7 @@ def main():
     average_table()
     traces()
     logs()
+    dynamic()
     tallies(quick)
 
 

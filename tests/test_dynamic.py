@@ -1,0 +1,68 @@
+"""SampleDynamic (sample_dynamic.h of MyTRIM): a layered sample whose
+composition changes as atoms leave and join its layers.
+
+CPU: SampleDynamic::addAtomsToLayer and averages() of the C++ API against the
+reference's output for the same sequence of additions and removals
+(tests/golden/dynamic_layers.txt, written by the reference-built harness).
+Integer and floating-point state is printed with %.17g, so the comparison is
+byte for byte.
+
+GPU: the drain loop of apps/mytrim_dynamic.C (tests/native/dynamic_trim.cpp:
+each flight's atom leaves the layer it starts in and joins the layer it stops
+in) through the C++ drop-in API, every TrimBase::trim() flight on the GPU,
+against the same driver built on the reference library
+(tests/golden/dynamic_trim.txt): the same flights in the same order and the
+same final layers.  Positions are compared with the absolute tolerance of the
+reference's tests/csv_diff.sh (1e-4); the flight count, generations, species
+and end states exactly."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "mytrim_amd", "host", "_build")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _exe(name):
+    p = os.path.join(BUILD, name)
+    if not os.path.exists(p):
+        pytest.fail(name + " not built: run __graft_entry__.build()")
+    return p
+
+
+def test_layer_updates_match_reference():
+    out = subprocess.run([_exe("dynamic_check")], capture_output=True, text=True, timeout=60, check=True).stdout
+    with open(os.path.join(GOLD, "dynamic_layers.txt")) as f:
+        gold = f.read()
+    assert out == gold
+
+
+def _parse(text):
+    flights, layers = [], []
+    for line in text.splitlines():
+        tok = line.split()
+        if tok[0] == "F":
+            flights.append([float(t) for t in tok[1:]])
+        elif tok[0] == "L":
+            layers.append([float(t) for t in tok[1:]])
+    return np.array(flights), layers
+
+
+@pytest.mark.gpu
+def test_drain_loop_through_gpu_flights():
+    r = subprocess.run([_exe("dynamic_trim"), "6"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    mine, mine_layers = _parse(r.stdout)
+    with open(os.path.join(GOLD, "dynamic_trim.txt")) as f:
+        gold, gold_layers = _parse(f.read())
+    assert mine.shape == gold.shape
+    # generation, Z, end state: exact; final position: csv_diff tolerance
+    assert np.array_equal(mine[:, [0, 1, 5]], gold[:, [0, 1, 5]])
+    assert np.max(np.abs(mine[:, 2:5] - gold[:, 2:5])) <= 1e-4
+    assert len(mine_layers) == len(gold_layers)
+    for a, b in zip(mine_layers, gold_layers):
+        assert len(a) == len(b)
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
