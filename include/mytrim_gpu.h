@@ -185,8 +185,14 @@ typedef struct {
  * the flight key (unique per flight; the reference numbers ions in creation
  * order from a global counter).  Records of one history are in trim.C order;
  * histories interleave.  Recoils in kinds 6 and 7 carry id 0 and tag -1, as
- * they have not been queued when the reference prints them. */
-enum { MT_LOG_DEFECTS = 1, MT_LOG_HISTORY = 2, MT_LOG_PHONONS = 4, MT_LOG_VACMAP = 8 };
+ * they have not been queued when the reference prints them.
+ * MT_LOG_TRANSPORT (layers geometry only) counts the atom moves a
+ * SampleDynamic drain loop applies with addAtomsToLayer
+ * (apps/mytrim_dynamic.C:160-178), the layers held fixed during the run: per
+ * layer k and element Z, +1 for every flight that ends at x > 0, in the layer
+ * it ends in, and -1 for every recoil flight (gen > 0), in the layer it starts
+ * in. */
+enum { MT_LOG_DEFECTS = 1, MT_LOG_HISTORY = 2, MT_LOG_PHONONS = 4, MT_LOG_VACMAP = 8, MT_LOG_TRANSPORT = 16 };
 typedef struct {
   double pos[3];
   double E, value, m;
@@ -202,6 +208,8 @@ typedef struct {
   int vacmap_Z[3];          /* TrimVacMap z1 z2 z3 (trim.h:176-201) */
   int* vacmap;              /* caller-allocated int[20][20][3] (x cell, y cell, constituent):
                                the run's counts are added (MT_LOG_VACMAP) */
+  long long* transport;     /* caller-allocated long long[n_mat][93], indexed [layer][Z]:
+                               the run's net moves are added (MT_LOG_TRANSPORT) */
 } mt_log;
 
 /* Scattering surrogate (no counterpart in the reference): an MLP

@@ -186,13 +186,13 @@ struct mt_ctx {
   bool mlp_ready = false;
   int mlp_bf16 = 0;
   DBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
-  DBuf log_rec, log_map;  // event records and TrimVacMap map
+  DBuf log_rec, log_map, log_tr;  // event records, TrimVacMap map, layer transport
 
   ~mt_ctx() {
     graph.release();
     DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &occ, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
                    &wave_energy, &hist_all, &h_out, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e,
-                   &log_rec, &log_map};
+                   &log_rec, &log_map, &log_tr};
     for (DBuf* b : all) b->release();
     stage_res.release();
     stage_hist.release();
@@ -480,8 +480,13 @@ int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, m
 
 int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_log* lg, mt_tallies* out) {
   if (!lg) return fail(MT_E_ARG, "NULL log");
-  if (lg->mask & ~(MT_LOG_DEFECTS | MT_LOG_HISTORY | MT_LOG_PHONONS | MT_LOG_VACMAP))
+  if (lg->mask & ~(MT_LOG_DEFECTS | MT_LOG_HISTORY | MT_LOG_PHONONS | MT_LOG_VACMAP | MT_LOG_TRANSPORT))
     return fail(MT_E_ARG, "unknown log bits");
+  if (lg->mask & MT_LOG_TRANSPORT) {
+    if (!lg->transport) return fail(MT_E_ARG, "transport buffer missing");
+    if (ctx && ctx->has_target && ctx->tgt.geo.kind != MT_GEO_LAYERS)
+      return fail(MT_E_ARG, "layer transport needs a layers geometry");
+  }
   if (lg->max_records < 0 || (lg->max_records > 0 && !lg->records))
     return fail(MT_E_ARG, "records buffer missing");
   if ((lg->mask & MT_LOG_VACMAP) && !lg->vacmap) return fail(MT_E_ARG, "vacmap buffer missing");
@@ -704,6 +709,12 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
       P.log.vacmap = ctx->log_map.as<int>();
       span(ctx->log_map.p, sizeof(int) * kVacMapN * kVacMapN * 3);
     }
+    if (lg->mask & MT_LOG_TRANSPORT) {
+      const size_t nb = sizeof(long long) * size_t(ctx->tgt.geo.n_mat) * kTransportZ;
+      HIPCHECK(ctx->log_tr.ensure(nb));
+      P.log.transport = ctx->log_tr.as<unsigned long long>();
+      span(ctx->log_tr.p, nb);
+    }
   }
   if (o->arith == MT_FP32_SURROGATE) {
     P.mlp.p = ctx->mlp_p.as<float>();
@@ -820,6 +831,11 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
       std::vector<int> m(kVacMapN * kVacMapN * 3);
       HIPCHECK(hipMemcpy(m.data(), ctx->log_map.p, m.size() * sizeof(int), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < m.size(); ++i) lg->vacmap[i] += m[i];
+    }
+    if (lg->mask & MT_LOG_TRANSPORT) {
+      std::vector<long long> t(size_t(ctx->tgt.geo.n_mat) * kTransportZ);
+      HIPCHECK(hipMemcpy(t.data(), ctx->log_tr.p, t.size() * sizeof(long long), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < t.size(); ++i) lg->transport[i] += t[i];
     }
   }
   if (perhist && o->per_history) {

@@ -73,7 +73,8 @@ struct LogRecord {
   uint32_t hist;
   int Z, gen, tag, kind, pad;
 };
-constexpr int kLogDefects = 1, kLogHistory = 2, kLogPhonons = 4, kLogVacmap = 8;
+constexpr int kLogDefects = 1, kLogHistory = 2, kLogPhonons = 4, kLogVacmap = 8, kLogTransport = 16;
+constexpr int kTransportZ = 93;  // transport columns per layer: Z = 0..92
 constexpr int kVacMapN = 20;  // TrimVacMap cells per axis (trim.h:178)
 struct LogDev {
   int mask;                // kLog* bits; 0: no records
@@ -81,6 +82,7 @@ struct LogDev {
   unsigned long long cap;
   int vz[3];               // TrimVacMap z1 z2 z3
   int* vacmap;             // [20][20][3]
+  unsigned long long* transport;  // [layer][kTransportZ], two's-complement counts
 };
 
 struct Params {

@@ -536,6 +536,15 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       int mtag;
       const double pos[3] = {F.px, F.py, F.pz};
       const int k = lookup_material<GEO>(G, pos, mtag);
+      if constexpr (PERHIST && GEO == 1) {
+        // layer transport: a recoil's atom leaves the layer its flight starts
+        // in (apps/mytrim_dynamic.C:160-164)
+        if (F.ic == 1 && F.gen > 0) {
+          const ArgPtr A = cold_args();
+          if (A->log.mask & kLogTransport)
+            atomicAdd(&A->log.transport[k * kTransportZ + H->species[F.sp].Z], ~0ull);
+        }
+      }
       if (k < 0) {
         vacuum = true;
         ended = true;
@@ -884,6 +893,16 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           atomicAdd(&A->h_coll[hl], (unsigned long long)(vacuum ? F.ic - 1 : F.ic));
           atomicAdd(&A->h_flights[hl], 1);
           atomicAdd(&A->h_Eel[hl], F.eel);
+          if constexpr (GEO == 1) {
+            // layer transport: the atom joins the layer its flight ends in
+            // when it stopped at x > 0 (apps/mytrim_dynamic.C:174-178)
+            if ((A->log.mask & kLogTransport) && F.px > 0.0) {
+              int t;
+              const double pos[3] = {F.px, F.py, F.pz};
+              const int k = lookup_material<GEO>(G, pos, t);
+              atomicAdd(&A->log.transport[k * kTransportZ + H->species[F.sp].Z], 1ull);
+            }
+          }
         }
         if (F.gen == 0) {
           const ArgPtr A = cold_args();

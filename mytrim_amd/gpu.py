@@ -23,7 +23,7 @@ ip = C.POINTER(C.c_int)
 lp = C.POINTER(C.c_longlong)
 
 MT_FP64, MT_FP32, MT_FP32_SURROGATE = 0, 1, 2
-MT_LOG_DEFECTS, MT_LOG_HISTORY, MT_LOG_PHONONS, MT_LOG_VACMAP = 1, 2, 4, 8
+MT_LOG_DEFECTS, MT_LOG_HISTORY, MT_LOG_PHONONS, MT_LOG_VACMAP, MT_LOG_TRANSPORT = 1, 2, 4, 8, 16
 # one event record (mt_record); kinds: 1 V, 2 I, 3 S, 4 R (TrimDefectLog),
 # 5 TrimHistory, 6 / 7 / 8 TrimPhononOut (spawn, dissipation, ion at rest)
 RECORD = np.dtype([("pos", "<f8", 3), ("E", "<f8"), ("value", "<f8"), ("m", "<f8"), ("id", "<u8"),
@@ -69,7 +69,8 @@ ENERGIES = ["E0", "Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac"]
 
 class LogSpec(C.Structure):
     _fields_ = [("mask", C.c_int), ("records", C.c_void_p), ("max_records", C.c_longlong),
-                ("n_records", C.c_longlong), ("vacmap_Z", C.c_int * 3), ("vacmap", ip)]
+                ("n_records", C.c_longlong), ("vacmap_Z", C.c_int * 3), ("vacmap", ip),
+                ("transport", lp)]
 
 
 class Tallies(C.Structure):
@@ -246,7 +247,8 @@ class Engine:
     def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64, log=None) -> dict:
         """Full cascades for every history of `prob` (sets the target if needed).
         `log` (optional): dict(mask=MT_LOG_*, max_records=n, vacmap_Z=(z1, z2, z3));
-        the result then holds "records" (RECORD array), "n_records" and "vacmap"."""
+        the result then holds "records" (RECORD array), "n_records", "vacmap" and
+        "transport" (n_mat x 93 net atom moves per layer and Z, MT_LOG_TRANSPORT)."""
         pk, o, keep = self._launch_args(prob, per_history, arith)
         out = Tallies()
         hists = {"range_hist": np.zeros(prob.range_bins.nb + 2, np.int64),
@@ -278,11 +280,14 @@ class Engine:
                 lg.vacmap_Z[i] = int(vz[i])
             vmap = np.zeros((20, 20, 3), np.int32)
             lg.vacmap = _p(vmap, ip)
+            trans = np.zeros((len(prob.materials), 93), np.int64)
+            lg.transport = _p(trans, lp)
             _check(lib().mt_run_cascades_log(self.h, C.byref(pk), C.byref(o), C.byref(lg), C.byref(out)),
                    "mt_run_cascades_log")
         res = {k: getattr(out, k) for k in COUNTERS + ENERGIES}
         if log is not None:
-            res.update(records=recs[:min(lg.n_records, len(recs))], n_records=lg.n_records, vacmap=vmap)
+            res.update(records=recs[:min(lg.n_records, len(recs))], n_records=lg.n_records, vacmap=vmap,
+                       transport=trans)
         res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
                    kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
                    queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate)

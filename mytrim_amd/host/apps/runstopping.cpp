@@ -27,7 +27,7 @@ using namespace MyTRIM_NS;
     return 1;                                            \
   } while (0)
 
-int main(int argc, char* argv[]) {
+int main(int argc, char* /*argv*/[]) {
   if (argc > 1) mytrimError("Please supply the input file via stdin (e.g. ./runstopping < input.json`)");
   json_lite::Value root;
   try {

@@ -14,6 +14,7 @@
 #include <iosfwd>
 #include <vector>
 
+#include "sample_dynamic.h"
 #include "trim.h"
 
 namespace MyTRIM_NS {
@@ -37,7 +38,9 @@ class TrimGPU : public TrimBase {
     enum Arith { FP64 = 0, FP32 = 1, FP32_SURROGATE = 2 } arith = FP64;
     // event streams (the records TrimDefectLog, TrimHistory and
     // TrimPhononOut write, trim.C:466-527): bits DEFECTS 1, HISTORY 2,
-    // PHONONS 4, VACMAP 8; at most max_records records are kept
+    // PHONONS 4, VACMAP 8; at most max_records records are kept.  Bit
+    // TRANSPORT 16 (layers samples): the atom moves of a SampleDynamic drain
+    // loop (apps/mytrim_dynamic.C:160-178) with the layers held fixed
     int log_mask = 0;
     long long max_records = 1 << 20;
     int vacmap_Z[3] = {-1, -1, -1};  // TrimVacMap z1 z2 z3
@@ -68,7 +71,15 @@ class TrimGPU : public TrimBase {
     std::vector<Record> records;  // log_mask != 0
     long long n_records = 0;      // produced (records past max_records are dropped)
     std::vector<int> vacmap;      // [20][20][3] (LOG VACMAP)
+    // [layer][Z] net atoms of element Z that came to rest in the layer (+1
+    // per flight ending at x > 0) minus those whose recoil flight started
+    // there (-1 per flight with gen > 0) (LOG TRANSPORT; Z = 0..92)
+    std::vector<long long> transport;
   };
+
+  // the composition update between batches: addAtomsToLayer(layer, n, Z) for
+  // every non-zero entry of t.transport, layer by layer and Z by Z
+  static void applyTransport(SampleDynamic& sample, const Tallies& t);
 
   // the lines TrimDefectLog (kinds 1-4) or TrimPhononOut (kinds 6-8) would
   // print for `records` (IonBase's operator<<, ion.C:74-79), in record order

@@ -1,6 +1,7 @@
 // TrimGPU::runCascades: the batched device path (mt_run_cascades).
 #include "trim_gpu.h"
 
+#include <climits>
 #include <ostream>
 #include <stdexcept>
 
@@ -54,7 +55,7 @@ Launch launch_args(long long n, bool uniform, const std::vector<int>& Z, const s
 TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<int>& Z,
                      const std::vector<double>& m, const std::vector<double>& E, const std::vector<double>& pos,
                      const std::vector<double>& dir, double Ef, uint32_t seed, long long hist0,
-                     const TrimGPU::Options& o) {
+                     const TrimGPU::Options& o, size_t n_mat) {
   Launch L = launch_args(n, uniform, Z, m, E, pos, dir, Ef, seed, hist0, o);
   TrimGPU::Tallies t;
   t.range_hist.assign(size_t(o.range_bins + 2), 0);
@@ -96,6 +97,10 @@ TrimGPU::Tallies run(mt_ctx* ctx, long long n, bool uniform, const std::vector<i
     for (int i = 0; i < 3; ++i) lg.vacmap_Z[i] = o.vacmap_Z[i];
     t.vacmap.assign(20 * 20 * 3, 0);
     lg.vacmap = t.vacmap.data();
+    if (o.log_mask & MT_LOG_TRANSPORT) {
+      t.transport.assign(n_mat * 93, 0);
+      lg.transport = t.transport.data();
+    }
     device::check(mt_run_cascades_log(ctx, &L.pk, &L.ro, &lg, &out), "mt_run_cascades_log");
     t.n_records = lg.n_records;
     t.records.resize(size_t(std::min<long long>(lg.n_records, lg.max_records)));
@@ -131,7 +136,7 @@ TrimGPU::Tallies TrimGPU::runCascades(const IonBase& p, long long n, uint32_t se
                                       long long hist0) {
   std::vector<int> Z{p._Z};
   std::vector<double> m{p._m}, E{p._E}, pos{p._pos(0), p._pos(1), p._pos(2)}, dir{p._dir(0), p._dir(1), p._dir(2)};
-  Tallies t = run(device(), n, true, Z, m, E, pos, dir, p._Ef, seed, hist0, o);
+  Tallies t = run(device(), n, true, Z, m, E, pos, dir, p._Ef, seed, hist0, o, _sample->material.size());
   _simconf->vacancies_created += int(t.vacancies);
   _simconf->EelTotal += t.Eel;
   _simconf->EnucTotal += t.Enuc;
@@ -153,11 +158,24 @@ TrimGPU::Tallies TrimGPU::runCascades(const std::vector<IonBase*>& pkas, uint32_
       dir.push_back(p->_dir(i));
     }
   }
-  Tallies t = run(device(), (long long)pkas.size(), false, Z, m, E, pos, dir, Ef, seed, hist0, o);
+  Tallies t = run(device(), (long long)pkas.size(), false, Z, m, E, pos, dir, Ef, seed, hist0, o,
+                  _sample->material.size());
   _simconf->vacancies_created += int(t.vacancies);
   _simconf->EelTotal += t.Eel;
   _simconf->EnucTotal += t.Enuc;
   return t;
+}
+
+void TrimGPU::applyTransport(SampleDynamic& sample, const Tallies& t) {
+  const size_t nl = sample.material.size();
+  if (t.transport.size() != nl * 93) throw std::runtime_error("applyTransport: no transport tally for this sample");
+  for (size_t k = 0; k < nl; ++k)
+    for (int Z = 1; Z <= 92; ++Z) {
+      const long long n = t.transport[k * 93 + size_t(Z)];
+      if (n == 0) continue;
+      if (n > INT_MAX || n < INT_MIN) throw std::runtime_error("applyTransport: count out of int range");
+      sample.addAtomsToLayer(int(k), int(n), Z);
+    }
 }
 
 void TrimGPU::writeRecords(std::ostream& os, const std::vector<Record>& records, bool phonons) {

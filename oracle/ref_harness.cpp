@@ -22,6 +22,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <queue>
 #include <sstream>
 #include <string>
@@ -447,11 +448,23 @@ int cmd_log(const Spec& s, const std::string& mode, int z1, int z2, int z3) {
     trim = th = new TrimHistory(&sc, smp);
   } else if (mode == "vacmap") {
     trim = tv = new TrimVacMap(&sc, smp, z1, z2, z3);
+  } else if (mode == "transport") {
+    trim = new TrimBase(&sc, smp);
   } else {
     fprintf(stderr, "unknown log mode %s\n", mode.c_str());
     return 2;
   }
   trim->_potential = TrimBase::Potential(s.potential);
+  // transport: the layer bookkeeping of apps/mytrim_dynamic.C:160-178 with the
+  // composition held fixed (per layer and Z: -1 where a recoil starts, +1
+  // where a flight ending at x > 0 stops)
+  SampleLayers* layers = dynamic_cast<SampleLayers*>(smp);
+  std::map<std::pair<int, int>, long long> moves;
+  const bool transport = mode == "transport";
+  if (transport && !layers) {
+    fprintf(stderr, "transport needs a layers sample\n");
+    return 2;
+  }
   size_t nh = 0;
   for (long long h = s.hist0; h < s.hist0 + s.nhist; ++h) {
     sc.seed(history_seed(s.seed, h));
@@ -463,7 +476,9 @@ int cmd_log(const Spec& s, const std::string& mode, int z1, int z2, int z3) {
       IonBase* ion = q.front();
       q.pop();
       smp->averages(ion);
+      if (transport && ion->_gen > 0) moves[{layers->lookupLayer(ion->_pos), ion->_Z}] -= 1;
       trim->trim(ion, q);
+      if (transport && ion->_pos(0) > 0) moves[{layers->lookupLayer(ion->_pos), ion->_Z}] += 1;
       delete ion;
     }
     printf("H %lld\n", h);
@@ -474,6 +489,7 @@ int cmd_log(const Spec& s, const std::string& mode, int z1, int z2, int z3) {
       for (; nh < v.size(); ++nh) printf("%.17g %.17g %.17g\n", v[nh](0), v[nh](1), v[nh](2));
     }
   }
+  for (const auto& m : moves) printf("T %d %d %lld\n", m.first.first, m.first.second, m.second);
   if (tv) {
     printf("M");
     for (int x = 0; x < 20; ++x)

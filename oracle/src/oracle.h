@@ -196,6 +196,10 @@ struct Hooks {
   virtual void dissipateRecoilEnergy() {}
   // not a reference hook: called once after the collision loop ends
   virtual void flightEnd(bool /*vacuum*/) {}
+  // not reference hooks: the drain loop calls these before and after each
+  // trim() (the places apps/mytrim_dynamic.C:160-178 updates SampleDynamic)
+  virtual void beforeTrim(const Ion&, Sample&) {}
+  virtual void afterTrim(const Ion&, Sample&) {}
 };
 
 // ---------------------------------------------------------------- draws

@@ -111,10 +111,11 @@ int orc_workcount(const orc_run_spec* spec, double* out);
 long long orc_scatter_inputs(const orc_run_spec* spec, double* rows, long long max_rows);
 /* event records (mask: 1 defects, 2 history, 4 phonons) and the vacancy map
  * (int[20][20][3], accumulated; NULL: none) of the spec's histories, run in
- * order on one thread; returns the number of records (those past
- * max_records are not stored) */
+ * order on one thread; mask 16: the layer transport (long long[n_mat][93],
+ * accumulated); returns the number of records (those past max_records are
+ * not stored) */
 long long orc_run_log(const orc_run_spec* spec, int mask, const int* vacmap_Z, orc_record* records,
-                      long long max_records, int* vacmap);
+                      long long max_records, int* vacmap, long long* transport);
 /* per-collision trace of history `h` (rows of 21 doubles, see runs.cpp) */
 int orc_trace(const orc_run_spec* spec, long long h, double* rows, int max_rows);
 

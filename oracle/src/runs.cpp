@@ -179,7 +179,10 @@ struct TallyHooks : Hooks {
 //   7    TrimPhononOut::dissipateRecoilEnergy: recoil E + Elbind, the recoil (trim.C:512-518)
 //   8    TrimPhononOut::checkPKAState: the ion's E, the ion at rest (trim.C:503-510)
 // and TrimVacMap's 20 x 20 map of vacancies per (x, y) cell for three recoil
-// Z (trim.C:479-501).  Records are written in the reference's order; the
+// Z (trim.C:479-501), and the layer transport of apps/mytrim_dynamic.C:160-178
+// (mask 16): per (layer, Z), +1 for every flight that ends at x > 0, in the
+// layer it ends in, and -1 for every recoil flight (gen > 0), in the layer it
+// starts in: the atom moves SampleDynamic::addAtomsToLayer applies.  Records are written in the reference's order; the
 // phonon records of a recoil carry the id and tag it has when they are
 // printed (not yet assigned: 0 here, uninitialised in the reference, and -1).
 struct LogHooks : TallyHooks {
@@ -188,9 +191,27 @@ struct LogHooks : TallyHooks {
   std::vector<orc_record>& rec;
   int* vacmap;
   long long hist = 0;
+  long long* transport = nullptr;  // [material][93]
   LogHooks(const orc_run_spec& spec, Partial& part, const orc_run_out* o, int m, const int* z,
            std::vector<orc_record>& r, int* vm)
       : TallyHooks(spec, part, o), mask(m), vz(z), rec(r), vacmap(vm) {}
+  // the layer index lookupLayer() returns (material index; -1: vacuum)
+  static int layer_of(Sample& smp, const double pos[3]) {
+    const Material* m = smp.lookupMaterial(pos);
+    for (size_t k = 0; k < smp.material.size(); ++k)
+      if (smp.material[k] == m) return int(k);
+    return -1;
+  }
+  void beforeTrim(const Ion& ion, Sample& smp) override {  // apps/mytrim_dynamic.C:160-164
+    if (!transport || ion.gen <= 0) return;
+    const int k = layer_of(smp, ion.pos);
+    if (k >= 0) transport[k * 93 + ion.Z] -= 1;
+  }
+  void afterTrim(const Ion& ion, Sample& smp) override {  // apps/mytrim_dynamic.C:174-178
+    if (!transport || !(ion.pos[0] > 0.0)) return;
+    const int k = layer_of(smp, ion.pos);
+    if (k >= 0) transport[k * 93 + ion.Z] += 1;
+  }
   void put(int kind, const Ion& i, double value, int id, int tag) {
     orc_record r{};
     for (int k = 0; k < 3; ++k) r.pos[k] = i.pos[k];
@@ -344,7 +365,9 @@ static void run_history(const orc_run_spec& s, long long h, Sample& smp, Draws& 
     Ion* ion = q.front();
     q.pop_front();
     smp.averages(*ion, cfg.tmin);
+    hk.beforeTrim(*ion, smp);
     trim(ion, q, smp, cfg, draws, hk);
+    hk.afterTrim(*ion, smp);
     delete ion;
   }
 }
@@ -596,7 +619,7 @@ struct TraceHooks : Hooks {
 }  // namespace
 
 long long orc_run_log(const orc_run_spec* sp, int mask, const int* vacmap_Z, orc_record* records,
-                      long long max_records, int* vacmap) {
+                      long long max_records, int* vacmap, long long* transport) {
   const orc_run_spec& s = *sp;
   Partial p;
   p.range_hist.assign(s.range_nb + 2, 0);
@@ -611,6 +634,7 @@ long long orc_run_log(const orc_run_spec* sp, int mask, const int* vacmap_Z, orc
   const int zero[3] = {0, 0, 0};
   LogHooks hk(s, p, &none, mask, vacmap_Z ? vacmap_Z : zero, rec, vacmap);
   for (int i = 0; i < 3; ++i) hk.sample_w[i] = s.w[i];
+  if (mask & 16) hk.transport = transport;
   MtRng rng;
   MtDraws mtd(rng);
   PhiloxDraws phd(s.seed);

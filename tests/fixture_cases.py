@@ -3,7 +3,8 @@ outputs from the compiled reference; the tests re-run the same inputs)."""
 import numpy as np
 
 from mytrim_amd import configs
-from mytrim_amd.problem import Primaries
+from mytrim_amd.problem import (CUT, GEO_LAYERS, PBC, Bins, Element, Geometry, Material, Primaries,
+                                Problem)
 
 # (Z, mass) of ions; mass 0 takes the table mass (material.C:256-259)
 IONS = [(1, 1.008), (2, 4.0026), (5, 10.81), (6, 12.011), (14, 28.086), (26, 55.845),
@@ -59,3 +60,25 @@ def tally_cases(quick=False):
         "wire": (configs.wire_b_si(int(2000 * s)), 8),
         "bwire": (configs.burried_wire_p_si(int(400 * s)), 8),
     }
+
+
+def xe_si_sio2(n_hist, seed=2344):
+    """30 keV Xe into 100 A Si over 400 A SiO2, 500 x 100 x 100 A: the layer
+    stack of a SampleDynamic run; many ions leave through either face."""
+    t = (100.0, 400.0)
+    w = (sum(t), 100.0, 100.0)
+    return Problem(
+        name="xe_si_sio2_30keV",
+        materials=[Material(2.33, [Element(14, 28.086, 1.0, 25.0, 3.0)]),
+                   Material(2.2, [Element(14, 28.086, 1.0, 25.0, 3.0), Element(8, 15.999, 2.0, 28.0, 3.0)])],
+        geometry=Geometry(GEO_LAYERS, w, (CUT, PBC, PBC), layers=t),
+        primaries=Primaries.single(54, 131.0, 3.0e4, (0.0, w[1] / 2, w[2] / 2)),
+        n_hist=n_hist, seed=seed,
+        range_bins=Bins(0.0, 10.0, 100), vac_bins=Bins(0.0, 10.0, 100),
+        lat_bw=10.0, lat_nb=40, lat_y0=w[1] / 2, lat_z0=w[2] / 2)
+
+
+def transport_cases():
+    """name -> problem for the layer-transport fixtures (SampleDynamic's atom
+    moves with the composition held fixed)"""
+    return {"c3": configs.c3_ag_layers(24), "xe_si_sio2": xe_si_sio2(200)}

@@ -40,7 +40,7 @@ import oracle_lib as O  # noqa: E402
 
 HARNESS = O.REF_HARNESS
 
-from fixture_cases import ENERGIES, IONS, TARGETS, log_cases, tally_cases, trace_cases  # noqa: E402
+from fixture_cases import ENERGIES, IONS, TARGETS, log_cases, tally_cases, trace_cases, transport_cases  # noqa: E402
 
 
 def run(args, stdin=None):
@@ -319,6 +319,22 @@ def runstopping_fixtures():
         print(name, len(out.splitlines()), "lines", flush=True)
 
 
+def transport():
+    """Layer transport (ref_harness log ... transport: the bookkeeping of
+    apps/mytrim_dynamic.C:160-178 with fixed layers) of transport_cases()."""
+    res = {}
+    for name, prob in transport_cases().items():
+        path = spec_file(prob)
+        try:
+            out = run(["log", path, "transport"])
+        finally:
+            os.unlink(path)
+        res[name] = [[int(v) for v in line.split()[1:]] for line in out.splitlines() if line.startswith("T ")]
+        print(name, "transport", res[name], flush=True)
+    with open(os.path.join(HERE, "transport.json"), "w") as f:
+        json.dump(res, f, indent=0)
+
+
 def dynamic():
     """SampleDynamic: the layer updates of addAtomsToLayer (ref_harness
     dynamic) and the drain loop of tests/native/dynamic_trim.cpp built against
@@ -348,6 +364,7 @@ def main():
         return
     if "--dynamic" in sys.argv:
         dynamic()
+        transport()
         return
     if only or only_tr:  # regenerate the named tally / trace fixtures only
         if only_tr:
@@ -360,6 +377,7 @@ def main():
     traces()
     logs()
     dynamic()
+    transport()
     tallies(quick)
 
 

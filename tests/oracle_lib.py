@@ -85,7 +85,7 @@ def lib():
         L.orc_workcount.argtypes = [C.POINTER(RunSpec), dp]
         L.orc_trace.argtypes = [C.POINTER(RunSpec), C.c_longlong, dp, C.c_int]
         L.orc_run_log.restype = C.c_longlong
-        L.orc_run_log.argtypes = [C.POINTER(RunSpec), C.c_int, ip, C.c_void_p, C.c_longlong, ip]
+        L.orc_run_log.argtypes = [C.POINTER(RunSpec), C.c_int, ip, C.c_void_p, C.c_longlong, ip, lp]
         _lib = L
     return _lib
 
@@ -200,18 +200,22 @@ RECORD = np.dtype([("pos", "<f8", 3), ("E", "<f8"), ("value", "<f8"), ("m", "<f8
 assert RECORD.itemsize == 80
 
 
-def run_log(prob, mask: int, vacmap_Z=None, rng: int = 0, max_records: int = 1 << 20):
+def run_log(prob, mask: int, vacmap_Z=None, rng: int = 0, max_records: int = 1 << 20, transport=False):
     """Event records (mask 1 defects, 2 history, 4 phonons) of `prob`'s histories
-    in the reference's order, and the 20 x 20 x 3 vacancy map when vacmap_Z is given."""
+    in the reference's order, and the 20 x 20 x 3 vacancy map when vacmap_Z is
+    given; with transport, also the layer transport (n_mat x 93, mask 16)."""
     L = lib()
     s, keep = make_spec(prob, rng, 1)
     rec = np.zeros(max_records, RECORD)
     vz = np.array(list(vacmap_Z) if vacmap_Z is not None else [0, 0, 0], np.int32)
     vm = np.zeros((20, 20, 3), np.int32) if vacmap_Z is not None else None
-    n = L.orc_run_log(C.byref(s), mask, _p(vz, ip), rec.ctypes.data_as(C.c_void_p), max_records,
-                      _p(vm, ip) if vm is not None else None)
+    tr = np.zeros((s.n_mat, 93), np.int64) if transport else None
+    n = L.orc_run_log(C.byref(s), mask | (16 if transport else 0), _p(vz, ip), rec.ctypes.data_as(C.c_void_p),
+                      max_records, _p(vm, ip) if vm is not None else None, _p(tr, lp) if tr is not None else None)
     if n > max_records:
         raise RuntimeError("record buffer too small: %d records" % n)
+    if transport:
+        return rec[:n], vm, tr
     return rec[:n], vm
 
 

@@ -15,6 +15,7 @@ against the same driver built on the reference library
 same final layers.  Positions are compared with the absolute tolerance of the
 reference's tests/csv_diff.sh (1e-4); the flight count, generations, species
 and end states exactly."""
+import dataclasses
 import os
 import subprocess
 
@@ -66,3 +67,57 @@ def test_drain_loop_through_gpu_flights():
     for a, b in zip(mine_layers, gold_layers):
         assert len(a) == len(b)
         assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+def _parse_batches(text):
+    """-> list of (layers, transport, (collisions, vacancies)) per batch, and the final layers;
+    layers: [(thickness, rho, [(Z, m, t, Edisp, Elbind)])]"""
+    batches, layers, trans = [], [], {}
+    for line in text.splitlines():
+        tok = line.split()
+        if tok[0] == "L":
+            if trans is None:
+                layers, trans = [], {}
+            n = int(tok[3])
+            els = [(int(tok[4 + 5 * i]), *[float(v) for v in tok[5 + 5 * i:9 + 5 * i]]) for i in range(n)]
+            layers.append((float(tok[1]), float(tok[2]), els))
+        elif tok[0] == "T":
+            trans[(int(tok[1]), int(tok[2]))] = int(tok[3])
+        elif tok[0] == "B":
+            batches.append((layers, trans, (int(tok[1]), int(tok[2]))))
+            trans = None
+    return batches, layers
+
+
+@pytest.mark.gpu
+def test_between_batch_updates_through_cpp_api():
+    """TrimGPU::runCascades with the transport tally on a SampleDynamic, then
+    TrimGPU::applyTransport and a second batch (tests/native/dynamic_batch.cpp):
+    each batch equals the Python engine's run of the same histories on the
+    layers the driver printed before it, and the update between them is
+    addAtomsToLayer applied to every non-zero entry."""
+    from fixture_cases import xe_si_sio2
+    from mytrim_amd import gpu
+    from mytrim_amd.problem import Element, Material
+    n = 200
+    r = subprocess.run([_exe("dynamic_batch"), str(n)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    batches, final = _parse_batches(r.stdout)
+    assert len(batches) == 2
+    eng = gpu.Engine(0)
+    base = xe_si_sio2(n)
+    try:
+        for b, (layers, trans, (coll, vac)) in enumerate(batches):
+            prob = base.with_(
+                materials=[Material(rho, [Element(*e) for e in els]) for _, rho, els in layers],
+                geometry=dataclasses.replace(base.geometry, layers=tuple(t for t, _, _ in layers)),
+                hist0=b * n)
+            g = eng.run(prob, log=dict(mask=gpu.MT_LOG_TRANSPORT))
+            mine = {(k, z): int(v) for (k, z), v in np.ndenumerate(g["transport"]) if v}
+            assert mine == trans
+            assert (g["n_coll"], g["vac"]) == (coll, vac)
+    finally:
+        eng.close()
+    # the first update: Xe came to rest, silicon moved; the layers changed
+    assert any(z == 54 for _, _, els in batches[1][0] for z, *_ in els)
+    assert batches[1][0] != batches[0][0] and final != batches[1][0]

@@ -413,6 +413,40 @@ def test_event_streams_match_oracle(eng, case):
     assert np.sum(d["kind"] == 1) == g["vac"]
 
 
+@pytest.mark.parametrize("case", ["c3", "xe_si_sio2"])
+def test_layer_transport_matches_oracle(eng, case):
+    """SampleDynamic's atom moves per (layer, Z) (MT_LOG_TRANSPORT) against the
+    oracle on the device draw scheme: equal when every history ran the same
+    flights (per-history collision and vacancy counts), within the moves of
+    the histories that diverged otherwise."""
+    from fixture_cases import transport_cases
+    prob = transport_cases()[case]
+    g = eng.run(prob, per_history=True, log=dict(mask=16))
+    _, _, tr = O.run_log(prob, 0, rng=1, transport=True)
+    o = O.run(prob, rng=1, nthreads=4, per_history=True)
+    same = (g["h_coll"] == o["h_coll"]) & (g["h_vac"] == o["h_vac"])
+    print("%s: %d of %d histories identical, %d moves" % (case, same.sum(), prob.n_hist, np.abs(tr).sum()))
+    assert same.mean() >= 0.95
+    assert g["transport"].shape == tr.shape
+    if same.all():
+        assert np.array_equal(g["transport"], tr)
+    else:
+        assert np.abs(g["transport"] - tr).sum() <= 0.05 * np.abs(tr).sum()
+    # the recoil flights leave their layers: one -1 per flight with gen > 0,
+    # none for the primaries
+    assert (tr < 0).any() and (tr > 0).any()
+    # no other tally changes with the transport tally on
+    plain = eng.run(prob)
+    for k in ("n_coll", "n_flight", "vac", "repl", "inter"):
+        assert plain[k] == g[k], k
+
+
+def test_layer_transport_needs_layers(eng):
+    prob = configs.c2_cu_cu(2)  # SampleSolid
+    with pytest.raises(RuntimeError, match="layers"):
+        eng.run(prob, log=dict(mask=16))
+
+
 def test_event_stream_capacity_and_counters(eng):
     """A too-small record buffer keeps the first records and reports the total."""
     prob = configs.vacancy_case("cu", 1e4, 8)

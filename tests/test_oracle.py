@@ -264,3 +264,22 @@ def test_event_logs_bitwise(name):
     assert np.array_equal(np.column_stack([h["hist"], np.zeros(len(h)), h["pos"]]), g["history"])
     assert np.array_equal(vm, g["vacmap"])
     assert vm.sum() > 0
+
+
+def _transport_entries(tr):
+    return sorted([k, z, int(tr[k, z])] for k, z in zip(*np.nonzero(tr)))
+
+
+@pytest.mark.parametrize("name", ["c3", "xe_si_sio2"])
+def test_layer_transport_matches_reference(name):
+    """SampleDynamic's atom moves (apps/mytrim_dynamic.C:160-178: -1 where a
+    recoil starts, +1 where a flight ending at x > 0 stops, per layer and Z)
+    with the layers held fixed: the oracle's tally equals the reference's
+    drain loop over the same histories."""
+    from fixture_cases import transport_cases
+    prob = transport_cases()[name]
+    with open(os.path.join(GOLD, "transport.json")) as f:
+        gold = json.load(f)[name]
+    _, _, tr = O.run_log(prob, 0, rng=0, transport=True)
+    assert _transport_entries(tr) == sorted(gold)
+    assert any(v < 0 for _, _, v in gold) and any(v > 0 for _, _, v in gold)
