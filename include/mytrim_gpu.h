@@ -88,7 +88,9 @@ typedef struct {
 } mt_target;
 
 /* A batch of primaries (generation 0).  With pka_uniform != 0 the arrays hold
- * one ion used for every history, else one entry per history. */
+ * one ion used for every history, else one entry per history.  Runs refuse
+ * (MT_E_ARG) an energy that is not positive and finite, a non-finite position
+ * and a direction that is zero or not finite (it need not be normalised). */
 typedef struct {
   long long n_hist;
   long long hist0;   /* index of the first history (selects its random streams) */

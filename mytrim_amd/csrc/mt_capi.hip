@@ -518,7 +518,13 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
   if (!msg.empty()) return fail(MT_E_LIMIT, msg);
   double E0 = 0.0;
   for (long long i = 0; i < n_entries; ++i) {
-    if (!(pk->E[i] > 0.0)) return fail(MT_E_ARG, "primary energy must be positive");
+    if (!(pk->E[i] > 0.0) || !std::isfinite(pk->E[i])) return fail(MT_E_ARG, "primary energy must be positive and finite");
+    const double* q = pk->pos + 3 * i;
+    const double* d = pk->dir + 3 * i;
+    if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]))
+      return fail(MT_E_ARG, "primary position must be finite");
+    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (!(dd > 0.0) || !std::isfinite(dd)) return fail(MT_E_ARG, "primary direction must be finite and non-zero");
     E0 += pk->E[i];
   }
   if (pk->pka_uniform) E0 = pk->E[0] * double(n_hist);

@@ -192,8 +192,8 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     const double pmax = mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125 * m_exp(0.05 * (lE + mc.ln_f_epsdg)));
     ls = m_rcp(M_PI * (pmax * pmax) * mc.arho);
     if (ic == 1) {
-      const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
-      ls = u53(b0.x, b0.y) * mn(ls, cw);
+      const uint64_t f = first_draw(((uint64_t)khi << 32) | klo, seed, hist);
+      ls = u53((uint32_t)(f >> 32), (uint32_t)f) * mn(ls, cw);
     }
     // impact parameter and target atom (trim.C:143-160)
     const double p = pmax * m_sqrt(u53(blk.x, blk.y));
@@ -210,8 +210,8 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     const float pmax = (float)mc.a * m_rcp(eeg + m_sqrt(eeg) + 0.125f * m_exp(0.05f * (lEf + (float)mc.ln_f_epsdg)));
     float lsf = m_rcp(3.14159265358979f * (pmax * pmax) * (float)mc.arho);
     if (ic == 1) {
-      const P4 b0 = philox10(0u, klo, khi, 0u, seed, hist);
-      lsf = u24f(b0.y) * mn(lsf, (float)cw);
+      const uint64_t f = first_draw(((uint64_t)khi << 32) | klo, seed, hist);
+      lsf = u24f((uint32_t)(f >> 32)) * mn(lsf, (float)cw);
     }
     const float p = pmax * m_sqrt(u24f(blk.x));
     eps = (float)ec.fi * Ef;

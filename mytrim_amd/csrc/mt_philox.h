@@ -9,7 +9,10 @@
 //   r2 = u53(w0, w1)  impact parameter     (trim.C:143)
 //   hh = u32(w2)      target element        (trim.C:147)
 //   az = u32(w3)      azimuth of the recoil (replaces trim.C:322-333)
-// and block ic = 0 gives r1 = u53(w0, w1) for the first free path (trim.C:71).
+// The first free path's r1 (trim.C:71) is u53 of a splitmix64 hash of
+// (flight key, history, seed) (first_draw), a quarter of a Philox block's
+// work: a wave evaluates it at almost every step, for its few lanes that
+// start a flight.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -56,6 +59,16 @@ __host__ __device__ __forceinline__ float u24f(uint32_t a) {
 __host__ __device__ __forceinline__ uint64_t child_key(uint64_t parent, uint32_t ic) {
   uint64_t z = parent ^ ((uint64_t)ic * 0x9E3779B97F4A7C15ull);
   z += 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// 64 random bits for the first draw of a flight (r1): splitmix64 finaliser
+// over the flight key mixed with (history, seed)
+__host__ __device__ __forceinline__ uint64_t first_draw(uint64_t key, uint32_t seed, uint32_t hist) {
+  uint64_t z = key ^ (((((uint64_t)hist) << 32) | seed) * 0xD1B54A32D192ED03ull);
+  z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);

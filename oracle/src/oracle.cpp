@@ -432,8 +432,8 @@ void MtDraws::perp(const double d[3], double out[3]) {  // trim.C:322-333
 }
 
 double PhiloxDraws::first(const Ion& ion) {
-  const Philox4 b = philox4x32_10(0u, uint32_t(ion.key), uint32_t(ion.key >> 32), 0u, seed, ion.hist);
-  return u53(b.v[0], b.v[1]);
+  const uint64_t f = first_draw(ion.key, seed, ion.hist);
+  return u53(uint32_t(f >> 32), uint32_t(f));
 }
 
 void PhiloxDraws::collision(const Ion& ion, int ic) {

@@ -100,6 +100,8 @@ void orc_average(int Z1, double m1, double tmin, double rho, int n_el, const int
 int orc_scatter(double eps, double b, int potential, double* out3);
 void orc_philox(const unsigned* ctr4, const unsigned* key2, unsigned* out4);
 unsigned long long orc_flight_key_child(unsigned long long parent, unsigned ic);
+/* 64 bits of a flight's first draw (r1) */
+unsigned long long orc_first_draw(unsigned long long key, unsigned seed, unsigned hist);
 void orc_mt_stream(unsigned seed, long long skip, long long n, double* out);
 unsigned orc_mt_history_seed(unsigned master, long long h);
 int orc_uo2(int seed, double r, double cbf, int nev, char* erec, int erec_len, char* clcoor,

@@ -69,4 +69,16 @@ inline uint64_t flight_key_child(uint64_t parent, uint32_t ic) {
   return z ^ (z >> 31);
 }
 
+// 64 random bits for a flight's first draw r1: the splitmix64 finaliser
+// (Steele, Lea and Flood, OOPSLA 2014; constants of Vigna's splitmix64.c)
+// over the flight key xor (history:seed) times an odd constant, after the
+// splitmix64 state increment
+inline uint64_t first_draw(uint64_t key, uint32_t seed, uint32_t hist) {
+  uint64_t z = key ^ ((((uint64_t)hist << 32) | seed) * 0xD1B54A32D192ED03ull);
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 }  // namespace orc

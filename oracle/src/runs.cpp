@@ -447,6 +447,10 @@ unsigned long long orc_flight_key_child(unsigned long long parent, unsigned ic) 
   return flight_key_child(parent, ic);
 }
 
+unsigned long long orc_first_draw(unsigned long long key, unsigned seed, unsigned hist) {
+  return first_draw(key, seed, hist);
+}
+
 static int copy_out(const std::string& s, char* buf, int len) {
   if (int(s.size()) + 1 > len) return -int(s.size()) - 1;
   std::memcpy(buf, s.c_str(), s.size() + 1);

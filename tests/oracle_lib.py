@@ -77,6 +77,8 @@ def lib():
         L.orc_philox.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
         L.orc_flight_key_child.restype = C.c_ulonglong
         L.orc_flight_key_child.argtypes = [C.c_ulonglong, C.c_uint]
+        L.orc_first_draw.restype = C.c_ulonglong
+        L.orc_first_draw.argtypes = [C.c_ulonglong, C.c_uint, C.c_uint]
         L.orc_uo2.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int] + [C.c_char_p, C.c_int] * 3
         L.orc_mt_stream.argtypes = [C.c_uint, C.c_longlong, C.c_longlong, dp]
         L.orc_mt_history_seed.restype = C.c_uint

@@ -275,6 +275,16 @@ def test_invalid_input_raises(eng):
         with pytest.raises(gpu.MtError):
             eng.tune(**kw)
     eng.tune()
+    # primaries the kernel cannot fly are refused before any launch
+    base = configs.c1_he_fe(4)
+    for E, pos, d, msg in ((float("inf"), (0, 5000, 5000), (1, 0, 0), "energy"),
+                           (-1.0, (0, 5000, 5000), (1, 0, 0), "energy"),
+                           (1e5, (0, float("nan"), 5000), (1, 0, 0), "position"),
+                           (1e5, (0, 5000, 5000), (0, 0, 0), "direction"),
+                           (1e5, (0, 5000, 5000), (float("inf"), 0, 0), "direction")):
+        with pytest.raises(gpu.MtError, match=msg):
+            eng.run(base.with_(primaries=Primaries.single(2, 4.0026, E, pos, d)))
+    assert eng.run(base)["n_coll"] > 0
 
 
 # ---------------------------------------------------------------- FP32 physics path

@@ -41,6 +41,21 @@ def test_philox_known_answers(ctr, key, expect):
     assert O.philox(ctr, key) == list(expect)
 
 
+def test_first_draw_is_splitmix64():
+    """With key, seed and history 0 the first draw is splitmix64's first
+    output from state 0 (published: 0xe220a8397b1dcdaf); with key = the
+    golden-ratio increment it is the second (0x6e789e6aa1b965f4)."""
+    f = O.lib().orc_first_draw
+    assert f(0, 0, 0) == 0xE220A8397B1DCDAF
+    assert f(0x9E3779B97F4A7C15, 0, 0) == 0x6E789E6AA1B965F4
+    # history and seed enter the hash: distinct for distinct (history, seed)
+    vals = {f(0, s, h) for s in range(50) for h in range(400)}
+    assert len(vals) == 50 * 400
+    # the top 53 bits, as r1 uses them, are close to uniform
+    u = np.array([(f(0, 2344, h) >> 11) / 2.0 ** 53 for h in range(20000)])
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.003
+
+
 def test_flight_keys_distinct():
     keys = {O.lib().orc_flight_key_child(0, ic) for ic in range(1, 20000)}
     keys |= {O.lib().orc_flight_key_child(k, 1) for k in list(keys)[:5000]}
