@@ -408,7 +408,10 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
           if (on[i] > on[m]) m = i;
         on[m] = (on[m] + 1) / 2;
       }
-      for (int i = 0; i < 3; ++i) g.on[i] = int(on[i]);
+      for (int i = 0; i < 3; ++i) {
+        g.on[i] = int(on[i]);
+        g.on_w[i] = double(on[i]) / g.w[i];
+      }
       const long long nbits = on[0] * on[1] * on[2];
       std::vector<uint32_t> bits(size_t((nbits + 31) / 32), 0u);
       const double pad = 1e-9 * std::max(g.w[0], std::max(g.w[1], g.w[2]));

@@ -22,23 +22,33 @@ namespace mt {
 __device__ __forceinline__ int lookup_cluster(const Geometry& g, const double pos[3]) {
   int k[3], k1[3], k2[3];
   int oc[3];
+  bool out[3];
   for (int i = 0; i < 3; ++i) {
-    k[i] = (int)floor((pos[i] * g.kn[i]) / g.w[i]);
     double pw = pos[i];
-    if (pos[i] < 0.0 || pos[i] >= g.w[i]) {
+    out[i] = pos[i] < 0.0 || pos[i] >= g.w[i];
+    if (out[i]) {
       if (g.bc[i] == 2) return -2;
       if (g.bc[i] == 1) return -1;
-      k[i] = k[i] % g.kn[i];
-      if (k[i] < 0) k[i] += g.kn[i];
       pw -= floor(pw / g.w[i]) * g.w[i];  // periodic image inside the sample
     }
-    oc[i] = min(max((int)floor((pw * g.on[i]) / g.w[i]), 0), g.on[i] - 1);
+    // occupancy cell by a multiplication with on/w: a position within
+    // rounding of a cell face is within the 1e-9 w padding of any sphere that
+    // contains it, so both cells at that face carry the sphere's bit
+    oc[i] = min(max((int)floor(pw * g.on_w[i]), 0), g.on[i] - 1);
   }
   // clear occupancy cell: no cluster contains the position (the scan below
   // would find none); this skips the (2 ks + 1)^3-cell scan for most positions
   {
     const uint64_t bit = (uint64_t)oc[0] + (uint64_t)g.on[0] * ((uint64_t)oc[1] + (uint64_t)g.on[1] * (uint64_t)oc[2]);
     if (((g.occ[bit >> 5] >> (bit & 31u)) & 1u) == 0u) return -1;
+  }
+  // hash cell as the reference computes it (sample_clusters.C:59-81)
+  for (int i = 0; i < 3; ++i) {
+    k[i] = (int)floor((pos[i] * g.kn[i]) / g.w[i]);
+    if (out[i]) {
+      k[i] = k[i] % g.kn[i];
+      if (k[i] < 0) k[i] += g.kn[i];
+    }
   }
   for (int i = 0; i < 3; ++i) {
     const int ks = (int)(g.cmr / g.kd[i]) + 1;

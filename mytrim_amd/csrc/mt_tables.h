@@ -102,6 +102,7 @@ struct Geometry {
   // 1e-9 of the sample size) meets it, so a position in a clear cell is in
   // no cluster and needs no hash scan
   int on[3];
+  double on_w[3];  // on[i] / w[i]
   const uint32_t* occ;
 };
 
