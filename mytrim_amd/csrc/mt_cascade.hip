@@ -130,12 +130,14 @@ __device__ __forceinline__ uint64_t sum_add(uint64_t h, uint64_t w) {
 }
 __device__ __forceinline__ uint32_t sum_end(uint64_t h) { return (uint32_t)(h >> 32); }
 
-// Kernel arguments used off the collision path are read from the argument
-// segment where they are used (scalar loads) rather than held in scalar
+// Kernel arguments are read from the argument segment where they are used
+// (scalar loads, which the scalar cache serves) rather than held in scalar
 // registers across the loop: the argument block (~150 dwords) does not fit
 // beside the loop state in the 102 scalar registers, and every use of a
-// spilled one costs a v_readlane.  The empty asm makes the pointer opaque,
-// so the loads stay at their uses instead of being hoisted out of the loop.
+// spilled one costs a v_readlane, a vector instruction.  This holds for the
+// rare paths and for the per-collision constants (seed, cw, tau, boundaries,
+// follow rule, bins).  The empty asm makes the pointer opaque, so the loads
+// stay at their uses instead of being hoisted out of the loop.
 typedef const __attribute__((address_space(4))) Params* ArgPtr;
 __device__ __forceinline__ ArgPtr cold_args() {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -556,11 +558,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           F.dz *= inv;
         }
         const uint32_t klo = (uint32_t)F.key, khi = (uint32_t)(F.key >> 32);
-        const P4 blk = philox10((uint32_t)F.ic, klo, khi, 0u, P.seed, F.hist);
+        const ArgPtr C0 = cold_args();
+        const uint32_t seed = C0->seed;
+        const P4 blk = philox10((uint32_t)F.ic, klo, khi, 0u, seed, F.hist);
         const MatConst& mc = T.mat[F.sp * T.n_mat + k];
         double ls, dee, s2, ct, st;
         int it, nn;
-        physics<ARITH, POT>(T, F.sp, k, F.E, F.ic, klo, khi, P.seed, F.hist, blk, P.cw, P.mlp, nn, ls, dee, s2, ct,
+        physics<ARITH, POT>(T, F.sp, k, F.E, F.ic, klo, khi, seed, F.hist, blk, C0->cw, P.mlp, nn, ls, dee, s2, ct,
                             st, it);
         const ElemConst& ec = T.el[F.sp * T.n_el + mc.el_begin + nn];
         ruth = (it == 0);
@@ -587,7 +591,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         rx = F.px;
         ry = F.py;
         rz = F.pz;
-        const double step = (ls - P.tau) * P.invscale;
+        const ArgPtr C = cold_args();
+        const double step = (ls - C->tau) * C->invscale;
         F.px += F.dx * step;
         F.py += F.dy * step;
         F.pz += F.dz * step;
@@ -636,9 +641,9 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         rdz = rdzr;
         // boundaries (trim.C:344-352)
         bool lost = false;
-        if (G.bc[0] == 2 && (F.px > G.w[0] || F.px < 0.0)) lost = true;
-        if (G.bc[1] == 2 && (F.py > G.w[1] || F.py < 0.0)) lost = true;
-        if (G.bc[2] == 2 && (F.pz > G.w[2] || F.pz < 0.0)) lost = true;
+        if (C->geo.bc[0] == 2 && (F.px > C->geo.w[0] || F.px < 0.0)) lost = true;
+        if (C->geo.bc[1] == 2 && (F.py > C->geo.w[1] || F.py < 0.0)) lost = true;
+        if (C->geo.bc[2] == 2 && (F.pz > C->geo.w[2] || F.pz < 0.0)) lost = true;
         if (PERHIST) {
           lg_rZ = ec.Z;
           lg_rm = H->species[ec.recoil_species].m;
@@ -652,7 +657,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           atomicAdd(&cold_args()->energy[eElost], F.E);
           atomicAdd(&cold_args()->energy[eEdrop], den);
         } else if (rE > ec.Edisp - ec.Elbind) {
-          const bool follow = P.follow_max_gen < 0 || F.gen + 1 < P.follow_max_gen;
+          const bool follow = C->follow_max_gen < 0 || F.gen + 1 < C->follow_max_gen;
           if (follow) {
             push = true;
             kadd(enuc_s, enuc_e, ec.Elbind);
@@ -667,13 +672,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           }
           if (F.E > ec.Edisp) {
             // vacancyCreation (trim.C:439-443, TrimVacCount.C:31-42, KP trim.C:445-464)
-            const int bn = bin_of(rx, P.vac_x0, P.vac_bw, P.vac_nb);
-            if (P.lds_hist)
+            const int bn = bin_of(rx, C->vac_x0, C->vac_bw, C->vac_nb);
+            if (C->lds_hist)
               atomicAdd(&lvac[bn], 1u);
             else
               atomicAdd(&cold_args()->vac_hist[bn], 1ull);
             int extra = 0;
-            if (P.kp && F.gen + 1 == P.follow_max_gen) {
+            if (C->kp && F.gen + 1 == C->follow_max_gen) {
               const double ed = 0.0115 * m_pow(mc.az, -7.0 / 3.0) * rE;
               const double g = 3.4008 * m_pow(ed, 1.0 / 6.0) + 0.40244 * m_pow(ed, 3.0 / 4.0) + ed;
               const double kd = 0.1337 * m_pow(mc.az, 2.0 / 3.0) / m_sqrt(mc.am);
@@ -684,8 +689,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
             if (PERHIST) atomicAdd(&cold_args()->h_vac[F.hist - (uint32_t)P.hist0], 1 + extra);
             state = -1;  // vacancy marker, ion keeps moving
           } else {
-            const int bn = bin_of(rx, P.vac_x0, P.vac_bw, P.vac_nb);
-            if (P.lds_hist)
+            const int bn = bin_of(rx, C->vac_x0, C->vac_bw, C->vac_nb);
+            if (C->lds_hist)
               atomicAdd(&lrepl[bn], 1u);
             else
               atomicAdd(&cold_args()->repl_hist[bn], 1ull);
@@ -694,7 +699,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           }
         } else {
           kadd(enuc_s, enuc_e, rE + ec.Elbind);
-          if (F.E < P.Ef) {
+          if (F.E < C->Ef) {
             state = 3;
             kadd(enuc_s, enuc_e, F.E);
           }
@@ -797,7 +802,10 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       const int r = lane_rank(push_m);
       const uint32_t meta = (uint32_t)rsp | ((uint32_t)(F.gen + 1) << 8);
       const uint64_t rkey = child_key(F.key, (uint32_t)F.ic);
-      if (push) atomicAdd(&P.h_out[F.hist - (uint32_t)P.hist0], 1);
+      if (push) {
+        const ArgPtr C1 = cold_args();
+        atomicAdd(&C1->h_out[F.hist - (uint32_t)C1->hist0], 1);
+      }
       if (push && r < room) {
         const int j = scount + r;
         S.px[j] = rx;
@@ -874,7 +882,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       // so its latency overlaps the next collision
       close_pending();
       if (fin) {
-        pend_old = atomicAdd(&P.h_out[F.hist - (uint32_t)P.hist0], -1);
+        const ArgPtr C2 = cold_args();
+        pend_old = atomicAdd(&C2->h_out[F.hist - (uint32_t)C2->hist0], -1);
         pend = true;
       }
       fe_pending += __popcll(fin_m);
