@@ -24,7 +24,10 @@
 //
 // Tallies: per-lane 32-bit event counters summed over the wave at exit,
 // per-lane compensated FP64 energy sums, histogram bins in LDS (or HBM
-// atomics) flushed at exit.
+// atomics) flushed at exit.  The per-history instantiations (PERHIST) add
+// per-history results, the records of the stream-writing TrimBase subclasses
+// (trim.C:466-527) and the layer transport of a SampleDynamic drain loop;
+// the hot instantiations carry none of that code.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
