@@ -298,3 +298,23 @@ def test_layer_transport_matches_reference(name):
     _, _, tr = O.run_log(prob, 0, rng=0, transport=True)
     assert _transport_entries(tr) == sorted(gold)
     assert any(v < 0 for _, _, v in gold) and any(v > 0 for _, _, v in gold)
+
+
+@pytest.mark.parametrize("path", sorted(p for p in os.listdir(GOLD) if p.startswith("tally_")))
+def test_tally_identities_in_reference_output(path):
+    """The identities the full-size GPU runs are checked by hold in the
+    compiled reference's own tallies: the ledger closes, every flight ends
+    once, the histograms hold every binned event, and with every recoil
+    followed, flights = histories + vacancies + replacements + substitutions."""
+    with open(os.path.join(GOLD, path)) as f:
+        d = json.load(f)
+    n = int(d["n_hist"])
+    g = {k: int(d[k]) for k in ("n_flight", "vac", "repl", "subst", "inter", "lost", "vacuum", "range_n")}
+    ledger = sum(float(d[k]) for k in ("Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac"))
+    assert abs(ledger - float(d["E0"])) <= 1e-12 * float(d["E0"])
+    assert g["n_flight"] == g["repl"] + g["subst"] + g["inter"] + g["lost"] + g["vacuum"]
+    assert sum(d["range_hist"]) == g["range_n"]
+    assert sum(d["repl_hist"]) == g["repl"] + g["subst"]
+    if "primaries" not in path:  # Kinchin-Pease adds vacancies without flights
+        assert g["n_flight"] == n + g["vac"] + g["repl"] + g["subst"]
+        assert sum(d["vac_hist"]) == g["vac"]
