@@ -411,6 +411,8 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
       for (int i = 0; i < 3; ++i) {
         g.on[i] = int(on[i]);
         g.on_w[i] = double(on[i]) / g.w[i];
+        g.ks[i] = int(g.cmr / g.kd[i]) + 1;
+        g.inv_w[i] = 1.0 / g.w[i];
       }
       const long long nbits = on[0] * on[1] * on[2];
       std::vector<uint32_t> bits(size_t((nbits + 31) / 32), 0u);

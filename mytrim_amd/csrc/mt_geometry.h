@@ -50,37 +50,43 @@ __device__ __forceinline__ int lookup_cluster(const Geometry& g, const double po
       if (k[i] < 0) k[i] += g.kn[i];
     }
   }
+  int m1[3];  // k1 mod kn: the first cell index of the scan on each axis
   for (int i = 0; i < 3; ++i) {
-    const int ks = (int)(g.cmr / g.kd[i]) + 1;
-    k1[i] = k[i] - ks;
-    k2[i] = k[i] + ks;
+    k1[i] = k[i] - g.ks[i];
+    k2[i] = k[i] + g.ks[i];
     if (k1[i] < 0 && g.bc[i] != 0) k1[i] = 0;
     if (k2[i] >= g.kn[i] && g.bc[i] != 0) k2[i] = g.kn[i] - 1;
+    m1[i] = k1[i] % g.kn[i];
+    if (m1[i] < 0) m1[i] += g.kn[i];
   }
+  // cells in the reference's order; the periodic cell index advances with
+  // the loop counter and wraps at kn (one modulo per axis and lookup)
+  int c0 = m1[0];
   for (int j0 = k1[0]; j0 <= k2[0]; ++j0) {
-    int c0 = j0 % g.kn[0];
-    if (c0 < 0) c0 += g.kn[0];
+    int c1 = m1[1];
     for (int j1 = k1[1]; j1 <= k2[1]; ++j1) {
-      int c1 = j1 % g.kn[1];
-      if (c1 < 0) c1 += g.kn[1];
+      int c2 = m1[2];
       for (int j2 = k1[2]; j2 <= k2[2]; ++j2) {
-        int c2 = j2 % g.kn[2];
-        if (c2 < 0) c2 += g.kn[2];
         int l = g.sh[c0 + g.kn[0] * (c1 + g.kn[1] * c2)];
         while (l >= 0) {
           double d0 = pos[0] - g.cx[l];
           double d1 = pos[1] - g.cy[l];
           double d2 = pos[2] - g.cz[l];
-          if (g.bc[0] == 0) d0 -= round(d0 / g.w[0]) * g.w[0];
-          if (g.bc[1] == 0) d1 -= round(d1 / g.w[1]) * g.w[1];
-          if (g.bc[2] == 0) d2 -= round(d2 / g.w[2]) * g.w[2];
+          // minimum image with 1/w: it can differ from d / w only where
+          // |d| is half the box, far outside any cluster smaller than that
+          if (g.bc[0] == 0) d0 -= round(d0 * g.inv_w[0]) * g.w[0];
+          if (g.bc[1] == 0) d1 -= round(d1 * g.inv_w[1]) * g.w[1];
+          if (g.bc[2] == 0) d2 -= round(d2 * g.inv_w[2]) * g.w[2];
           const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
           const double rc = g.cr[l];
           if (r2 < rc * rc) return l;
           l = g.cl[l];
         }
+        c2 = (c2 + 1 == g.kn[2]) ? 0 : c2 + 1;
       }
+      c1 = (c1 + 1 == g.kn[1]) ? 0 : c1 + 1;
     }
+    c0 = (c0 + 1 == g.kn[0]) ? 0 : c0 + 1;
   }
   return -1;
 }

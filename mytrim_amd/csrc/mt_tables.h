@@ -91,6 +91,8 @@ struct Geometry {
   int kn[3];
   double kd[3];
   double cmr;
+  int ks[3];        // cells scanned on each side: int(cmr / kd[i]) + 1
+  double inv_w[3];  // 1 / w[i], for the minimum image
   const int* sh;
   const int* cl;
   const double* cx;
