@@ -159,11 +159,12 @@ def main():
                   "fixed": True}
     elif args.autotune_sample > 0:
         # the tuning sample holds at least --autotune-sample histories and about
-        # 1e9 collisions, at most one timed step: small cascades need more
-        # histories to reach the queue occupancy of a full launch
+        # 3e9 collisions, at most one timed step: small cascades need more
+        # histories to reach the queue occupancy of a full launch, and mixed
+        # workloads a sample close to a launch's size
         pilot = eng.run(prob.with_(hist0=3 * args.histories, n_hist=2000))
         per_hist = max(pilot["n_coll"] / 2000.0, 1.0)
-        n_sample = int(min(max(args.autotune_sample, 1e9 / per_hist), max(args.histories // max(args.steps, 1), 1)))
+        n_sample = int(min(max(args.autotune_sample, 3e9 / per_hist), max(args.histories // max(args.steps, 1), 1)))
         best, rate = eng.autotune(prob.with_(hist0=2 * args.histories, n_hist=n_sample), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
                   "sample_histories": n_sample, "sample_rate": rate}
