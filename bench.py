@@ -178,10 +178,9 @@ def main():
         nv = min(args.variant_histories, args.histories)
         c64, w64, k64, r64 = timed(eng, prob, gpu.MT_FP64, nv, 1)
         # each precision runs at its own autotuned launch shape (the FP32
-        # kernels need fewer registers, so more waves fit)
-        sample = prob.with_(hist0=2 * args.histories,
-                            n_hist=tuning["sample_histories"] if tuning and "sample_histories" in tuning
-                            else max(args.autotune_sample, 2000))
+        # kernels need fewer registers, so more waves fit), tuned on a sample
+        # the size of the variant's own launch
+        sample = prob.with_(hist0=2 * args.histories, n_hist=nv)
         t32 = _tuned(eng, sample, gpu.MT_FP32, args.autotune_sample)
         c32, w32, k32, r32 = timed(eng, prob, gpu.MT_FP32, nv, 1)
         sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)
