@@ -5,7 +5,9 @@ Workload: 500 keV Cu ions into a 2 um Cu cube (8.96 g/cm3, Ed 30 eV, Eb 3 eV),
 1e6 histories, FP64, every recoil followed (mytrim_amd.configs.c2_cu_cu).  A
 step is one batch of histories (one persistent-kernel launch); the K timed
 steps together cover all --histories histories (distinct random streams per
-history), the W warm-up steps run on extra histories.  The launch shape is
+history), the W warm-up steps run on extra histories.  By default the whole
+job is one launch: every launch ends with a tail in which few lanes still
+hold a flight, so fewer, larger launches keep more of the chip busy.  The launch shape is
 autotuned on a sample first (outside the timed region).
 
 Prints ONE JSON line: collisions/s (value) and histories/s of the whole job,
@@ -120,7 +122,7 @@ def timed(eng, prob, arith, n_hist, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=int, default=1000000)
     ap.add_argument("--config", default="c2")
