@@ -60,53 +60,65 @@ def random_bubbles(w: float, n: int, rmin: float, rmax: float, sep: float,
                    seed: int) -> np.ndarray:
     """Non-overlapping spheres with surface separation >= sep in a periodic cube,
     placed by rejection like sampleClusters::addRandomClusters
-    (sample_clusters.C:227-252), with radii uniform in [rmin, rmax]."""
+    (sample_clusters.C:227-252), with radii uniform in [rmin, rmax].
+
+    Candidates are drawn in chunks and taken in order: a candidate is kept
+    when it overlaps neither a sphere kept before the chunk nor an earlier
+    kept candidate of the chunk (minimum-image distance under
+    r_a + r_b + sep), until n are kept."""
     from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
     out = np.zeros((0, 4))
+
+    def overlaps(a, b):
+        d = a[:, :3] - b[:, :3]
+        d -= np.round(d / w) * w
+        return np.einsum("ij,ij->i", d, d) < (a[:, 3] + b[:, 3] + sep) ** 2
+
     while len(out) < n:
         k = min(4 * (n - len(out)) + 64, 200000)
         cand = np.column_stack([rng.uniform(0, w, (k, 3)), rng.uniform(rmin, rmax, k)])
-        keep = []
-        tree = cKDTree(out[:, :3], boxsize=w) if len(out) else None
-        acc = []
-        for c in cand:
-            ok = True
-            if tree is not None:
-                for j in tree.query_ball_point(c[:3], c[3] + rmax + sep):
-                    d = out[j, :3] - c[:3]
-                    d -= np.round(d / w) * w
-                    if np.dot(d, d) < (c[3] + out[j, 3] + sep) ** 2:
-                        ok = False
-                        break
-            if ok:
-                for a in acc:
-                    d = a[:3] - c[:3]
-                    d -= np.round(d / w) * w
-                    if np.dot(d, d) < (c[3] + a[3] + sep) ** 2:
-                        ok = False
-                        break
-            if ok:
-                acc.append(c)
-                if len(acc) >= 256 or len(out) + len(acc) >= n:
-                    out = np.vstack([out, np.array(acc)])
-                    acc = []
-                    tree = cKDTree(out[:, :3], boxsize=w)
-                    if len(out) >= n:
-                        break
-        if acc:
-            out = np.vstack([out, np.array(acc)])
+        bad = np.zeros(k, bool)
+        if len(out):
+            tree = cKDTree(out[:, :3], boxsize=w)
+            near = tree.query_ball_point(cand[:, :3], cand[:, 3] + rmax + sep)
+            ci = np.repeat(np.arange(k), [len(x) for x in near])
+            if len(ci):
+                oj = np.concatenate([np.asarray(x, np.int64) for x in near])
+                bad[ci[overlaps(out[oj], cand[ci])]] = True
+        # conflicts among the chunk's own candidates (pairs i < j)
+        pairs = cKDTree(cand[:, :3], boxsize=w).query_pairs(2 * rmax + sep, output_type="ndarray")
+        later = [[] for _ in range(k)]
+        if len(pairs):
+            i, j = pairs[:, 0], pairs[:, 1]
+            lo, hi = np.minimum(i, j), np.maximum(i, j)
+            hit = overlaps(cand[lo], cand[hi])
+            for a, b in zip(lo[hit], hi[hit]):
+                later[b].append(a)
+        kept = np.zeros(k, bool)
+        need = n - len(out)
+        taken = []
+        for c in range(k):
+            if bad[c] or any(kept[a] for a in later[c]):
+                continue
+            kept[c] = True
+            taken.append(c)
+            if len(taken) >= need:
+                break
+        out = np.vstack([out, cand[taken]])
     return out[:n]
 
 
 def c4_mixed_uo2(n_hist: int = 1000000, seed: int = 2344, w: float = 10000.0,
-                 bubble_density: float = 1.0e-7) -> Problem:
+                 bubble_density: float = 1.0e-7, geometry_seed: int = -1) -> Problem:
     """H, He, C, Xe ions, energies log-uniform in 10 keV..2 MeV, started at
     uniform random points with isotropic directions in periodic UO2
-    (10.96 g/cm3) holding Xe bubbles (3.5 g/cm3, r in 10..50 A, 1e17 cm^-3)."""
+    (10.96 g/cm3) holding Xe bubbles (3.5 g/cm3, r in 10..50 A, 1e17 cm^-3).
+    The primaries and the random streams come from `seed`, the bubbles from
+    `geometry_seed` (default seed + 1)."""
     rng = np.random.default_rng(seed)
     n_b = int(round(bubble_density * w ** 3))
-    bubbles = random_bubbles(w, n_b, 10.0, 50.0, 5.0, seed + 1)
+    bubbles = random_bubbles(w, n_b, 10.0, 50.0, 5.0, seed + 1 if geometry_seed < 0 else geometry_seed)
     species = np.array([(1, 1.008), (2, 4.0026), (6, 12.011), (54, 131.29)])
     pick = rng.integers(0, 4, n_hist)
     d = rng.normal(size=(n_hist, 3))

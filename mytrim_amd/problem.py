@@ -98,6 +98,23 @@ class Problem:
     def with_(self, **kw) -> "Problem":
         return dataclasses.replace(self, **kw)
 
+    def batch(self, first: int, count: int) -> "Problem":
+        """Histories first .. first+count-1 of this problem (hist0 + first on):
+        the same random streams, and for per-history primaries the rows of
+        those histories."""
+        if first < 0 or count < 0 or first + count > self.n_hist:
+            raise ValueError("batch %d+%d outside %d histories" % (first, count, self.n_hist))
+        p = self.primaries
+        if not p.uniform:
+            sl = slice(first, first + count)
+            p = Primaries(p.Z[sl], p.m[sl], p.E[sl], p.pos[sl], p.dir[sl])
+        return dataclasses.replace(self, primaries=p, hist0=self.hist0 + first, n_hist=count)
+
+    def primary_energies(self) -> np.ndarray:
+        """E0 of every history of the problem."""
+        p = self.primaries
+        return np.full(self.n_hist, float(p.E[0])) if p.uniform else np.asarray(p.E[:self.n_hist], float)
+
     # flat views ----------------------------------------------------------
     def flat_materials(self):
         rho = np.array([m.rho for m in self.materials], np.float64)
@@ -115,8 +132,10 @@ class Problem:
                 np.array(mm, np.float64), np.array(t, np.float64),
                 np.array(ed, np.float64), np.array(eb, np.float64))
 
-    def spec_text(self) -> str:
-        """Key/value text read by the reference fixture harness."""
+    def spec_text(self, pkafile: Optional[str] = None) -> str:
+        """Key/value text read by the reference fixture harness.  Per-history
+        primaries go to `pkafile` (rows Z m E x y z dx dy dz, row i for global
+        history i), which the caller writes with :meth:`primary_rows`."""
         g = self.geometry
         kind = {GEO_SOLID: "solid", GEO_LAYERS: "layers", GEO_CLUSTERS: "clusters", GEO_WIRE: "wire",
                 GEO_BURRIED_WIRE: "burriedwire"}[g.kind]
@@ -136,7 +155,9 @@ class Problem:
                 out.append("cluster %r %r %r %r" % tuple(float(v) for v in c))
         p = self.primaries
         if not p.uniform:
-            raise ValueError("the fixture harness takes one primary species")
+            if pkafile is None:
+                raise ValueError("per-history primaries need a pkafile")
+            out.append("pkafile " + pkafile)
         out += ["tmin %r" % self.tmin, "tau %r" % self.tau, "cw %r" % self.cw,
                 "scale %r" % self.scale, "Ef %r" % self.Ef, "potential %d" % self.potential,
                 "pka %d %r %r %r %r %r %r %r %r" % (
@@ -148,6 +169,16 @@ class Problem:
                 "vac %r %r %d" % (self.vac_bins.x0, self.vac_bins.bw, self.vac_bins.nb),
                 "lat %r %r %r %d" % (self.lat_y0, self.lat_z0, self.lat_bw, self.lat_nb)]
         return "\n".join(out) + "\n"
+
+    def primary_rows(self) -> str:
+        """Rows of the harness's pkafile for histories 0 .. hist0+n_hist-1 of a
+        problem whose primaries start at history 0."""
+        p = self.primaries
+        if self.hist0 != 0:
+            raise ValueError("primary_rows: problem must start at history 0")
+        return "".join("%d %r %r %r %r %r %r %r %r\n" % (
+            int(p.Z[i]), float(p.m[i]), float(p.E[i]), *[float(v) for v in p.pos[i]],
+            *[float(v) for v in p.dir[i]]) for i in range(self.n_hist))
 
 
 def log_uniform(rng: np.random.Generator, lo: float, hi: float, n: int) -> np.ndarray:

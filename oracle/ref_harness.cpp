@@ -66,9 +66,13 @@ struct Spec {
   int potential = 0;
   int pZ = 1;
   double pm = 1, pE = 1, ppos[3] = {0, 0, 0}, pdir[3] = {1, 0, 0};
+  // per-history primaries (pkafile): Z m E x y z dx dy dz, one row per
+  // global history id, from history 0
+  std::vector<double> prow;
   long long nhist = 1, hist0 = 0;
   unsigned seed = 1;
   int follow = -1, kp = 0;
+  int perhist = 0;  // tally: also print per-history arrays
   double range_x0 = 0, range_bw = 10, vac_x0 = 0, vac_bw = 10, lat_y0 = 0, lat_z0 = 0, lat_bw = 10;
   int range_nb = 100, vac_nb = 100, lat_nb = 100;
 };
@@ -122,11 +126,26 @@ Spec read_spec(const char* path) {
     else if (k == "pka")
       ss >> s.pZ >> s.pm >> s.pE >> s.ppos[0] >> s.ppos[1] >> s.ppos[2] >> s.pdir[0] >> s.pdir[1] >>
           s.pdir[2];
-    else if (k == "nhist") ss >> s.nhist;
+    else if (k == "pkafile") {
+      std::string path;
+      ss >> path;
+      std::ifstream pf(path.c_str());
+      if (!pf) {
+        std::cerr << "cannot open " << path << "\n";
+        exit(2);
+      }
+      double v;
+      while (pf >> v) s.prow.push_back(v);
+      if (s.prow.size() % 9) {
+        std::cerr << "pkafile: rows of 9 numbers expected\n";
+        exit(2);
+      }
+    } else if (k == "nhist") ss >> s.nhist;
     else if (k == "hist0") ss >> s.hist0;
     else if (k == "seed") ss >> s.seed;
     else if (k == "follow") ss >> s.follow;
     else if (k == "kp") ss >> s.kp;
+    else if (k == "perhist") ss >> s.perhist;
     else if (k == "range") ss >> s.range_x0 >> s.range_bw >> s.range_nb;
     else if (k == "vac") ss >> s.vac_x0 >> s.vac_bw >> s.vac_nb;
     else if (k == "lat") ss >> s.lat_y0 >> s.lat_z0 >> s.lat_bw >> s.lat_nb;
@@ -276,13 +295,28 @@ class TraceTrim : public TrimBase {
   }
 };
 
-IonBase* primary(const Spec& s) {
+// the primary ion of global history h: the spec's single species, or row h
+// of its pkafile
+IonBase* primary(const Spec& s, long long h) {
   IonBase* p = new IonBase;
-  p->_Z = s.pZ;
-  p->_m = s.pm;
-  p->_E = s.pE;
-  p->_pos = Point(s.ppos[0], s.ppos[1], s.ppos[2]);
-  p->_dir = Point(s.pdir[0], s.pdir[1], s.pdir[2]);
+  if (!s.prow.empty()) {
+    if ((h + 1) * 9 > (long long)s.prow.size()) {
+      std::cerr << "pkafile holds no row for history " << h << "\n";
+      exit(2);
+    }
+    const double* r = &s.prow[h * 9];
+    p->_Z = int(r[0]);
+    p->_m = r[1];
+    p->_E = r[2];
+    p->_pos = Point(r[3], r[4], r[5]);
+    p->_dir = Point(r[6], r[7], r[8]);
+  } else {
+    p->_Z = s.pZ;
+    p->_m = s.pm;
+    p->_E = s.pE;
+    p->_pos = Point(s.ppos[0], s.ppos[1], s.ppos[2]);
+    p->_dir = Point(s.pdir[0], s.pdir[1], s.pdir[2]);
+  }
   p->_gen = 0;
   p->_Ef = s.Ef;
   return p;
@@ -295,6 +329,12 @@ unsigned history_seed(unsigned master, long long h) {
 void print_vec(const char* name, const std::vector<long long>& v) {
   printf("%s", name);
   for (long long x : v) printf(" %lld", x);
+  printf("\n");
+}
+
+void print_dvec(const char* name, const std::vector<double>& v) {
+  printf("%s", name);
+  for (double x : v) printf(" %.17g", x);
   printf("\n");
 }
 
@@ -313,13 +353,19 @@ int cmd_tally(const Spec& s) {
   t.repl_hist.assign(s.vac_nb + 2, 0);
   TallyTrim trim(&sc, smp, s, t);
   trim._potential = TrimBase::Potential(s.potential);
-  std::vector<long long> hvac;
+  std::vector<long long> hvac, hcoll;
+  // per history: Eel, Enuc, E0, and the primary's end (x y z state; state -1
+  // for a vacuum exit, as the batched kernels report it)
+  std::vector<double> heel, henuc, he0, hend;
   for (long long h = s.hist0; h < s.hist0 + s.nhist; ++h) {
-    const long long vac0 = t.vac;
+    const long long vac0 = t.vac, coll0 = t.n_coll;
+    const double eel0 = t.Eel, enuc0 = t.Enuc;
+    double end[4] = {0, 0, 0, 0};
     sc.seed(history_seed(s.seed, h));
     std::queue<IonBase*> q;
-    IonBase* p = primary(s);
+    IonBase* p = primary(s, h);
     t.E0 += p->_E;
+    he0.push_back(p->_E);
     q.push(p);
     while (!q.empty()) {
       IonBase* ion = q.front();
@@ -340,9 +386,17 @@ int cmd_tally(const Spec& s) {
         t.lat_y[bin_of(ion->_pos(1) - s.lat_y0, -0.5 * s.lat_nb * s.lat_bw, s.lat_bw, s.lat_nb)]++;
         t.lat_z[bin_of(ion->_pos(2) - s.lat_z0, -0.5 * s.lat_nb * s.lat_bw, s.lat_bw, s.lat_nb)]++;
       }
+      if (ion->_gen == 0) {
+        for (int i = 0; i < 3; ++i) end[i] = ion->_pos(i);
+        end[3] = vacuum ? -1.0 : double(ion->_state);
+      }
       delete ion;
     }
     hvac.push_back(t.vac - vac0);
+    hcoll.push_back(t.n_coll - coll0);
+    heel.push_back(t.Eel - eel0);
+    henuc.push_back(t.Enuc - enuc0);
+    hend.insert(hend.end(), end, end + 4);
   }
   printf("n_coll %lld\nn_flight %lld\nvac %lld\nrepl %lld\nsubst %lld\ninter %lld\nlost %lld\n",
          t.n_coll, t.n_flight, t.vac, t.repl, t.subst, t.inter, t.lost);
@@ -356,6 +410,13 @@ int cmd_tally(const Spec& s) {
   print_vec("vac_hist", t.vac_hist);
   print_vec("repl_hist", t.repl_hist);
   print_vec("h_vac", hvac);
+  if (s.perhist) {
+    print_vec("h_coll", hcoll);
+    print_dvec("h_Eel", heel);
+    print_dvec("h_Enuc", henuc);
+    print_dvec("h_E0", he0);
+    print_dvec("h_end", hend);
+  }
   return 0;
 }
 
@@ -370,7 +431,7 @@ int cmd_trace(const Spec& s, long long h, int max_rows) {
   trim._potential = TrimBase::Potential(s.potential);
   sc.seed(history_seed(s.seed, h));
   std::queue<IonBase*> q;
-  q.push(primary(s));
+  q.push(primary(s, h));
   while (!q.empty() && int(trim.rows.size() / 21) < max_rows) {
     IonBase* ion = q.front();
     q.pop();
@@ -469,7 +530,7 @@ int cmd_log(const Spec& s, const std::string& mode, int z1, int z2, int z3) {
   for (long long h = s.hist0; h < s.hist0 + s.nhist; ++h) {
     sc.seed(history_seed(s.seed, h));
     std::queue<IonBase*> q;
-    IonBase* p = primary(s);
+    IonBase* p = primary(s, h);
     p->_id = 0;
     q.push(p);
     while (!q.empty()) {

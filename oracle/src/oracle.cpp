@@ -18,6 +18,12 @@ namespace orc {
 
 Coef g_coef[92];
 
+namespace {
+int g_planted_fault = 0;
+}
+int planted_fault() { return g_planted_fault; }
+void set_planted_fault(int f) { g_planted_fault = f; }
+
 // Reads the table written by mytrim_amd/data/gen_tables.py (one row per Z:
 // mm1 m1 mnat rho atrho vfermi heat pcoef[8] lfctr).
 int load_tables(const char* path) {
@@ -464,7 +470,10 @@ void PhiloxDraws::perp(const double d[3], double out[3]) {
   }
   const double v[3] = {d[1] * u[2] - d[2] * u[1], d[2] * u[0] - d[0] * u[2],
                        d[0] * u[1] - d[1] * u[0]};
-  const double phi = 6.283185307179586 * u32d(blk.v[3]);
+  // a planted azimuth error (orc_plant_fault(1)) draws phi over half the
+  // circle; tests use it to show the statistical comparison detects it
+  const double span = planted_fault() == 1 ? 3.141592653589793 : 6.283185307179586;
+  const double phi = span * u32d(blk.v[3]);
   const double c = std::cos(phi), s = std::sin(phi);
   for (int i = 0; i < 3; ++i) out[i] = c * u[i] + s * v[i];
 }

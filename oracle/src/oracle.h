@@ -216,6 +216,11 @@ struct MtDraws {
   void perp(const double d[3], double out[3]);
 };
 
+// Deliberate error planted in the device draw scheme for tests of the tests
+// (0: none; 1: azimuth over [0, pi) instead of [0, 2 pi)).  Test use only.
+int planted_fault();
+void set_planted_fault(int f);
+
 // Device scheme: one Philox4x32-10 block per collision, keyed by
 // (seed, history) and counted by (collision index, flight key); the azimuth
 // about the flight direction is drawn directly.

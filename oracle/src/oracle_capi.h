@@ -90,6 +90,8 @@ typedef struct {
 } orc_record;
 
 int orc_load_tables(const char* path);
+// Plants a deliberate error in the device draw scheme (see oracle.h); 0 clears it.
+void orc_plant_fault(int f);
 double orc_rpstop(int z2, double e);
 double orc_rstop(int z1, double m1, double E, int z2);
 double orc_getrstop(int Z1, double m1, double E, double rho, int n_el, const int* Z2,

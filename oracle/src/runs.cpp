@@ -379,6 +379,7 @@ using namespace orc;
 extern "C" {
 
 int orc_load_tables(const char* path) { return load_tables(path); }
+void orc_plant_fault(int f) { orc::set_planted_fault(f); }
 double orc_rpstop(int z2, double e) { return rpstop(z2, e); }
 double orc_rstop(int z1, double m1, double E, int z2) { return rstop(z1, m1, E, z2); }
 

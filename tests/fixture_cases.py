@@ -62,6 +62,28 @@ def tally_cases(quick=False):
     }
 
 
+def stat_cases():
+    """name -> problem for the statistical fixtures (tests/golden/stat_*.npz):
+    larger samples of the workloads, with per-history tallies and batched
+    depth histograms, for the two-sample tests of tests/tally_stats.py.  The
+    mixed C4 workload runs on its benchmark box with its per-history H, He,
+    C and Xe primaries."""
+    return {
+        "c1": configs.c1_he_fe(16000),
+        "c2": configs.c2_cu_cu(4800),
+        "c3": configs.c3_ag_layers(3200),
+        "c4mix": configs.c4_mixed_uo2(6400),
+        "c5": configs.c5_c_w(6400),
+        "c2_primaries_kp": configs.c2_cu_cu(16000).with_(follow_max_gen=1, kp=True),
+        "vac_cu_10keV": configs.vacancy_case("cu", 1e4, 9600),
+        "wire": configs.wire_b_si(3200),
+        "bwire": configs.burried_wire_p_si(960),
+    }
+
+
+STAT_BATCHES = 32   # reference batches per statistical fixture (equal sizes)
+
+
 def xe_si_sio2(n_hist, seed=2344):
     """30 keV Xe into 100 A Si over 400 A SiO2, 500 x 100 x 100 A: the layer
     stack of a SampleDynamic run; many ions leave through either face."""

@@ -15,12 +15,18 @@ computed by the reference library; the harness only drives its API.
   log_<case>.npz      what TrimDefectLog and TrimPhononOut write (17 digits), the
                       positions TrimHistory stores and TrimVacMap's map
                       (trim.C:466-527, trim.h:139-227), histories in order
+  stat_<case>.npz     larger full-cascade samples for the two-sample tests:
+                      per-history vacancies, collisions, Eel, Enuc and E0,
+                      the end-point histograms (range, lateral) and the
+                      depth histograms (vacancies, replacements) of each of
+                      STAT_BATCHES equal batches of histories
   runmytrim/<case>/   a JSON deck of the reference (tests/json, validation/) with
                       fewer ions, and the output file and totals the reference's
                       runmytrim loop writes for it (oracle/_ref/runmytrim_ref:
                       the reference's driver loop and output modules; jsoncpp is
                       absent, so the deck is handed over as a flat spec)
 Usage:  python tests/golden/make_fixtures.py [--quick] [--tally=NAME] [--trace=NAME] [--logs] [--runmytrim]
+                                            [--stat[=NAME]]
 """
 import json
 import os
@@ -40,7 +46,8 @@ import oracle_lib as O  # noqa: E402
 
 HARNESS = O.REF_HARNESS
 
-from fixture_cases import ENERGIES, IONS, TARGETS, log_cases, tally_cases, trace_cases, transport_cases  # noqa: E402
+from fixture_cases import (ENERGIES, IONS, STAT_BATCHES, TARGETS, log_cases, stat_cases,  # noqa: E402
+                           tally_cases, trace_cases, transport_cases)
 
 
 def run(args, stdin=None):
@@ -140,11 +147,59 @@ def parse_tally(txt):
     res = {}
     for line in txt.splitlines():
         k, *v = line.split()
-        if len(v) == 1:
+        if len(v) == 1 and not k.startswith("h_"):
             res[k] = v[0]
+        elif k in ("h_Eel", "h_Enuc", "h_E0", "h_end"):
+            res[k] = [float(x) for x in v]
         else:
             res[k] = [int(x) for x in v]
     return res
+
+
+def stat_fixtures(only=None):
+    """Each case in STAT_BATCHES equal batches, run in parallel; the batches'
+    depth histograms are kept apart (batch means give their covariance)."""
+    for name, prob in stat_cases().items():
+        if only and name not in only:
+            continue
+        assert prob.n_hist % STAT_BATCHES == 0
+        m = prob.n_hist // STAT_BATCHES
+        tmp = tempfile.mkdtemp()
+        try:
+            pkafile = None
+            if not prob.primaries.uniform:
+                pkafile = os.path.join(tmp, "pka.txt")
+                with open(pkafile, "w") as f:
+                    f.write(prob.primary_rows())
+            specs = []
+            for i in range(STAT_BATCHES):
+                path = os.path.join(tmp, "b%d.spec" % i)
+                with open(path, "w") as f:
+                    f.write(prob.with_(hist0=i * m, n_hist=m).spec_text(pkafile) + "perhist 1\n")
+                specs.append(path)
+            with ThreadPoolExecutor(min(8, os.cpu_count() or 1)) as ex:
+                parts = list(ex.map(lambda p: parse_tally(run(["tally", p])), specs))
+        finally:
+            for f in os.listdir(tmp):
+                os.unlink(os.path.join(tmp, f))
+            os.rmdir(tmp)
+        out = {"n_hist": prob.n_hist, "seed": prob.seed, "batches": STAT_BATCHES}
+        for k in ("h_vac", "h_coll"):
+            out[k] = np.concatenate([np.array(p[k], np.int64) for p in parts])
+        for k in ("h_Eel", "h_Enuc", "h_E0"):
+            out[k] = np.concatenate([np.array(p[k], np.float64) for p in parts])
+        for k in ("range_hist", "lat_y_hist", "lat_z_hist"):
+            out[k] = np.sum([np.array(p[k], np.int64) for p in parts], axis=0)
+        for k in ("vac_hist", "repl_hist"):
+            out[k + "_batches"] = np.array([p[k] for p in parts], np.int32)
+        for k in ("n_coll", "n_flight", "vac", "repl", "subst", "inter", "lost", "vacuum", "disp",
+                  "range_n"):
+            out[k] = np.int64(sum(int(p[k]) for p in parts))
+        for k in ("E0", "Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac", "range_sum", "range_sum2"):
+            out[k] = np.float64(sum(float(p[k]) for p in parts))
+        np.savez_compressed(os.path.join(HERE, "stat_%s.npz" % name), problem=prob.name, **out)
+        print(name, "n_hist", prob.n_hist, "coll/hist", out["n_coll"] / prob.n_hist,
+              "vac/ion", out["vac"] / prob.n_hist, flush=True)
 
 
 def tallies(quick, only=None):
@@ -362,6 +417,10 @@ def main():
     if "--logs" in sys.argv:
         logs()
         return
+    stat = [a for a in sys.argv if a.startswith("--stat")]
+    if stat:
+        stat_fixtures([a.split("=", 1)[1] for a in stat if "=" in a] or None)
+        return
     if "--dynamic" in sys.argv:
         dynamic()
         transport()
@@ -379,6 +438,7 @@ def main():
     dynamic()
     transport()
     tallies(quick)
+    stat_fixtures()
 
 
 if __name__ == "__main__":

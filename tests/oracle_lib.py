@@ -77,6 +77,7 @@ def lib():
         L.orc_philox.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
         L.orc_flight_key_child.restype = C.c_ulonglong
         L.orc_flight_key_child.argtypes = [C.c_ulonglong, C.c_uint]
+        L.orc_plant_fault.argtypes = [C.c_int]
         L.orc_first_draw.restype = C.c_ulonglong
         L.orc_first_draw.argtypes = [C.c_ulonglong, C.c_uint, C.c_uint]
         L.orc_uo2.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int] + [C.c_char_p, C.c_int] * 3
@@ -171,6 +172,12 @@ def run(prob, rng: int = 1, nthreads: int = 1, per_history: bool = False) -> dic
     res.update(hists)
     res.update(per)
     return res
+
+
+def plant_fault(f: int):
+    """Plant a deliberate error in the oracle's device draw scheme (1: the
+    azimuth over half the circle); 0 clears it.  For tests of the tests."""
+    lib().orc_plant_fault(int(f))
 
 
 WORK_FIELDS = ["flop_eq", "collisions", "flights", "uniforms", "add", "mul", "div", "cmp", "sqrt",

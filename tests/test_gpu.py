@@ -13,7 +13,8 @@ Tolerances (FP64 path):
   decision deep in a long cascade, so
   >= 97 % of histories must agree exactly in every integer tally and totals
   within 1 %.
-* against the reference (std::mt19937 streams): statistics within 2.5 sigma.
+* against the reference (std::mt19937 streams): the two-sample criteria of
+  tally_stats (test_gpu_stats.py); the vacancy table within 3 sigma + 1.5 %.
 """
 import json
 import math
@@ -146,42 +147,7 @@ def test_history_offsets_select_the_same_streams(eng):
 
 
 # ---------------------------------------------------------------- vs the reference
-@pytest.mark.parametrize("name,prob,n", [
-    ("c1", lambda n: configs.c1_he_fe(n, seed=31), 20000),
-    ("vac_cu_10keV", lambda n: configs.vacancy_case("cu", 1e4, n, seed=31), 20000),
-    ("c5", lambda n: configs.c5_c_w(n, seed=31), 2000),
-    ("c2", lambda n: configs.c2_cu_cu(n, seed=31), 3000),
-    ("wire", lambda n: configs.wire_b_si(n, seed=31), 2000),
-    ("bwire", lambda n: configs.burried_wire_p_si(n, seed=31), 400),
-])
-def test_cascade_statistics_vs_reference(eng, name, prob, n):
-    ref = load_tally(name)
-    g = eng.run(prob(n), per_history=True)
-    rv = np.array(ref["h_vac"], float)
-    sig = math.sqrt(np.var(rv) / len(rv) + np.var(g["h_vac"]) / n)
-    print("%s vac/ion gpu %.3f ref %.3f (sigma %.3f), %.1f ms" %
-          (name, g["vac"] / n, rv.mean(), sig, g["kernel_ms"]))
-    assert abs(g["vac"] / n - rv.mean()) < 2.5 * sig
-    if ref["range_n"] == 0:  # no primary came to rest inside (a thin wire)
-        assert g["range_n"] <= max(3, n // 1000)
-    else:
-        m_ref = ref["range_sum"] / ref["range_n"]
-        v_ref = ref["range_sum2"] / ref["range_n"] - m_ref ** 2
-        m = g["range_sum"] / g["range_n"]
-        v = g["range_sum2"] / g["range_n"] - m ** 2
-        assert abs(m - m_ref) < 2.5 * math.sqrt(v_ref / ref["range_n"] + v / g["range_n"])
-    # collisions per history scale with the cascade size: within 5 % or three
-    # times the relative sigma of vacancies per history, whichever is larger
-    cpi = ref["n_coll"] / ref["n_hist"]
-    assert g["n_coll"] / n == pytest.approx(cpi, rel=max(0.05, 3 * sig / rv.mean()))
-    if name == "wire":
-        # flights leaving through a CUT face or out of the wire's side: mostly
-        # the primaries themselves, so nearly Poisson counts (4 sigma)
-        for k in ("lost", "vacuum"):
-            a, b = g[k] / n, ref[k] / ref["n_hist"]
-            assert abs(a - b) <= 4 * math.sqrt(g[k] / n ** 2 + ref[k] / ref["n_hist"] ** 2), k
-
-
+# (the two-sample tally comparisons are in test_gpu_stats.py)
 def test_vacancy_table_on_gpu(eng):
     rows = np.loadtxt(os.path.join(GOLD, "validation", "vacancy_count_comparison.dat"),
                       skiprows=2, delimiter=",")

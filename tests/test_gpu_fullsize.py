@@ -8,19 +8,20 @@ the hot kernel, checked by properties that hold at any size:
   interstitials + lost + vacuum exits;
 * the histograms hold every event they bin (range: gen-0 ions at rest,
   vacancies, replacement collisions);
-* vacancies and collisions per history agree with the compiled reference's
-  tallies of the same workload (tests/golden/tally_*.json, std::mt19937
-  streams) within 3 sigma (collisions: 5) of the reference sample's mean.
+* vacancies per ion, collisions per history, Eel/E0, Enuc/E0 and the mean
+  range agree with the compiled reference's histories of the same workload
+  (tests/golden/stat_*.npz, std::mt19937 streams; the mixed C4 workload with
+  its own H/He/C/Xe primaries in the same bubbles): jointly p >= 1e-3 and
+  each within 3.3 sigma (tally_stats).  At 1e6 histories the GPU sample's
+  own variance is under 1 % of the reference sample's and is neglected.
 The compiled reference's own tallies satisfy the same identities
 (test_oracle.py, test_tally_identities_in_reference_output); here they are
 checked at the size the benchmark runs."""
-import json
-import math
 import os
 
-import numpy as np
 import pytest
 
+import tally_stats as S
 from mytrim_amd import configs, gpu
 
 pytestmark = pytest.mark.gpu
@@ -55,16 +56,10 @@ def test_full_size_invariants(eng, name):
     assert int(r["range_hist"].sum()) == r["range_n"]
     assert int(r["vac_hist"].sum()) == r["vac"]
     assert int(r["repl_hist"].sum()) == r["repl"] + r["subst"]
-    path = os.path.join(GOLD, "tally_%s.json" % name)
-    if name == "c4":
-        return  # the C4 fixture is a single-species variant of the workload
-    with open(path) as f:
-        ref = json.load(f)
-    hv = np.array(ref["h_vac"], np.float64)
-    sig = math.sqrt(hv.var(ddof=1) / len(hv))
-    assert abs(r["vac"] / n - hv.mean()) <= 3.0 * sig + 1e-9
-    # collisions per history, with the vacancies' relative spread as the
-    # estimate of theirs (the fixture keeps per-history vacancies only) and
-    # a wider 5-sigma bound for that estimate
-    m_ref = float(ref["n_coll"]) / ref["n_hist"]
-    assert abs(r["n_coll"] / n - m_ref) <= 5.0 * sig * m_ref / max(hv.mean(), 1e-9) + 1e-9
+    ref = S.load_ref({"c4": "c4mix"}.get(name, name))
+    sc, chi2, p = S.scalars(ref, S.Batches([r], n, [float(prob.primary_energies().sum())]))
+    for k, (a, b, z) in sc.items():
+        print("%s %-14s ref %.6g  gpu %.6g  z %+.2f" % (name, k, a, b, z))
+    print("%s joint chi2 %.2f p %.3g" % (name, chi2, p))
+    assert p >= S.P_MIN
+    assert all(abs(z) <= S.Z_SCALAR_MAX for (_, _, z) in sc.values())

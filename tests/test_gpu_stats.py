@@ -1,0 +1,75 @@
+"""GPU cascades against the compiled reference's tallies, statistically.
+
+The hot kernel (no per-history records) runs each workload in 32 equal
+batches; its tallies are compared with the reference's std::mt19937
+histories (tests/golden/stat_*.npz) by the two-sample criteria of
+tally_stats: vacancies per ion, collisions per history, Eel/E0, Enuc/E0 and
+the mean projected range (jointly, p >= 1e-3, each |z| <= 3.3, the count
+within 2 sigma printed), the range and lateral histograms (chi-square
+homogeneity), the vacancy and replacement depth histograms (Hotelling T^2).
+The FP32 kernel is held to the same criteria.  The GPU sample is several
+times the reference's, so the reference sample's own error dominates sigma.
+"""
+import pytest
+
+import tally_stats as S
+from fixture_cases import stat_cases
+from mytrim_amd import configs, gpu
+
+pytestmark = pytest.mark.gpu
+
+# GPU histories per workload (32 equal batches)
+N_GPU = {"c1": 128000, "c2": 38400, "c3": 25600, "c4mix": 51200, "c5": 51200,
+         "c2_primaries_kp": 128000, "vac_cu_10keV": 76800, "wire": 25600, "bwire": 7680}
+K = 32
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = gpu.Engine(0)
+    yield e
+    e.close()
+
+
+def gpu_problem(name):
+    """The workload of stat_<name>.npz with other random streams: seed 31 for
+    the histories; for the mixed C4 workload other primaries in the same
+    bubble geometry."""
+    if name == "c4mix":
+        return configs.c4_mixed_uo2(N_GPU[name], seed=31, geometry_seed=2344 + 1)
+    prob = stat_cases()[name]
+    return prob.with_(seed=31, n_hist=N_GPU[name])
+
+
+def gpu_batches(eng, name, arith):
+    prob = gpu_problem(name)
+    eng.set_target(prob)
+    return S.run_batches(lambda b: eng.run(b, arith=arith), prob, K)
+
+
+@pytest.mark.parametrize("name", sorted(N_GPU))
+def test_fp64_tallies_match_reference(eng, name):
+    ref = S.load_ref(name)
+    fails, lines = S.check(name, ref, gpu_batches(eng, name, gpu.MT_FP64))
+    print("\n" + "\n".join("fp64 " + l for l in lines))
+    assert not fails, fails
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "c4mix", "c5"])
+def test_fp32_tallies_match_reference(eng, name):
+    ref = S.load_ref(name)
+    fails, lines = S.check(name, ref, gpu_batches(eng, name, gpu.MT_FP32))
+    print("\n" + "\n".join("fp32 " + l for l in lines))
+    assert not fails, fails
+
+
+def test_wire_escapes_match_reference(eng):
+    """Flights leaving the wire through a CUT face or its side (mostly the
+    primaries themselves, so nearly Poisson counts): within 4 sigma."""
+    import math
+    ref = S.load_ref("wire")
+    b = gpu_batches(eng, "wire", gpu.MT_FP64)
+    n_r, n_g = int(ref["n_hist"]), b.n
+    for k in ("lost", "vacuum"):
+        a, g = float(ref[k]) / n_r, b.col(k).sum() / n_g
+        assert abs(a - g) <= 4 * math.sqrt(float(ref[k]) / n_r ** 2 + b.col(k).sum() / n_g ** 2), k

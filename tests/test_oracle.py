@@ -235,28 +235,6 @@ def test_vacancy_table_reproduced():
         assert abs(mean - val) < 3 * sem * math.sqrt(2) + 0.015 * val, (ion, e, mean, val)
 
 
-@pytest.mark.parametrize("name,prob,n", [
-    ("c1", lambda n: configs.c1_he_fe(n, seed=7), 2000),
-    ("vac_cu_10keV", lambda n: configs.vacancy_case("cu", 1e4, n, seed=7), 2000),
-    ("c5", lambda n: configs.c5_c_w(n, seed=7), 300),
-])
-def test_device_draw_scheme_matches_reference_statistically(name, prob, n):
-    """Philox streams + direct azimuth (the GPU scheme) vs the reference's
-    mt19937 + rejection azimuth: vacancies/ion and mean range within 2.5 sigma."""
-    ref = load_tally(name)
-    mine = O.run(prob(n), rng=1, nthreads=8, per_history=True)
-    rv = np.array(ref["h_vac"], float)
-    sig = math.sqrt(np.var(rv) / len(rv) + np.var(mine["h_vac"]) / n)
-    assert abs(mine["vac"] / n - rv.mean()) < 2.5 * sig
-    assert mine["range_n"] > 0
-    m_ref = ref["range_sum"] / ref["range_n"]
-    v_ref = ref["range_sum2"] / ref["range_n"] - m_ref ** 2
-    m = mine["range_sum"] / mine["range_n"]
-    v = mine["range_sum2"] / mine["range_n"] - m ** 2
-    sig = math.sqrt(v_ref / ref["range_n"] + v / mine["range_n"])
-    assert abs(m - m_ref) < 2.5 * sig + 1e-9
-
-
 # ---------------------------------------------------------------- event streams
 @pytest.mark.parametrize("name", ["vac_cu_10keV", "wire", "c4"])
 def test_event_logs_bitwise(name):
