@@ -3,12 +3,11 @@
 
 Workload: 500 keV Cu ions into a 2 um Cu cube (8.96 g/cm3, Ed 30 eV, Eb 3 eV),
 1e6 histories, FP64, every recoil followed (mytrim_amd.configs.c2_cu_cu).  A
-step is one batch of histories (one persistent-kernel launch); the K timed
-steps together cover all --histories histories (distinct random streams per
-history), the W warm-up steps run on extra histories.  By default the whole
-job is one launch: every launch ends with a tail in which few lanes still
-hold a flight, so fewer, larger launches keep more of the chip busy.  The launch shape is
-autotuned on a sample first (outside the timed region).
+step is one pass of the persistent cascade kernel over one batch of
+--histories histories (default 1e6, the workload's size): step i runs
+histories [i H, (i + 1) H), so every step has its own random streams.  The W
+warm-up steps run smaller batches on other histories, untimed.  The launch
+shape is autotuned on a sample first (outside the timed region).
 
 Prints ONE JSON line: collisions/s (value) and histories/s of the whole job,
 the same workload in FP32 and FP32 + learned scattering surrogate on a
@@ -31,7 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix alike)
-FP32_VALU_PEAK_TFLOPS = 157.3
+# MI355X FP32 vector peak without packed FP32 instructions: 157.3 TFLOP/s
+# counts the packed (two-wide) FMA; the kernels are built with packed FP32
+# disabled, so one FP32 FMA per lane per clock, the same rate as FP64
+FP32_UNPACKED_PEAK_TFLOPS = 78.6
+CPU_CORES_MAX = 16        # the host-core share of one GPU on the benchmark machines
 METRIC = "collisions/sec + ion histories/sec on 1 MI355X (plain FP64 vs FP32+surrogate)"
 
 
@@ -107,16 +110,33 @@ def _tuned(eng, sample, arith, n_sample):
             "sample_rate": rate}
 
 
-def timed(eng, prob, arith, n_hist, steps):
-    bounds = [n_hist * i // steps for i in range(steps + 1)]
+def step_problem(prob, i, n_hist):
+    """Step i: histories [i n, (i + 1) n) (their own random streams); per-history
+    primaries (the mixed workload) repeat the first n rows."""
+    return prob.batch(0, n_hist).with_(hist0=i * n_hist)
+
+
+def timed(eng, prob, arith, n_hist, steps, first=0):
     coll, kms, res = 0, 0.0, []
     t0 = time.perf_counter()
     for i in range(steps):
-        r = eng.run(prob.with_(hist0=bounds[i], n_hist=bounds[i + 1] - bounds[i]), arith=arith)
+        r = eng.run(step_problem(prob, first + i, n_hist), arith=arith)
         coll += r["n_coll"]
         kms += r["kernel_ms"]
         res.append(r)
     return coll, time.perf_counter() - t0, kms, res
+
+
+def roofline(coll, launches, kernel_ms, flop_eq, peak, pipe, kernel, traffic_bpc=None):
+    """Achieved TFLOP/s of one kernel: algorithmic work per launch (flop-eq
+    per collision x collisions) over the average launch duration."""
+    avg_ms = kernel_ms / launches
+    flop_per_launch = coll * flop_eq / launches
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
+    return {"bound": "valu", "pipe": pipe, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic_bpc * coll / launches if traffic_bpc else None,
+            "kernel": kernel, "launches": launches, "avg_kernel_ms": avg_ms,
+            "flop_eq_per_collision": flop_eq, "flop_eq_per_launch": flop_per_launch}
 
 
 def main():
@@ -124,7 +144,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--histories", type=int, default=1000000)
+    ap.add_argument("--histories", type=int, default=1000000, help="histories per step")
     ap.add_argument("--config", default="c2")
     ap.add_argument("--autotune-sample", type=int, default=10000,
                     help="least histories in the tuning sample (0 = default or given launch shape)")
@@ -164,61 +184,66 @@ def main():
         # 3e9 collisions, at most one timed step: small cascades need more
         # histories to reach the queue occupancy of a full launch, and mixed
         # workloads a sample close to a launch's size
-        pilot = eng.run(prob.with_(hist0=3 * args.histories, n_hist=2000))
+        # tuning histories come after every timed and warm-up step's
+        spare = (args.steps + args.warmup + 2) * args.histories
+        pilot = eng.run(step_problem(prob, 0, 2000).with_(hist0=spare))
         per_hist = max(pilot["n_coll"] / 2000.0, 1.0)
-        n_sample = int(min(max(args.autotune_sample, 3e9 / per_hist), max(args.histories // max(args.steps, 1), 1)))
-        best, rate = eng.autotune(prob.with_(hist0=2 * args.histories, n_hist=n_sample), sample=n_sample)
+        n_sample = int(min(max(args.autotune_sample, 3e9 / per_hist), args.histories))
+        best, rate = eng.autotune(step_problem(prob, 0, n_sample).with_(hist0=spare + 2000), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
                   "sample_histories": n_sample, "sample_rate": rate}
-    wsize = max(1, min(20000, args.histories // max(args.steps, 1)))
+    wsize = max(1, min(20000, args.histories))
     for i in range(args.warmup):
-        eng.run(prob.with_(hist0=args.histories + i * wsize, n_hist=wsize))
+        eng.run(step_problem(prob, 0, wsize).with_(hist0=args.steps * args.histories + i * wsize))
     coll, wall, kernel_ms, steps = timed(eng, prob, gpu.MT_FP64, args.histories, args.steps)
 
     variants = None
     if args.variant_histories > 0:
         nv = min(args.variant_histories, args.histories)
+        w_eq = work_per_collision(args.config)["flop_eq"]
         c64, w64, k64, r64 = timed(eng, prob, gpu.MT_FP64, nv, 1)
         # each precision runs at its own autotuned launch shape (the FP32
         # kernels need fewer registers, so more waves fit), tuned on a sample
         # the size of the variant's own launch
-        sample = prob.with_(hist0=2 * args.histories, n_hist=nv)
+        spare = (args.steps + args.warmup + 4) * args.histories
+        sample = step_problem(prob, 0, nv).with_(hist0=spare)
         t32 = _tuned(eng, sample, gpu.MT_FP32, args.autotune_sample)
         c32, w32, k32, r32 = timed(eng, prob, gpu.MT_FP32, nv, 1)
         sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)
         tsur = _tuned(eng, sample, gpu.MT_FP32_SURROGATE, args.autotune_sample)
         cs, ws, ks, rs = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
+        pipe32 = "FP32 VALU, packed FP32 disabled (one FMA per lane per clock: the FP64 rate)"
         variants = {
             "histories": nv,
-            "fp64": {"collisions_per_s": c64 / w64, "kernel_ms": k64},
+            "fp64": {"collisions_per_s": c64 / w64, "kernel_ms": k64,
+                     "roofline": roofline(c64, 1, k64, w_eq, FP64_PEAK_TFLOPS, "FP64 VALU",
+                                          "cascade_kernel<0,0,0,false>")},
             "fp32": {"collisions_per_s": c32 / w32, "kernel_ms": k32, "speedup_vs_fp64": (c32 / w32) / (c64 / w64),
-                     "tuning": t32, "tally_rel_err_vs_fp64": tally_errors(r64[0], r32[0])},
+                     "tuning": t32, "tally_rel_err_vs_fp64": tally_errors(r64[0], r32[0]),
+                     "roofline": roofline(c32, 1, k32, w_eq, FP32_UNPACKED_PEAK_TFLOPS, pipe32,
+                                          "cascade_kernel<1,0,0,false>")},
             "fp32_surrogate": {"collisions_per_s": cs / ws, "kernel_ms": ks, "tuning": tsur,
                                "speedup_vs_fp64": (cs / ws) / (c64 / w64),
                                "surrogate_share": rs[0]["surrogate"] / max(rs[0]["n_coll"], 1),
                                "tally_rel_err_vs_fp64": tally_errors(r64[0], rs[0]),
+                               "roofline": roofline(cs, 1, ks, w_eq, FP32_UNPACKED_PEAK_TFLOPS, pipe32,
+                                                    "cascade_kernel<2,0,0,false>"),
                                "training": sur},
         }
     eng.close()
 
     w = work_per_collision(args.config)
-    n_launch = len(steps)
-    avg_ms = kernel_ms / n_launch
-    flop_per_launch = coll * w["flop_eq"] / n_launch
-    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
-    traffic = None
+    bpc = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
     if os.path.exists(tpath):
         with open(tpath) as f:
             bpc = json.load(f).get("bytes_per_collision")
-        if bpc:
-            traffic = bpc * coll / n_launch
     s0 = steps[0]
     out = {
         "metric": METRIC,
         "value": coll / wall,
         "unit": "collisions/s",
-        "histories_per_s": args.histories / wall,
+        "histories_per_s": args.histories * args.steps / wall,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -228,26 +253,30 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": prob.name, "histories": args.histories,
+        "config": {"workload": prob.name, "histories_per_step": args.histories,
+                   "histories": args.histories * args.steps,
                    "cascades": "full (every recoil followed)", "arith": "FP64",
-                   "collisions": coll, "collisions_per_history": coll / args.histories,
+                   "collisions": coll, "collisions_per_history": coll / (args.histories * args.steps),
                    "grid": s0["grid"], "block": s0["block"], "waves": s0["waves"],
                    "queue_peak": max(r["queue_peak"] for r in steps), "tuning": tuning},
-        "roofline": {"bound": "mfma", "pipe": "FP64 VALU (the FP64 vector and matrix peaks are equal)",
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "cascade_kernel<0,0,0,false>", "launches": n_launch,
-                     "avg_kernel_ms": avg_ms, "flop_eq_per_collision": w["flop_eq"],
-                     "flop_eq_per_launch": flop_per_launch},
+        "roofline": roofline(coll, len(steps), kernel_ms, w["flop_eq"], FP64_PEAK_TFLOPS,
+                             "FP64 VALU (no matrix work: the kernel issues no MFMA)",
+                             "cascade_kernel<0,0,0,false>", bpc),
         "variants": variants,
     }
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # one thread per core of the GPU's host-core share (at most 16 worker
+        # threads per GPU on the benchmark machines, whatever os.cpu_count()
+        # reports there)
+        threads = min(CPU_CORES_MAX, os.cpu_count() or 1)
         n_cpu = args.cpu_histories
         if n_cpu <= 0:
             # about 15 s of CPU work at ~2.6e6 collisions/s per core
-            n_cpu = max(16, int(15.0 * 2.6e6 * threads / max(coll / args.histories, 1.0)))
+            n_cpu = max(16, int(15.0 * 2.6e6 * threads / max(coll / (args.histories * args.steps), 1.0)))
         out["cpu_baseline"] = cpu_baseline(prob, n_cpu, threads)
+        out["cpu_baseline"]["os_cpu_count"] = os.cpu_count()
+        out["cpu_baseline"]["cores_note"] = ("one std::thread per core, capped at the %d host cores one GPU may use "
+                                             "on the benchmark machines" % CPU_CORES_MAX)
     print(json.dumps(out), flush=True)
 
 

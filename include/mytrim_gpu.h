@@ -24,6 +24,8 @@
  *                                       can replay the virtual hooks
  *   mt_stopping_batch                   MaterialBase::getrstop (material.C:114-122)
  *   mt_scatter_batch                    the scattering block of trim.C:172-272
+ *   mt_stopping_batch_arith,            the same two in the arithmetic of a
+ *   mt_scatter_batch_arith              cascade precision mode
  */
 #ifndef MYTRIM_GPU_H
 #define MYTRIM_GPU_H
@@ -42,7 +44,9 @@ enum {
   MT_E_STATE = -3,    /* call out of order (e.g. no target set) */
   MT_E_DEVICE = -4,   /* device-side failure flag (queue overflow, runaway loop, NaN) */
   MT_E_NOMEM = -5,
-  MT_E_LIMIT = -6     /* input exceeds a documented limit */
+  MT_E_LIMIT = -6,    /* input exceeds a documented limit */
+  MT_E_MORE = -7      /* mt_trim_flight: the uniform stream or the event
+                         buffer ran out; call again with more */
 };
 
 /* boundary conditions (SampleBase::sampleBoundary, sample.h:49-54) */
@@ -266,9 +270,11 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, 
                 int reps, mt_tuning* best, double* best_rate);
 
 /* Single flight with the caller's uniform stream u[0..n_u): consumes draws in
- * the reference's order (trim.C:71, 143, 147, 322-333).  Returns MT_E_LIMIT
- * with *n_used = n_u when the stream ran out (call again with more).  The ion
- * is updated in place; events[0..*n_events) hold every collision. */
+ * the reference's order (trim.C:71, 143, 147, 322-333).  Returns MT_E_MORE
+ * when the stream ran out (*n_used = n_u) or the event buffer filled
+ * (*n_events = max_events): call again with more.  MT_E_DEVICE for a runaway
+ * flight or a non-finite energy or position.  The ion is updated in place;
+ * events[0..*n_events) hold every collision. */
 int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, long long* n_used,
                    mt_event* events, int max_events, int* n_events);
 
@@ -296,6 +302,18 @@ int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, 
  * st = sin(theta), iters = Newton iterations (0: Rutherford branch). */
 int mt_scatter_batch(mt_ctx* ctx, int potential, const double* eps, const double* b, long long n,
                      double* s2, double* ct, double* st, int* iters);
+
+/* The two batches above in the arithmetic of a cascade precision mode
+ * (MT_FP64, MT_FP32, MT_FP32_SURROGATE): each runs the device function the
+ * cascade kernel of that mode evaluates (getrstop of material.C:114-122 and
+ * the scattering of trim.C:172-272 in single precision; with
+ * MT_FP32_SURROGATE the installed network answers inside its domain for
+ * eps <= 10, with iters = -1 there), for per-function tolerance checks.
+ * Inputs are rounded to single precision first in the FP32 modes. */
+int mt_stopping_batch_arith(mt_ctx* ctx, int arith, int Z1, double m1, int mat, const double* E, long long n,
+                            double* se_out);
+int mt_scatter_batch_arith(mt_ctx* ctx, int arith, int potential, const double* eps, const double* b,
+                           long long n, double* s2, double* ct, double* st, int* iters);
 
 #ifdef __cplusplus
 }

@@ -1,13 +1,20 @@
 // Per-function batch kernels: electronic stopping (MaterialBase::getrstop,
 // material.C:114-122) over an energy array, and the scattering block of
 // trim.C:172-272 over (eps, b) pairs.  One lane per element, grid-stride.
+// ARITH selects the device functions of a cascade precision mode: 0 the FP64
+// twins of the reference, 1 the single-precision ones, 2 the single-precision
+// ones with the learned scattering map in front (the stopping of modes 1 and
+// 2 is the same).  Inputs and outputs are doubles in every mode.
 #include <hip/hip_runtime.h>
 
+#include "mt_mlp.h"
 #include "mt_physics.h"
+#include "mt_scatter32.h"
 #include "mt_tables.h"
 
 namespace mt {
 
+template <int ARITH>
 __global__ void __launch_bounds__(256) stopping_kernel(const unsigned char* blob, int s, int k, const double* E,
                                                        long long n, double* out) {
   const TableHeader* H = (const TableHeader*)blob;
@@ -19,20 +26,35 @@ __global__ void __launch_bounds__(256) stopping_kernel(const unsigned char* blob
   T.mat = (const MatConst*)(blob + H->off_mat);
   T.el = (const ElemConst*)(blob + H->off_el);
   T.stop = (const StopConst*)(blob + H->off_stop);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    out[i] = getrstop<double>(T, s, k, E[i]);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    if constexpr (ARITH == 0) {
+      out[i] = getrstop<double>(T, s, k, E[i]);
+    } else {
+      const float Ef = (float)E[i];
+      out[i] = (double)getrstop<float>(T, s, k, Ef, m_log(Ef));
+    }
+  }
 }
 
-template <int POT>
-__global__ void __launch_bounds__(256) scatter_kernel(const double* eps, const double* b, long long n, double* s2,
-                                                      double* ct, double* st, int* iters) {
+template <int ARITH, int POT>
+__global__ void __launch_bounds__(256) scatter_kernel(MlpDev mlp, const double* eps, const double* b, long long n,
+                                                      double* s2, double* ct, double* st, int* iters) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double a, c, d;
     int it;
-    scatter<double, POT>(eps[i], b[i], a, c, d, it);
-    s2[i] = a;
-    ct[i] = c;
-    st[i] = d;
+    if constexpr (ARITH == 0) {
+      double a, c, d;
+      scatter<double, POT>(eps[i], b[i], a, c, d, it);
+      s2[i] = a;
+      ct[i] = c;
+      st[i] = d;
+    } else {
+      const float ef = (float)eps[i], bf = (float)b[i];
+      float a, c, d;
+      scatter_single<ARITH, POT>(mlp, ef, bf, m_log(ef), a, c, d, it);
+      s2[i] = a;
+      ct[i] = c;
+      st[i] = d;
+    }
     iters[i] = it;
   }
 }
@@ -42,20 +64,37 @@ static int grid_for(long long n) {
   return (int)(g < 65536 ? (g > 0 ? g : 1) : 65536);
 }
 
-hipError_t stopping_launch(const unsigned char* tables, int species, int mat, const double* E, long long n,
+hipError_t stopping_launch(int arith, const unsigned char* tables, int species, int mat, const double* E, long long n,
                            double* out, hipStream_t stream) {
-  hipLaunchKernelGGL(stopping_kernel, dim3(grid_for(n)), dim3(256), 0, stream, tables, species, mat, E, n, out);
+  if (arith == 0)
+    hipLaunchKernelGGL(stopping_kernel<0>, dim3(grid_for(n)), dim3(256), 0, stream, tables, species, mat, E, n, out);
+  else
+    hipLaunchKernelGGL(stopping_kernel<1>, dim3(grid_for(n)), dim3(256), 0, stream, tables, species, mat, E, n, out);
   return hipGetLastError();
 }
 
-hipError_t scatter_launch(int potential, const double* eps, const double* b, long long n, double* s2, double* ct,
-                          double* st, int* iters, hipStream_t stream) {
+template <int ARITH>
+static void scatter_launch_a(int potential, const MlpDev& mlp, const double* eps, const double* b, long long n,
+                             double* s2, double* ct, double* st, int* iters, hipStream_t stream) {
   if (potential == 1)
-    hipLaunchKernelGGL(scatter_kernel<1>, dim3(grid_for(n)), dim3(256), 0, stream, eps, b, n, s2, ct, st, iters);
+    hipLaunchKernelGGL((scatter_kernel<ARITH, 1>), dim3(grid_for(n)), dim3(256), 0, stream, mlp, eps, b, n, s2, ct,
+                       st, iters);
   else if (potential == 2)
-    hipLaunchKernelGGL(scatter_kernel<2>, dim3(grid_for(n)), dim3(256), 0, stream, eps, b, n, s2, ct, st, iters);
+    hipLaunchKernelGGL((scatter_kernel<ARITH, 2>), dim3(grid_for(n)), dim3(256), 0, stream, mlp, eps, b, n, s2, ct,
+                       st, iters);
   else
-    hipLaunchKernelGGL(scatter_kernel<0>, dim3(grid_for(n)), dim3(256), 0, stream, eps, b, n, s2, ct, st, iters);
+    hipLaunchKernelGGL((scatter_kernel<ARITH, 0>), dim3(grid_for(n)), dim3(256), 0, stream, mlp, eps, b, n, s2, ct,
+                       st, iters);
+}
+
+hipError_t scatter_launch(int arith, int potential, const MlpDev& mlp, const double* eps, const double* b, long long n,
+                          double* s2, double* ct, double* st, int* iters, hipStream_t stream) {
+  if (arith == 0)
+    scatter_launch_a<0>(potential, mlp, eps, b, n, s2, ct, st, iters, stream);
+  else if (arith == 1)
+    scatter_launch_a<1>(potential, mlp, eps, b, n, s2, ct, st, iters, stream);
+  else
+    scatter_launch_a<2>(0, mlp, eps, b, n, s2, ct, st, iters, stream);
   return hipGetLastError();
 }
 

@@ -27,10 +27,10 @@ const void* zero_spans_function();
 const void* flush_energy_function();
 hipError_t zero_spans_launch(const ZeroSpans& z, hipStream_t st);
 hipError_t flush_energy_launch(const double* we, int waves, double* out, hipStream_t st);
-hipError_t stopping_launch(const unsigned char* tables, int species, int mat, const double* E, long long n,
+hipError_t stopping_launch(int arith, const unsigned char* tables, int species, int mat, const double* E, long long n,
                            double* out, hipStream_t stream);
-hipError_t scatter_launch(int potential, const double* eps, const double* b, long long n, double* s2,
-                          double* ct, double* st, int* iters, hipStream_t stream);
+hipError_t scatter_launch(int arith, int potential, const MlpDev& mlp, const double* eps, const double* b, long long n,
+                          double* s2, double* ct, double* st, int* iters, hipStream_t stream);
 struct FlightParams {
   const unsigned char* tables;
   Geometry geo;
@@ -158,6 +158,7 @@ uint64_t next_pow2(uint64_t x) {
 
 }  // namespace
 
+
 struct mt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -269,6 +270,9 @@ int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols) {
     for (int j = 0; j < 8; ++j) c.pcoef[j] = v[7 + j];
     c.lfctr = v[15];
   }
+  // single-flight tables are cached per ion; the next flight rebuilds them
+  for (auto& kv : ctx->flight_tables) kv.second->release();
+  ctx->flight_tables.clear();
   return MT_OK;
 }
 
@@ -500,28 +504,33 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
 
 static_assert(sizeof(LogRecord) == sizeof(mt_record) && sizeof(mt_record) == 80, "record layout");
 
-static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_log* lg, mt_tallies* out) {
+// ------------------------------------------------------------ run pieces
+// (shared by mt_run_cascades and mt_run_cascades_log)
+
+// bytes per slot of each queue array (px py pz dx dy dz E key mt: 8, hist: 4, ready: 8)
+constexpr size_t kQueueFieldBytes[kQueueArrays] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 8};
+
+static int validate_run(const mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, const mt_tallies* out) {
   if (!ctx || !pk || !o || !out) return fail(MT_E_ARG, "NULL argument");
   if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
   if (pk->n_hist < 0 || pk->n_hist > (1ll << 31)) return fail(MT_E_LIMIT, "n_hist out of range");
   if (pk->hist0 < 0 || pk->hist0 + pk->n_hist > (1ll << 32)) return fail(MT_E_LIMIT, "history index out of range");
+  if (!pk->Z || !pk->m || !pk->E || !pk->pos || !pk->dir) return fail(MT_E_ARG, "NULL primaries array");
   if (o->arith < MT_FP64 || o->arith > MT_FP32_SURROGATE) return fail(MT_E_ARG, "unknown arithmetic mode");
   if (o->arith == MT_FP32_SURROGATE && !ctx->mlp_ready) return fail(MT_E_STATE, "no surrogate installed");
   if (o->arith == MT_FP32_SURROGATE && ctx->tgt.potential != MT_UNIVERSAL)
     return fail(MT_E_ARG, "the surrogate is trained on the universal potential");
   if (o->range_nb < 1 || o->lat_nb < 1 || o->vac_nb < 1) return fail(MT_E_ARG, "histograms need >= 1 bin");
   if (!(o->range_bw > 0.0) || !(o->lat_bw > 0.0) || !(o->vac_bw > 0.0)) return fail(MT_E_ARG, "bin widths must be positive");
-  HIPCHECK(hipSetDevice(ctx->device));
-  const long long n_hist = pk->n_hist;
-  const long long n_entries = pk->pka_uniform ? 1 : n_hist;
-  std::vector<Species> species;
-  std::vector<int> pka_species;
-  std::string msg = collect_species(ctx->tgt, pk->Z, pk->m, n_entries, species, pka_species);
-  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
-  std::vector<unsigned char> blob;
-  msg = build_tables(ctx->coef, ctx->tgt, species, blob);
-  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
-  double E0 = 0.0;
+  return MT_OK;
+}
+
+// finite positive energies and positions, non-zero directions; the batch's
+// total and largest primary energy
+static int check_primaries(const mt_primaries* pk, double& E0, double& e_max) {
+  const long long n_entries = pk->pka_uniform ? 1 : pk->n_hist;
+  E0 = 0.0;
+  e_max = 0.0;
   for (long long i = 0; i < n_entries; ++i) {
     if (!(pk->E[i] > 0.0) || !std::isfinite(pk->E[i])) return fail(MT_E_ARG, "primary energy must be positive and finite");
     const double* q = pk->pos + 3 * i;
@@ -531,8 +540,206 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
     const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     if (!(dd > 0.0) || !std::isfinite(dd)) return fail(MT_E_ARG, "primary direction must be finite and non-zero");
     E0 += pk->E[i];
+    e_max = std::max(e_max, pk->E[i]);
   }
-  if (pk->pka_uniform) E0 = pk->E[0] * double(n_hist);
+  if (pk->pka_uniform) E0 = pk->E[0] * double(pk->n_hist);
+  return MT_OK;
+}
+
+static bool perhist_of(const mt_run_opts* o, const mt_log* lg) { return o->per_history != 0 || (lg && lg->mask != 0); }
+
+struct RunShape {
+  int block = 0, cap = 0, grid = 0, waves = 0, lds_hist = 0;
+  size_t lds = 0;
+};
+
+// launch shape of the persistent kernel: the context's tuning, else the shape
+// measured best for Cu -> Cu cascades (two waves per SIMD, 128-entry LDS
+// stacks); depth bins in LDS where they fit
+static int run_shape(const mt_ctx* ctx, const mt_run_opts* o, size_t table_bytes, bool perhist, RunShape& s) {
+  s.block = ctx->tune.block ? ctx->tune.block : 256;
+  s.cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 128;
+  const int nwb = s.block / 64;
+  const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
+  s.lds = table_bytes + size_t(nwb) * cascade_stack_bytes(s.cap);
+  s.lds_hist = ctx->tune.lds_hist >= 0 ? 1 : 0;
+  const size_t lds_max = size_t(ctx->lds_max);
+  if (s.lds + lds_hist_bytes > lds_max) s.lds_hist = 0;
+  if (s.lds_hist) s.lds += lds_hist_bytes;
+  if (s.lds > lds_max) return fail(MT_E_LIMIT, "tables and stacks exceed the LDS of a workgroup");
+  int bpc = 0;
+  HIPCHECK(cascade_occupancy(o->arith, ctx->tgt.potential, ctx->tgt.geo.kind, perhist, s.block, s.lds, &bpc));
+  bpc = std::min(bpc, ctx->tune.blocks_per_cu > 0 ? ctx->tune.blocks_per_cu : kDefaultWavesPerCu * 64 / s.block);
+  if (bpc < 1) return fail(MT_E_LIMIT, "cascade kernel does not fit on a CU");
+  s.grid = bpc * ctx->cus;
+  s.waves = s.grid * nwb;
+  if (s.waves > 0xffff) return fail(MT_E_LIMIT, "too many waves for the idle counter");
+  return MT_OK;
+}
+
+// flights per cascade before any has completed: the Kinchin-Pease count
+// E / (2 Ed) of the most energetic primary against the lowest threshold
+static double flights_per_cascade0(const mt_ctx* ctx, double e_max) {
+  double ed_min = 1e300;
+  for (const HostMaterial& m : ctx->tgt.mat)
+    for (const HostElement& e : m.el) ed_min = std::min(ed_min, e.Edisp);
+  return 1.0 + e_max / (2.0 * std::max(ed_min, 1.0));
+}
+
+// device memory of one run
+struct RunMem {
+  const unsigned char* tables = nullptr;
+  int table_bytes = 0;
+  const int* pka_sp = nullptr;
+  const double *pka_E = nullptr, *pka_pos = nullptr, *pka_dir = nullptr;
+  void* q[kQueueArrays] = {};
+  uint64_t qcap = 0;
+  unsigned long long* res = nullptr;   // kResWords: control words, counters, energies
+  unsigned long long* hist = nullptr;  // the five histograms, at hoff[0..4]
+  double* wave_energy = nullptr;
+  int* h_out = nullptr;
+};
+
+// the kernel parameters every run shares (per-history arrays, event streams
+// and the surrogate are set by the caller)
+static void fill_params(Params& P, const mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o,
+                        const RunMem& M, const RunShape& sh, double f_init, const size_t* hoff) {
+  std::memset(&P, 0, sizeof(P));
+  P.tables = M.tables;
+  P.table_bytes = M.table_bytes;
+  P.stack_cap = sh.cap;
+  P.geo = ctx->tgt.geo;
+  P.tau = ctx->tgt.tau;
+  P.cw = ctx->tgt.cw;
+  P.invscale = 1.0 / ctx->tgt.scale;
+  P.Ef = pk->Ef;
+  P.follow_max_gen = o->follow_max_gen;
+  P.kp = o->kp;
+  P.n_hist = pk->n_hist;
+  P.hist0 = pk->hist0;
+  P.pka_uniform = pk->pka_uniform ? 1 : 0;
+  P.pka_species = M.pka_sp;
+  P.pka_E = M.pka_E;
+  P.pka_pos = M.pka_pos;
+  P.pka_dir = M.pka_dir;
+  P.seed = pk->seed;
+  P.ctrl = M.res + kResCtrl;
+  Queue& Q = P.q;
+  Q.px = static_cast<uint64_t*>(M.q[0]);
+  Q.py = static_cast<uint64_t*>(M.q[1]);
+  Q.pz = static_cast<uint64_t*>(M.q[2]);
+  Q.dx = static_cast<uint64_t*>(M.q[3]);
+  Q.dy = static_cast<uint64_t*>(M.q[4]);
+  Q.dz = static_cast<uint64_t*>(M.q[5]);
+  Q.E = static_cast<uint64_t*>(M.q[6]);
+  Q.key = static_cast<uint64_t*>(M.q[7]);
+  Q.mt = static_cast<uint64_t*>(M.q[8]);
+  Q.hist = static_cast<uint32_t*>(M.q[9]);
+  Q.ready = static_cast<uint64_t*>(M.q[kQueueReady]);
+  // shards of at least 2^12 slots, up to 16
+  int log2cap = 0;
+  while ((uint64_t(1) << log2cap) < M.qcap) ++log2cap;
+  int nq = kMaxShards;
+  while (nq > 1 && (M.qcap / uint64_t(nq)) < 4096) nq >>= 1;
+  int lq = 0;
+  while ((1 << lq) < nq) ++lq;
+  Q.nq = nq;
+  Q.sub_log2 = log2cap - lq;
+  Q.sub_mask = (uint64_t(1) << Q.sub_log2) - 1;
+  P.admit_limit = (long long)(M.qcap / 4);
+  P.budget = double(M.qcap) / 2.0;
+  P.f_init = f_init;
+  P.c_floor = sh.waves;
+  P.h_out = M.h_out;
+  P.max_ic = 10000000;
+  P.max_wait_ticks = (unsigned long long)(60.0 * ctx->clock_khz * 1000.0);
+  P.counters = M.res + kResCounters;
+  P.wave_energy = M.wave_energy;
+  P.energy = reinterpret_cast<double*>(M.res + kResEnergy);
+  P.range_hist = M.hist + hoff[0];
+  P.lat_y_hist = M.hist + hoff[1];
+  P.lat_z_hist = M.hist + hoff[2];
+  P.vac_hist = M.hist + hoff[3];
+  P.repl_hist = M.hist + hoff[4];
+  P.range_x0 = o->range_x0;
+  P.range_bw = o->range_bw;
+  P.range_nb = o->range_nb;
+  P.lat_y0 = o->lat_y0;
+  P.lat_z0 = o->lat_z0;
+  P.lat_bw = o->lat_bw;
+  P.lat_nb = o->lat_nb;
+  P.vac_x0 = o->vac_x0;
+  P.vac_bw = o->vac_bw;
+  P.vac_nb = o->vac_nb;
+  P.lds_hist = sh.lds_hist;
+  if (o->arith == MT_FP32_SURROGATE) {
+    P.mlp.p = ctx->mlp_p.as<float>();
+    P.mlp.w2bf = ctx->mlp_w2bf.as<uint32_t>();
+    P.mlp.n = ctx->mlp_norm;
+    P.mlp.bf16 = ctx->mlp_bf16;
+  }
+}
+
+// counters, energies and queue figures of a finished run's result block
+static void read_results(const unsigned long long* sr, double E0, mt_tallies* out) {
+  const unsigned long long* ctrl = sr + kResCtrl;
+  const unsigned long long* cnt = sr + kResCounters;
+  const double* en = reinterpret_cast<const double*>(sr + kResEnergy);
+  const double* rsum = reinterpret_cast<const double*>(sr + kResReduced);
+  out->n_coll = (long long)cnt[cColl];
+  out->n_flight = (long long)cnt[cFlight];
+  out->vac = (long long)cnt[cVac];
+  out->repl = (long long)cnt[cRepl];
+  out->subst = (long long)cnt[cSubst];
+  out->inter = (long long)cnt[cInter];
+  out->lost = (long long)cnt[cLost];
+  out->vacuum = (long long)cnt[cVacuum];
+  out->rutherford = (long long)cnt[cRuth];
+  out->newton = (long long)cnt[cNewton];
+  out->disp = (long long)cnt[cDisp];
+  out->range_n = (long long)cnt[cRangeN];
+  out->surrogate = (long long)cnt[cSurr];
+  out->E0 = E0;
+  out->Eel = rsum[0] + rsum[1];
+  out->Enuc = rsum[2] + rsum[3];
+  out->Eunf = en[eEunf];
+  out->Elost = en[eElost];
+  out->Edrop = en[eEdrop];
+  out->Evac = en[eEvac];
+  out->range_sum = en[eRangeSum];
+  out->range_sum2 = en[eRangeSum2];
+  out->queue_peak = (long long)ctrl[kQueuePeak];
+  out->queue_ops = (long long)ctrl[kQueueOps];
+}
+
+// MT_E_DEVICE with the names of the device error flags, if any were raised
+static int device_errors(const unsigned long long* ctrl) {
+  const unsigned errbits = unsigned(ctrl[kError]);
+  if (!errbits) return MT_OK;
+  std::string m = "device error flags:";
+  if (errbits & kErrQueueFull) m += " queue-full";
+  if (errbits & kErrRunaway) m += " runaway-flight";
+  if (errbits & kErrBadEntry) m += " bad-queue-entry";
+  if (errbits & kErrTimeout) m += " wait-timeout";
+  if (errbits & kErrNewton) m += " newton-cap";
+  if (errbits & kErrNonFinite) m += " non-finite-energy";
+  return fail(MT_E_DEVICE, m);
+}
+
+static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_log* lg, mt_tallies* out) {
+  if (int rc = validate_run(ctx, pk, o, out)) return rc;
+  HIPCHECK(hipSetDevice(ctx->device));
+  const long long n_hist = pk->n_hist;
+  const long long n_entries = pk->pka_uniform ? 1 : n_hist;
+  double E0 = 0.0, e_max = 0.0;
+  if (int rc = check_primaries(pk, E0, e_max)) return rc;
+  std::vector<Species> species;
+  std::vector<int> pka_species;
+  std::string msg = collect_species(ctx->tgt, pk->Z, pk->m, n_entries, species, pka_species);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  std::vector<unsigned char> blob;
+  msg = build_tables(ctx->coef, ctx->tgt, species, blob);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
 
   hipStream_t st = ctx->stream;
   HIPCHECK(ctx->tables.ensure(blob.size()));
@@ -547,39 +754,14 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
   HIPCHECK(hipMemcpyAsync(ctx->pka_pos.p, pk->pos, 3 * n_entries * sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(ctx->pka_dir.p, pk->dir, 3 * n_entries * sizeof(double), hipMemcpyHostToDevice, st));
 
-  // launch shape
-  // defaults: the shape measured best for Cu -> Cu cascades (two waves per
-  // SIMD, 128-entry LDS stacks)
-  const int block = ctx->tune.block ? ctx->tune.block : 256;
-  const int cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 128;
-  const int nwb = block / 64;
-  const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
-  size_t lds = blob.size() + size_t(nwb) * cascade_stack_bytes(cap);
-  int lds_hist = ctx->tune.lds_hist >= 0 ? 1 : 0;
-  const size_t lds_max = size_t(ctx->lds_max);
-  if (lds + lds_hist_bytes > lds_max) lds_hist = 0;
-  if (lds_hist) lds += lds_hist_bytes;
-  if (lds > lds_max) return fail(MT_E_LIMIT, "tables and stacks exceed the LDS of a workgroup");
+  RunShape sh;
+  if (int rc = run_shape(ctx, o, blob.size(), perhist_of(o, lg), sh)) return rc;
+  const int grid = sh.grid, block = sh.block, waves = sh.waves;
+  const size_t lds = sh.lds;
   const int pot = ctx->tgt.potential;
-  // event streams run in the per-history kernel
   const bool log_on = lg && lg->mask != 0;
-  const bool perhist = o->per_history != 0 || log_on;
-  int bpc = 0;
-  HIPCHECK(cascade_occupancy(o->arith, pot, ctx->tgt.geo.kind, perhist, block, lds, &bpc));
-  bpc = std::min(bpc, ctx->tune.blocks_per_cu > 0 ? ctx->tune.blocks_per_cu : kDefaultWavesPerCu * 64 / block);
-  if (bpc < 1) return fail(MT_E_LIMIT, "cascade kernel does not fit on a CU");
-  const int grid = bpc * ctx->cus;
-  const int waves = grid * nwb;
-  if (waves > 0xffff) return fail(MT_E_LIMIT, "too many waves for the idle counter");
-
-  // flights per cascade before any has completed: the Kinchin-Pease count
-  // E / (2 Ed) of the most energetic primary against the lowest threshold
-  double ed_min = 1e300;
-  for (const HostMaterial& m : ctx->tgt.mat)
-    for (const HostElement& e : m.el) ed_min = std::min(ed_min, e.Edisp);
-  double e_max = 0.0;
-  for (long long i = 0; i < n_entries; ++i) e_max = std::max(e_max, pk->E[i]);
-  const double f_init = 1.0 + e_max / (2.0 * std::max(ed_min, 1.0));
+  const bool perhist = perhist_of(o, lg);
+  const double f_init = flights_per_cascade0(ctx, e_max);
 
   // queue: room for every flight of the batch up to 2^27 entries (80 B each)
   const double want = std::max(double(n_hist), 1.0) * f_init;
@@ -587,13 +769,9 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
                       ? uint64_t(ctx->tune.queue_cap)
                       : next_pow2(uint64_t(std::min(std::max(want, 65536.0), double(uint64_t(1) << 27))));
   if (ctx->qcap != qcap) {
-    const size_t sz[kQueueArrays] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 8};
-    for (int i = 0; i < kQueueArrays; ++i) HIPCHECK(ctx->q[i].ensure(qcap * sz[i]));
+    for (int i = 0; i < kQueueArrays; ++i) HIPCHECK(ctx->q[i].ensure(qcap * kQueueFieldBytes[i]));
     ctx->qcap = qcap;
   }
-  int log2cap = 0;
-  while ((uint64_t(1) << log2cap) < qcap) ++log2cap;
-
   // tallies
   // control words, counters, energies and the flushed energy sums share one
   // allocation (res); the five histograms share another (hist_all); both are
@@ -628,75 +806,22 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
     }
   }
 
+  RunMem M;
+  M.tables = ctx->tables.as<unsigned char>();
+  M.table_bytes = int(blob.size());
+  M.pka_sp = ctx->pka_sp.as<int>();
+  M.pka_E = ctx->pka_E.as<double>();
+  M.pka_pos = ctx->pka_pos.as<double>();
+  M.pka_dir = ctx->pka_dir.as<double>();
+  for (int i = 0; i < kQueueArrays; ++i) M.q[i] = ctx->q[i].p;
+  M.qcap = qcap;
+  M.res = ctx->res.as<unsigned long long>();
+  M.hist = ctx->hist_all.as<unsigned long long>();
+  M.wave_energy = ctx->wave_energy.as<double>();
+  M.h_out = ctx->h_out.as<int>();
   Params P;
-  std::memset(&P, 0, sizeof(P));
-  P.tables = ctx->tables.as<unsigned char>();
-  P.table_bytes = int(blob.size());
-  P.stack_cap = cap;
-  P.geo = ctx->tgt.geo;
-  P.tau = ctx->tgt.tau;
-  P.cw = ctx->tgt.cw;
-  P.invscale = 1.0 / ctx->tgt.scale;
-  P.Ef = pk->Ef;
-  P.follow_max_gen = o->follow_max_gen;
-  P.kp = o->kp;
-  P.n_hist = n_hist;
-  P.hist0 = pk->hist0;
-  P.pka_uniform = pk->pka_uniform ? 1 : 0;
-  P.pka_species = ctx->pka_sp.as<int>();
-  P.pka_E = ctx->pka_E.as<double>();
-  P.pka_pos = ctx->pka_pos.as<double>();
-  P.pka_dir = ctx->pka_dir.as<double>();
-  P.seed = pk->seed;
-  unsigned long long* res = ctx->res.as<unsigned long long>();
-  P.ctrl = res + kResCtrl;
-  Queue& Q = P.q;
-  Q.px = ctx->q[0].as<uint64_t>();
-  Q.py = ctx->q[1].as<uint64_t>();
-  Q.pz = ctx->q[2].as<uint64_t>();
-  Q.dx = ctx->q[3].as<uint64_t>();
-  Q.dy = ctx->q[4].as<uint64_t>();
-  Q.dz = ctx->q[5].as<uint64_t>();
-  Q.E = ctx->q[6].as<uint64_t>();
-  Q.key = ctx->q[7].as<uint64_t>();
-  Q.mt = ctx->q[8].as<uint64_t>();
-  Q.hist = ctx->q[9].as<uint32_t>();
-  Q.ready = ctx->q[kQueueReady].as<uint64_t>();
-  // shards of at least 2^12 slots, up to 16
-  int nq = kMaxShards;
-  while (nq > 1 && (qcap / uint64_t(nq)) < 4096) nq >>= 1;
-  int lq = 0;
-  while ((1 << lq) < nq) ++lq;
-  Q.nq = nq;
-  Q.sub_log2 = log2cap - lq;
-  Q.sub_mask = (uint64_t(1) << Q.sub_log2) - 1;
-  P.admit_limit = (long long)(qcap / 4);
-  P.budget = double(qcap) / 2.0;
-  P.f_init = f_init;
-  P.c_floor = waves;
-  P.h_out = ctx->h_out.as<int>();
-  P.max_ic = 10000000;
-  P.max_wait_ticks = (unsigned long long)(60.0 * ctx->clock_khz * 1000.0);
-  P.counters = res + kResCounters;
-  P.wave_energy = ctx->wave_energy.as<double>();
-  P.energy = reinterpret_cast<double*>(res + kResEnergy);
-  unsigned long long* hists = ctx->hist_all.as<unsigned long long>();
-  P.range_hist = hists + hoff[0];
-  P.lat_y_hist = hists + hoff[1];
-  P.lat_z_hist = hists + hoff[2];
-  P.vac_hist = hists + hoff[3];
-  P.repl_hist = hists + hoff[4];
-  P.range_x0 = o->range_x0;
-  P.range_bw = o->range_bw;
-  P.range_nb = o->range_nb;
-  P.lat_y0 = o->lat_y0;
-  P.lat_z0 = o->lat_z0;
-  P.lat_bw = o->lat_bw;
-  P.lat_nb = o->lat_nb;
-  P.vac_x0 = o->vac_x0;
-  P.vac_bw = o->vac_bw;
-  P.vac_nb = o->vac_nb;
-  P.lds_hist = lds_hist;
+  fill_params(P, ctx, pk, o, M, sh, f_init, hoff);
+  unsigned long long* res = M.res;
   if (perhist) {
     P.h_vac = ctx->per[0].as<int>();
     P.h_coll = ctx->per[1].as<unsigned long long>();
@@ -801,32 +926,7 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
 
   const unsigned long long* sr = static_cast<const unsigned long long*>(ctx->stage_res.p);
   const unsigned long long* ctrl = sr + kResCtrl;
-  const unsigned long long* cnt = sr + kResCounters;
-  const double* en = reinterpret_cast<const double*>(sr + kResEnergy);
-  const double* rsum = reinterpret_cast<const double*>(sr + kResReduced);
-
-  out->n_coll = (long long)cnt[cColl];
-  out->n_flight = (long long)cnt[cFlight];
-  out->vac = (long long)cnt[cVac];
-  out->repl = (long long)cnt[cRepl];
-  out->subst = (long long)cnt[cSubst];
-  out->inter = (long long)cnt[cInter];
-  out->lost = (long long)cnt[cLost];
-  out->vacuum = (long long)cnt[cVacuum];
-  out->rutherford = (long long)cnt[cRuth];
-  out->newton = (long long)cnt[cNewton];
-  out->disp = (long long)cnt[cDisp];
-  out->range_n = (long long)cnt[cRangeN];
-  out->surrogate = (long long)cnt[cSurr];
-  out->E0 = E0;
-  out->Eel = rsum[0] + rsum[1];
-  out->Enuc = rsum[2] + rsum[3];
-  out->Eunf = en[eEunf];
-  out->Elost = en[eElost];
-  out->Edrop = en[eEdrop];
-  out->Evac = en[eEvac];
-  out->range_sum = en[eRangeSum];
-  out->range_sum2 = en[eRangeSum2];
+  read_results(sr, E0, out);
   long long* hdst[5] = {out->range_hist, out->lat_y_hist, out->lat_z_hist, out->vac_hist, out->repl_hist};
   for (int i = 0; i < 5; ++i)
     if (hdst[i])
@@ -859,19 +959,7 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
   out->grid = grid;
   out->block = block;
   out->waves = waves;
-  out->queue_peak = (long long)ctrl[kQueuePeak];
-  out->queue_ops = (long long)ctrl[kQueueOps];
-  const unsigned errbits = unsigned(ctrl[kError]);
-  if (errbits) {
-    std::string m = "device error flags:";
-    if (errbits & kErrQueueFull) m += " queue-full";
-    if (errbits & kErrRunaway) m += " runaway-flight";
-    if (errbits & kErrBadEntry) m += " bad-queue-entry";
-    if (errbits & kErrTimeout) m += " wait-timeout";
-    if (errbits & kErrNewton) m += " newton-cap";
-    return fail(MT_E_DEVICE, m);
-  }
-  return MT_OK;
+  return device_errors(ctrl);
 }
 
 int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long long sample, int reps,
@@ -987,9 +1075,10 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
   HIPCHECK(hipStreamSynchronize(st));
   *n_used = used[0];
   *n_events = int(used[1]);
-  if (used[2] == 1) return fail(MT_E_LIMIT, "uniform stream exhausted");
-  if (used[2] == 2) return fail(MT_E_LIMIT, "event buffer full");
+  if (used[2] == 1) return fail(MT_E_MORE, "uniform stream exhausted");
+  if (used[2] == 2) return fail(MT_E_MORE, "event buffer full");
   if (used[2] == 3) return fail(MT_E_DEVICE, "runaway flight");
+  if (used[2] == 4) return fail(MT_E_DEVICE, "non-finite-energy");
   HIPCHECK(hipMemcpyAsync(ion, ctx->f_ion.p, sizeof(mt_ion), hipMemcpyDeviceToHost, st));
   if (used[1] > 0)
     HIPCHECK(hipMemcpyAsync(events, ctx->f_ev.p, size_t(used[1]) * sizeof(mt_event), hipMemcpyDeviceToHost, st));
@@ -1179,8 +1268,10 @@ int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* o, const float* para
   return MT_OK;
 }
 
-int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, long long n, double* se_out) {
+int mt_stopping_batch_arith(mt_ctx* ctx, int arith, int Z1, double m1, int mat, const double* E, long long n,
+                            double* se_out) {
   if (!ctx || (n > 0 && (!E || !se_out))) return fail(MT_E_ARG, "NULL argument");
+  if (arith < MT_FP64 || arith > MT_FP32_SURROGATE) return fail(MT_E_ARG, "unknown arithmetic");
   if (!ctx->has_target) return fail(MT_E_STATE, "no target set");
   if (mat < 0 || mat >= int(ctx->tgt.mat.size())) return fail(MT_E_ARG, "material index out of range");
   if (Z1 < 1 || Z1 > 92) return fail(MT_E_ARG, "Z1 out of range 1..92");
@@ -1199,17 +1290,25 @@ int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, 
   HIPCHECK(ctx->scratch_b.ensure(n * sizeof(double)));
   HIPCHECK(hipMemcpyAsync(ctx->tables.p, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, E, n * sizeof(double), hipMemcpyHostToDevice, st));
-  HIPCHECK(stopping_launch(ctx->tables.as<unsigned char>(), ps[0], mat, ctx->scratch_a.as<double>(), n,
+  HIPCHECK(stopping_launch(arith, ctx->tables.as<unsigned char>(), ps[0], mat, ctx->scratch_a.as<double>(), n,
                            ctx->scratch_b.as<double>(), st));
   HIPCHECK(hipMemcpyAsync(se_out, ctx->scratch_b.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   return MT_OK;
 }
 
-int mt_scatter_batch(mt_ctx* ctx, int potential, const double* eps, const double* b, long long n, double* s2,
-                     double* ct, double* st_out, int* iters) {
+int mt_stopping_batch(mt_ctx* ctx, int Z1, double m1, int mat, const double* E, long long n, double* se_out) {
+  return mt_stopping_batch_arith(ctx, MT_FP64, Z1, m1, mat, E, n, se_out);
+}
+
+int mt_scatter_batch_arith(mt_ctx* ctx, int arith, int potential, const double* eps, const double* b, long long n,
+                           double* s2, double* ct, double* st_out, int* iters) {
   if (!ctx || (n > 0 && (!eps || !b || !s2 || !ct || !st_out || !iters))) return fail(MT_E_ARG, "NULL argument");
+  if (arith < MT_FP64 || arith > MT_FP32_SURROGATE) return fail(MT_E_ARG, "unknown arithmetic");
   if (potential < 0 || potential > 2) return fail(MT_E_ARG, "unknown potential");
+  if (arith == MT_FP32_SURROGATE && potential != MT_UNIVERSAL)
+    return fail(MT_E_ARG, "the surrogate replaces the universal potential's scattering");
+  if (arith == MT_FP32_SURROGATE && !ctx->mlp_ready) return fail(MT_E_STATE, "no surrogate installed");
   if (n <= 0) return MT_OK;
   HIPCHECK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -1224,14 +1323,27 @@ int mt_scatter_batch(mt_ctx* ctx, int potential, const double* eps, const double
   double* d_ct = ctx->scratch_d.as<double>();
   double* d_st = ctx->scratch_e.as<double>();
   int* d_it = reinterpret_cast<int*>(d_st + n);
-  HIPCHECK(scatter_launch(potential, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n, d_s2, d_ct, d_st,
-                          d_it, st));
+  MlpDev S;
+  std::memset(&S, 0, sizeof(S));
+  if (arith == MT_FP32_SURROGATE) {
+    S.p = ctx->mlp_p.as<float>();
+    S.w2bf = ctx->mlp_w2bf.as<uint32_t>();
+    S.n = ctx->mlp_norm;
+    S.bf16 = ctx->mlp_bf16;
+  }
+  HIPCHECK(scatter_launch(arith, potential, S, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n, d_s2,
+                          d_ct, d_st, d_it, st));
   HIPCHECK(hipMemcpyAsync(s2, d_s2, n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipMemcpyAsync(ct, d_ct, n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipMemcpyAsync(st_out, d_st, n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipMemcpyAsync(iters, d_it, n * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   return MT_OK;
+}
+
+int mt_scatter_batch(mt_ctx* ctx, int potential, const double* eps, const double* b, long long n, double* s2,
+                     double* ct, double* st_out, int* iters) {
+  return mt_scatter_batch_arith(ctx, MT_FP64, potential, eps, b, n, s2, ct, st_out, iters);
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@ namespace mt {
 // L2 write-back / invalidate fences); the ready word of a slot holds the lap
 // (position / capacity + 1) in its high half and a checksum of the entry in
 // its low half, so a reader that sees the lap but stale fields re-reads.
+// After copying its entry the reader stamps the slot consumed_word(lap).
 struct Queue {
   uint64_t *px, *py, *pz, *dx, *dy, *dz, *E;  // FP64 bit patterns
   uint64_t* key;
@@ -45,13 +46,20 @@ enum Ctrl {
 };
 __host__ __device__ constexpr int kQHead(int s) { return kQBase + 32 * s; }
 __host__ __device__ constexpr int kQTail(int s) { return kQBase + 32 * s + 16; }
+// ready word of a slot whose lap-`lap` entry has been read out: the lap with
+// its top bit set (a real lap never has it), checksum half zero.  A pusher of
+// lap L writes only over the marker of lap L - 1 (or the zero of a fresh run).
+__host__ __device__ constexpr uint64_t consumed_word(uint32_t lap) {
+  return (uint64_t)(lap | 0x80000000u) << 32;
+}
 
 enum ErrBits {
   kErrQueueFull = 1,
   kErrRunaway = 2,    // flight exceeded max collisions
   kErrBadEntry = 4,   // queue entry failed validation
   kErrTimeout = 8,    // a wait exceeded the wall-clock limit
-  kErrNewton = 16     // Newton solve hit its iteration cap
+  kErrNewton = 16,    // Newton solve hit its iteration cap
+  kErrNonFinite = 32  // a flight's energy or position became NaN or infinite
 };
 
 enum Counter {

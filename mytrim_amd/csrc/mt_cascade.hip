@@ -37,6 +37,7 @@
 #include "mt_mlp.h"
 #include "mt_philox.h"
 #include "mt_physics.h"
+#include "mt_scatter32.h"
 
 namespace mt {
 
@@ -223,17 +224,8 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
     const float b = p * m_rcp((float)ec.ai);
     ls = (double)lsf;
     dee = ls * (double)getrstop<float>(T, sp, k, Ef, lEf);
-    float s2f = -1.0f, ctf, stf;
-    if constexpr (ARITH == 2) {
-      if (eps <= 10.0f) s2f = mlp_s2(mlp, eps, b);  // the closed form is cheaper above 10
-    }
-    if (s2f >= 0.0f) {
-      ctf = 1.0f - 2.0f * s2f;
-      stf = 2.0f * m_sqrt(fmaxf(0.0f, s2f * (1.0f - s2f)));
-      it = -1;
-    } else {
-      scatter_f32<POT>(eps, b, lEf + (float)ec.ln_fi, s2f, ctf, stf, it);
-    }
+    float s2f, ctf, stf;
+    scatter_single<ARITH, POT>(mlp, eps, b, lEf + (float)ec.ln_fi, s2f, ctf, stf, it);
     s2 = s2f;
     ct = ctf;
     st = stf;
@@ -415,6 +407,9 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
                   h = sum_add(h, wh);
                   if (sum_end(h) == (uint32_t)rw) {
                     if ((int)(m & 0xffu) >= T.n_species) err |= kErrBadEntry;
+                    // every field has been loaded (the checksum used each):
+                    // the slot may take the next lap
+                    st_agent(&A->q.ready[slot], consumed_word(lap));
                     break;
                   }
                 }
@@ -712,6 +707,17 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           err |= kErrRunaway;
           ended = true;
         }
+        // a NaN or infinite energy (a corrupt table row, an overflow) fails
+        // every threshold above and would keep the flight moving until
+        // max_ic; a NaN free path leaves the energy finite but the position
+        // not.  x - x is 0 exactly for finite x.  The flight ends and the run
+        // reports it (the reference's only such check is the warning of
+        // trim.C:277-285).
+        if (!(F.E - F.E == 0.0) || !(step - step == 0.0)) {
+          err |= kErrNonFinite;
+          ended = true;
+          push = false;
+        }
       }
     }
     l_vac32 += (active && state == -1) ? 1u : 0u;
@@ -803,11 +809,17 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       const int n_push = __popcll(push_m);
       const int room = cap - scount;
       const int r = lane_rank(push_m);
+      const bool over = push && r >= room;  // goes to the HBM queue
       const uint32_t meta = (uint32_t)rsp | ((uint32_t)(F.gen + 1) << 8);
       const uint64_t rkey = child_key(F.key, (uint32_t)F.ic);
+      int inc_old = 1;
       if (push) {
         const ArgPtr C1 = cold_args();
-        atomicAdd(&C1->h_out[F.hist - (uint32_t)C1->hist0], 1);
+        int* ho = &C1->h_out[F.hist - (uint32_t)C1->hist0];
+        if (over)  // returning: performed before the entry is published (below)
+          inc_old = __hip_atomic_fetch_add(ho, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else       // the entry stays with this wave, which ends it after this
+          atomicAdd(ho, 1);
       }
       if (push && r < room) {
         const int j = scount + r;
@@ -828,31 +840,55 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       if (n_over > 0) {
         const ArgPtr A = cold_args();
         ++c_qops;
+        // the home shard, unless it is over half full: then the least full
+        // of it and three others, so that no shard runs out of slots while
+        // others have room (a shard's fill is bounded by its own capacity,
+        // whatever the global cascade budget)
+        const long long half = (long long)(A->q.sub_mask >> 1);
+        int qs = home;
+        long long qfill = (long long)(ld_relaxed(&ctrl[kQTail(qs)]) - ld_relaxed(&ctrl[kQHead(qs)]));
+        for (int v = 1; v < 4 && qfill > half && A->q.nq > 1; ++v) {
+          const int s2 = (home + v * (A->q.nq >> 2 ? A->q.nq >> 2 : 1)) & (A->q.nq - 1);
+          const long long f2 = (long long)(ld_relaxed(&ctrl[kQTail(s2)]) - ld_relaxed(&ctrl[kQHead(s2)]));
+          if (f2 < qfill) {
+            qfill = f2;
+            qs = s2;
+          }
+        }
         uint64_t base = 0, head = 0;
         if (lane == 0) {
-          base = atomicAdd(&ctrl[kQTail(home)], (unsigned long long)n_over);
-          head = ld_relaxed(&ctrl[kQHead(home)]);
+          base = atomicAdd(&ctrl[kQTail(qs)], (unsigned long long)n_over);
+          head = ld_relaxed(&ctrl[kQHead(qs)]);
         }
         base = bcast64(base, 0);
         head = bcast64(head, 0);
-        // consumers may already have claimed reserved slots, so head can be
-        // past base: the occupancy is a signed difference
+        // occupancy (a diagnostic; consumers may already have claimed
+        // reserved slots, so head can be past base)
         const long long fill = (long long)(base + (uint64_t)n_over) - (long long)head;
         if (fill > peak_seen) {  // this wave's own maximum gates the shared one
           peak_seen = fill;
           if (lane == 0) atomicMax(&ctrl[kQueuePeak], (unsigned long long)fill);
         }
-        if (fill > (long long)A->q.sub_mask) {  // abort: every wave leaves, the caller reports the error
-          err |= kErrQueueFull;
-          if (lane == 0) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
-        }
-        const bool over = push && r >= room;
+        // a slot takes lap L only once its lap L - 1 entry has been read out
+        // (its ready word is that lap's consumed marker, or zero for lap 1);
+        // otherwise the ring is full there: abort, every wave leaves and the
+        // caller reports queue-full
+        bool full = false;
         uint64_t slot = 0;
-        uint32_t lap = 0, sum = 0;
+        uint32_t lap = 0;
         if (over) {
           const uint64_t pos = base + (uint64_t)(r - room);
-          slot = ((uint64_t)home << A->q.sub_log2) | (pos & A->q.sub_mask);
+          slot = ((uint64_t)qs << A->q.sub_log2) | (pos & A->q.sub_mask);
           lap = (uint32_t)(pos >> A->q.sub_log2) + 1u;
+          const uint64_t prev = ld_agent(&A->q.ready[slot]);
+          full = prev != (lap == 1u ? 0ull : consumed_word(lap - 1u));
+        }
+        const uint64_t full_m = __ballot(full);
+        if (full_m) {
+          err |= kErrQueueFull;
+          if (lane == __ffsll((long long)full_m) - 1) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
+        }
+        if (over && !full) {
           uint64_t h = sum_init(lap), w;
           w = (uint64_t)__double_as_longlong(rx); st_agent(&A->q.px[slot], w); h = sum_add(h, w);
           w = (uint64_t)__double_as_longlong(ry); st_agent(&A->q.py[slot], w); h = sum_add(h, w);
@@ -867,13 +903,21 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           st_agent(&A->q.mt[slot], w);
           h = sum_add(h, w);
           st_agent(&A->q.hist[slot], F.hist);
-          sum = sum_end(sum_add(h, F.hist));
+          const uint32_t sum = sum_end(sum_add(h, F.hist));
+          // A reader on another XCD may end this flight as soon as it sees
+          // the ready word, so the history's flight-count increment must be
+          // performed first: the ready word depends on the returned count,
+          // which makes the store wait for the atomic.  A count below 1
+          // cannot happen (the pushing flight is outstanding); it would mean
+          // corrupt bookkeeping, and the run is aborted.  The fields need no
+          // ordering: the reader checks the checksum.
+          const bool sane = inc_old >= 1;
+          if (!sane) err |= kErrBadEntry;
+          st_agent(&A->q.ready[slot], ((uint64_t)(sane ? lap : 0u) << 32) | sum);
         }
-        // this wave's outstanding memory operations (the entry and the
-        // history's flight-count increment) complete before the ready word;
-        // readers check the checksum in any case
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (over) st_agent(&A->q.ready[slot], ((uint64_t)lap << 32) | sum);
+        const uint64_t insane = __ballot((err & kErrBadEntry) != 0u);
+        if (insane && lane == __ffsll((long long)insane) - 1)
+          atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrBadEntry);
       }
     }
 
@@ -881,6 +925,9 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     const bool fin = active && (ended || vacuum);
     const uint64_t fin_m = __ballot(fin);
     if (fin_m) {
+      const uint64_t nf_m = __ballot((err & kErrNonFinite) != 0u);
+      if (nf_m && lane == __ffsll((long long)nf_m) - 1)  // every wave leaves
+        atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrNonFinite);
       // the decrement's old value is read one iteration later (close_pending)
       // so its latency overlaps the next collision
       close_pending();

@@ -26,7 +26,7 @@ struct FlightParams {
   long long* n_used;  // [0] draws used, [1] events written, [2] status
 };
 
-enum { kFlightDone = 0, kFlightNeedDraws = 1, kFlightNeedEvents = 2, kFlightRunaway = 3 };
+enum { kFlightDone = 0, kFlightNeedDraws = 1, kFlightNeedEvents = 2, kFlightRunaway = 3, kFlightNonFinite = 4 };
 
 template <int POT>
 __global__ void __launch_bounds__(64) flight_kernel(FlightParams P) {
@@ -164,6 +164,12 @@ __global__ void __launch_bounds__(64) flight_kernel(FlightParams P) {
     e.mat_tag = mtag;
     if (ic >= 100000000) {
       status = kFlightRunaway;
+      break;
+    }
+    // a NaN or infinite energy or position would fail every threshold and
+    // run to the collision cap (x - x is 0 exactly for finite x)
+    if (!(ion.E - ion.E == 0.0) || !(ion.pos[0] - ion.pos[0] == 0.0)) {
+      status = kFlightNonFinite;
       break;
     }
   } while (ion.state == 0);

@@ -123,6 +123,27 @@ static StopConst make_stop(const std::vector<CoefRow>& coef, int z1, double m_io
   return c;
 }
 
+std::string collect_species_into(const HostTarget& tgt, const int* pZ, const double* pm, long long n, Species* sp,
+                                 int* ns) {
+  int k = 0;
+  auto add = [&](int Z, double m) {
+    for (int i = 0; i < k; ++i)
+      if (sp[i].Z == Z && sp[i].m == m) return true;
+    if (k >= kMaxSpecies) return false;
+    sp[k++] = Species{Z, m};
+    return true;
+  };
+  for (long long i = 0; i < n; ++i) {
+    if (pZ[i] < 1 || pZ[i] > 92) return "primary Z out of range 1..92";
+    if (!add(pZ[i], pm[i])) return "too many ion species";
+  }
+  for (const auto& m : tgt.mat)
+    for (const auto& e : m.el)
+      if (!add(e.Z, e.m)) return "too many ion species";
+  *ns = k;
+  return "";
+}
+
 int species_index(std::vector<Species>& species, int Z, double m) {
   for (size_t i = 0; i < species.size(); ++i)
     if (species[i].Z == Z && species[i].m == m) return int(i);
@@ -147,16 +168,13 @@ std::string collect_species(const HostTarget& tgt, const int* pZ, const double* 
   return "";
 }
 
-std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt,
-                         const std::vector<Species>& species, std::vector<unsigned char>& blob) {
-  const int ns = int(species.size());
-  const int nm = int(tgt.mat.size());
-  int ne = 0;
-  for (const auto& m : tgt.mat) ne += int(m.el.size());
-  if (ns > kMaxSpecies || nm > kMaxMaterials || ne > kMaxElements) return "table limits exceeded";
-  auto align16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+namespace {
+size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+TableHeader table_header(const HostTarget& tgt, int ns, int ne) {
   TableHeader h;
   std::memset(&h, 0, sizeof(h));
+  const int nm = int(tgt.mat.size());
   h.n_species = ns;
   h.n_mat = nm;
   h.n_el = ne;
@@ -165,12 +183,36 @@ std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt
   h.off_el = int(align16(h.off_mat + sizeof(MatConst) * ns * nm));
   h.off_stop = int(align16(h.off_el + sizeof(ElemConst) * ns * ne));
   h.bytes = int(align16(h.off_stop + sizeof(StopConst) * ns * ne));
+  return h;
+}
+}  // namespace
+
+std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt,
+                         const std::vector<Species>& species, std::vector<unsigned char>& blob) {
+  int ne = 0;
+  for (const auto& m : tgt.mat) ne += int(m.el.size());
+  const int ns = int(species.size());
+  if (ns > kMaxSpecies || int(tgt.mat.size()) > kMaxMaterials || ne > kMaxElements) return "table limits exceeded";
+  blob.assign(size_t(table_header(tgt, ns, ne).bytes), 0);
+  size_t bytes = 0;
+  return build_tables_into(coef, tgt, species.data(), ns, blob.data(), blob.size(), &bytes);
+}
+
+std::string build_tables_into(const std::vector<CoefRow>& coef, const HostTarget& tgt, const Species* species,
+                              int ns, unsigned char* dst, size_t cap, size_t* bytes) {
+  const int nm = int(tgt.mat.size());
+  int ne = 0;
+  for (const auto& m : tgt.mat) ne += int(m.el.size());
+  if (ns > kMaxSpecies || nm > kMaxMaterials || ne > kMaxElements) return "table limits exceeded";
+  TableHeader h = table_header(tgt, ns, ne);
+  if (size_t(h.bytes) > cap) return "table buffer too small";
+  *bytes = size_t(h.bytes);
   for (int s = 0; s < ns; ++s) h.species[s] = species[s];
-  blob.assign(h.bytes, 0);
-  std::memcpy(blob.data(), &h, sizeof(h));
-  MatConst* M = reinterpret_cast<MatConst*>(blob.data() + h.off_mat);
-  ElemConst* E = reinterpret_cast<ElemConst*>(blob.data() + h.off_el);
-  StopConst* S = reinterpret_cast<StopConst*>(blob.data() + h.off_stop);
+  std::memset(dst, 0, size_t(h.bytes));
+  std::memcpy(dst, &h, sizeof(h));
+  MatConst* M = reinterpret_cast<MatConst*>(dst + h.off_mat);
+  ElemConst* E = reinterpret_cast<ElemConst*>(dst + h.off_el);
+  StopConst* S = reinterpret_cast<StopConst*>(dst + h.off_stop);
 
   for (int s = 0; s < ns; ++s) {
     const double m = species[s].m;

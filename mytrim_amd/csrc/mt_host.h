@@ -49,6 +49,21 @@ const std::vector<CoefRow>& default_coefs();
 std::string build_tables(const std::vector<CoefRow>& coef, const HostTarget& tgt,
                          const std::vector<Species>& species, std::vector<unsigned char>& blob);
 
+// build_tables into a caller buffer of `cap` bytes (no allocation); *bytes is
+// the blob's size.  Returns "" or an error message.
+std::string build_tables_into(const std::vector<CoefRow>& coef, const HostTarget& tgt, const Species* species,
+                              int ns, unsigned char* dst, size_t cap, size_t* bytes);
+
+// the species of collect_species into a caller array of kMaxSpecies entries,
+// without the per-primary indices (no allocation)
+std::string collect_species_into(const HostTarget& tgt, const int* pZ, const double* pm, long long n, Species* sp,
+                                 int* ns);
+
+// upper bound of a table blob's size under the table limits
+constexpr size_t kMaxTableBytes =
+    ((sizeof(TableHeader) + 15) & ~size_t(15)) + sizeof(MatConst) * kMaxSpecies * kMaxMaterials + 16 +
+    (sizeof(ElemConst) + sizeof(StopConst)) * kMaxSpecies * kMaxElements + 32;
+
 // species index of (Z, m), appended when new
 int species_index(std::vector<Species>& species, int Z, double m);
 

@@ -213,7 +213,18 @@ __device__ __forceinline__ int apsis(R eps, R b, R& r, R& v, R& v1) {
 // MAGIC coefficients (trim.C:236-262): cc, aa = 2 eps (1 + c/sqrt(eps)) b^cc
 // and ff for the three screenings; 1/sqrt(eps) comes from one reciprocal
 // square root, the quotients from reciprocals, b^cc = exp(cc log b) from the
-// log b of the apsis' starting point
+// log b of the apsis' starting point.  ff carries sqrt(aa^2 + 1) - aa, which
+// cancels for large aa (glancing collisions: aa ~ 500 leaves 3 % of a single
+// precision result); in single precision it is formed as the identical
+// 1 / (sqrt(aa^2 + 1) + aa), in FP64 as the reference writes it.
+template <typename R>
+__device__ __forceinline__ R magic_root_gap(R aa) {  // sqrt(aa^2 + 1) - aa
+  if constexpr (sizeof(R) == sizeof(float))
+    return m_rcp(m_sqrt(aa * aa + K(1.0)) + aa);
+  else
+    return m_sqrt(aa * aa + K(1.0)) - aa;
+}
+
 template <typename R, int POT>
 __device__ __forceinline__ void magic(R eps, R lb, R& aa, R& ff) {  // lb = log b
   const R ise = m_rsqrt(eps);
@@ -222,15 +233,15 @@ __device__ __forceinline__ void magic(R eps, R lb, R& aa, R& ff) {  // lb = log 
   if (POT == 1) {
     cc = (K(0.009611) + sqe) * m_rcp(K(0.005175) + sqe);
     aa = K(2.0) * eps * (K(1.0) + K(0.6743) * ise) * m_exp(cc * lb);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6.314) + eps) * m_rcp(K(10.0) + eps));
+    ff = magic_root_gap(aa) * ((K(6.314) + eps) * m_rcp(K(10.0) + eps));
   } else if (POT == 2) {
     cc = (K(0.235809) + sqe) * m_rcp(K(0.126000) + sqe);
     aa = K(2.0) * eps * (K(1.0) + K(1.0144) * ise) * m_exp(cc * lb);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(6935.0) + eps) * m_rcp(K(83550.0) + eps));
+    ff = magic_root_gap(aa) * ((K(6935.0) + eps) * m_rcp(K(83550.0) + eps));
   } else {
     cc = (K(0.011615) + sqe) * m_rcp(K(0.0071222) + sqe);
     aa = K(2.0) * eps * (K(1.0) + K(0.99229) * ise) * m_exp(cc * lb);
-    ff = (m_sqrt(aa * aa + K(1.0)) - aa) * ((K(9.3066) + eps) * m_rcp(K(14.813) + eps));
+    ff = magic_root_gap(aa) * ((K(9.3066) + eps) * m_rcp(K(14.813) + eps));
   }
 }
 

@@ -35,7 +35,8 @@ EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt
            "mt_default_tables",
            "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_run_cascades_log", "mt_autotune",
            "mt_trim_flight",
-           "mt_stopping_batch", "mt_scatter_batch", "mt_surrogate_norm", "mt_surrogate_train", "mt_surrogate_install",
+           "mt_stopping_batch", "mt_scatter_batch", "mt_stopping_batch_arith", "mt_scatter_batch_arith",
+           "mt_surrogate_norm", "mt_surrogate_train", "mt_surrogate_install",
            "mt_surrogate_eval", "mt_surrogate_grad"]
 
 
@@ -130,6 +131,7 @@ def lib():
         L.mt_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         L.mt_ctx_destroy.argtypes = [C.c_void_p]
         L.mt_ctx_set_tables.argtypes = [C.c_void_p, dp, C.c_int, C.c_int]
+        L.mt_default_tables.argtypes = [dp, C.c_int, C.c_int]
         L.mt_ctx_set_target.argtypes = [C.c_void_p, C.POINTER(Target)]
         L.mt_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.mt_run_cascades_log.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
@@ -143,6 +145,10 @@ def lib():
         L.mt_stopping_batch.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, dp,
                                         C.c_longlong, dp]
         L.mt_scatter_batch.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_longlong, dp, dp, dp, ip]
+        L.mt_stopping_batch_arith.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, dp,
+                                              C.c_longlong, dp]
+        L.mt_scatter_batch_arith.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, C.c_longlong, dp, dp,
+                                             dp, ip]
         fp = C.POINTER(C.c_float)
         so = C.POINTER(SurrogateOpts)
         L.mt_surrogate_train.argtypes = [C.c_void_p, so, fp, dp]
@@ -156,6 +162,13 @@ def lib():
 
 class MtError(RuntimeError):
     pass
+
+
+def default_tables() -> np.ndarray:
+    """The SCOEF/SLFCTR rows built into the library (92 x 16)."""
+    rows = np.zeros((92, 16))
+    _check(lib().mt_default_tables(_p(rows, dp), 92, 16), "mt_default_tables")
+    return rows
 
 
 def _check(rc: int, what: str):
@@ -191,6 +204,13 @@ class Engine:
     def tune(self, block=0, blocks_per_cu=0, stack_cap=0, queue_cap=0, lds_hist=0, graph=0):
         t = Tuning(block, blocks_per_cu, stack_cap, queue_cap, lds_hist, graph)
         _check(lib().mt_ctx_set_tuning(self.h, C.byref(t)), "mt_ctx_set_tuning")
+
+    def set_tables(self, rows: np.ndarray):
+        """Replace the SCOEF/SLFCTR rows (92 x 16, the layout of
+        default_tables()); runs after this use them."""
+        rows = np.ascontiguousarray(rows, np.float64)
+        _check(lib().mt_ctx_set_tables(self.h, _p(rows, dp), rows.shape[0], rows.shape[1]),
+               "mt_ctx_set_tables")
 
     def set_target(self, prob: Problem):
         rho, off, Z, m, t, ed, eb = prob.flat_materials()
@@ -296,21 +316,33 @@ class Engine:
         del keep
         return res
 
-    def stopping(self, Z1: int, m1: float, mat: int, E: np.ndarray) -> np.ndarray:
+    def stopping(self, Z1: int, m1: float, mat: int, E: np.ndarray, arith: int = MT_FP64) -> np.ndarray:
+        """getrstop of ion (Z1, m1) in material `mat` at energies E, by the
+        device function of precision mode `arith`."""
         E = np.ascontiguousarray(E, np.float64)
         out = np.zeros_like(E)
-        _check(lib().mt_stopping_batch(self.h, Z1, m1, mat, _p(E, dp), len(E), _p(out, dp)),
-               "mt_stopping_batch")
+        if arith == MT_FP64:
+            rc = lib().mt_stopping_batch(self.h, Z1, m1, mat, _p(E, dp), len(E), _p(out, dp))
+        else:
+            rc = lib().mt_stopping_batch_arith(self.h, arith, Z1, m1, mat, _p(E, dp), len(E), _p(out, dp))
+        _check(rc, "mt_stopping_batch")
         return out
 
-    def scatter(self, eps: np.ndarray, b: np.ndarray, potential: int = 0):
+    def scatter(self, eps: np.ndarray, b: np.ndarray, potential: int = 0, arith: int = MT_FP64):
+        """(s2, cos theta, sin theta, iterations) by the device scattering of
+        precision mode `arith` (iterations -1: the surrogate answered)."""
         eps = np.ascontiguousarray(eps, np.float64)
         b = np.ascontiguousarray(b, np.float64)
         n = len(eps)
         s2, ct, st = np.zeros(n), np.zeros(n), np.zeros(n)
         it = np.zeros(n, np.int32)
-        _check(lib().mt_scatter_batch(self.h, potential, _p(eps, dp), _p(b, dp), n, _p(s2, dp),
-                                      _p(ct, dp), _p(st, dp), _p(it, ip)), "mt_scatter_batch")
+        if arith == MT_FP64:
+            rc = lib().mt_scatter_batch(self.h, potential, _p(eps, dp), _p(b, dp), n, _p(s2, dp),
+                                        _p(ct, dp), _p(st, dp), _p(it, ip))
+        else:
+            rc = lib().mt_scatter_batch_arith(self.h, arith, potential, _p(eps, dp), _p(b, dp), n,
+                                              _p(s2, dp), _p(ct, dp), _p(st, dp), _p(it, ip))
+        _check(rc, "mt_scatter_batch")
         return s2, ct, st, it
 
     # ---------------------------------------------------------- surrogate

@@ -80,7 +80,8 @@ void TrimBase::trim(IonBase* pka, std::queue<IonBase*>& recoils) {
       ion = io;
       break;
     }
-    if (rc != MT_E_LIMIT) device::check(rc, "mt_trim_flight");
+    // only exhaustion of the draws or of the event buffer is retried
+    if (rc != MT_E_MORE) device::check(rc, "mt_trim_flight");
     if (used >= n_u)
       n_u *= 4;
     else

@@ -15,13 +15,12 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle_lib as O  # noqa: E402
-from fixture_cases import c4_uniform  # noqa: E402
 from mytrim_amd import configs  # noqa: E402
 
 
 def main():
     samples = {"c1": configs.c1_he_fe(2000), "c2": configs.c2_cu_cu(40),
-               "c3": configs.c3_ag_layers(40), "c4": c4_uniform(4000.0, 40),
+               "c3": configs.c3_ag_layers(40), "c4": configs.c4_mixed_uo2(400),
                "c5": configs.c5_c_w(200)}
     out = {}
     for name, prob in samples.items():

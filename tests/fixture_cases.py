@@ -65,19 +65,20 @@ def tally_cases(quick=False):
 def stat_cases():
     """name -> problem for the statistical fixtures (tests/golden/stat_*.npz):
     larger samples of the workloads, with per-history tallies and batched
-    depth histograms, for the two-sample tests of tests/tally_stats.py.  The
-    mixed C4 workload runs on its benchmark box with its per-history H, He,
-    C and Xe primaries."""
+    depth histograms, for the two-sample tests of tests/tally_stats.py.  Each
+    holds about 1e8 collisions or more (at most 32 000 histories).  The
+    mixed C4 workload runs on its benchmark box with its per-history H, He, C
+    and Xe primaries."""
     return {
-        "c1": configs.c1_he_fe(16000),
+        "c1": configs.c1_he_fe(32000),
         "c2": configs.c2_cu_cu(4800),
         "c3": configs.c3_ag_layers(3200),
-        "c4mix": configs.c4_mixed_uo2(6400),
-        "c5": configs.c5_c_w(6400),
-        "c2_primaries_kp": configs.c2_cu_cu(16000).with_(follow_max_gen=1, kp=True),
-        "vac_cu_10keV": configs.vacancy_case("cu", 1e4, 9600),
-        "wire": configs.wire_b_si(3200),
-        "bwire": configs.burried_wire_p_si(960),
+        "c4mix": configs.c4_mixed_uo2(12800),
+        "c5": configs.c5_c_w(25600),
+        "c2_primaries_kp": configs.c2_cu_cu(32000).with_(follow_max_gen=1, kp=True),
+        "vac_cu_10keV": configs.vacancy_case("cu", 1e4, 32000),
+        "wire": configs.wire_b_si(32000),
+        "bwire": configs.burried_wire_p_si(9600),
     }
 
 

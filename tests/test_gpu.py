@@ -85,6 +85,61 @@ def test_scatter_matches_oracle(eng, potential):
     assert np.mean(it == ref[:, 3]) >= 0.999
 
 
+# ---------------------------------------------------------------- single precision
+# The single-precision device functions of the FP32 cascade mode against the
+# reference's FP64 values (FP32 unit roundoff 6e-8).
+# * stopping: about thirty operations, the powers as exp2(y log2 x), whose
+#   rounding grows with |y log x| (up to ~10 in the effective-charge fits):
+#   relative 2e-5.
+# * scattering: single precision cannot resolve s2 = 1 - co^2 for glancing
+#   collisions, so the FP32 path forms s2 from the apsis identity
+#   r - b = v r^2 / (eps (r + b)), exact at the apsis.  The reference
+#   evaluates its formula at the Newton iterate that stopped at a relative
+#   step of 1e-3 (trim.C:233), not at the apsis: in FP64 the two formulas
+#   differ by up to 5.1e-4 in s2 for the universal and Moliere potentials,
+#   and single precision adds no more than that: relative 1e-3 in s2, 1e-4
+#   in cos theta.
+#   The reference's C-Kr derivative coefficients are not the derivative of its
+#   screening (trim.C:215-221: 0.531865 for 0.191 x 0.279 = 0.0533, 0.6437
+#   for 0.335 x 1.1919 = 0.3993), so its iteration converges only linearly
+#   and stops short of the apsis by up to 6e-2 in s2: bound 8e-2 there.
+FP32_STOP_RTOL = 2e-5
+FP32_S2_RTOL = {0: 1e-3, 1: 1e-3, 2: 8e-2}
+FP32_CT_ATOL = {0: 1e-4, 1: 1e-4, 2: 8e-2}
+
+
+def test_fp32_stopping_within_tolerance_of_reference(eng):
+    g = np.load(os.path.join(GOLD, "stopping_grid.npz"))
+    errs = []
+    for j, (_, rho, els) in enumerate(TARGETS):
+        eng.set_target(target_problem(rho, els))
+        for i, (Z1, m1) in enumerate(IONS):
+            se = eng.stopping(Z1, m1, 0, ENERGIES, arith=gpu.MT_FP32)
+            ref = g["se"][i, j]
+            errs.append(np.abs(se - ref) / np.abs(ref))
+    e = np.concatenate(errs)
+    print("fp32 stopping relative error: median %.2e p99 %.2e max %.2e" % (
+        np.median(e), np.quantile(e, 0.99), e.max()))
+    assert e.max() <= FP32_STOP_RTOL
+
+
+@pytest.mark.parametrize("potential", [0, 1, 2])
+def test_fp32_scatter_within_tolerance_of_oracle(eng, potential):
+    eps = np.repeat(np.logspace(-5, 3, 81), 41)
+    b = np.tile(np.concatenate([[1e-6, 1e-3], np.linspace(0.01, 20.0, 39)]), 81)
+    # the inputs the FP32 functions see (rounded to single precision)
+    eps32, b32 = eps.astype(np.float32).astype(float), b.astype(np.float32).astype(float)
+    s2, ct, st, it = eng.scatter(eps, b, potential, arith=gpu.MT_FP32)
+    ref = np.array([O.scatter(e, bb, potential) for e, bb in zip(eps32, b32)])
+    rs2 = np.abs(s2 - ref[:, 0]) / np.maximum(ref[:, 0], 1e-300)
+    dct = np.abs(ct - ref[:, 1])
+    print("fp32 scatter (potential %d): |d s2|/s2 median %.2e p99 %.2e max %.2e; max |d ct| %.2e" % (
+        potential, np.median(rs2), np.quantile(rs2, 0.99), rs2.max(), dct.max()))
+    assert rs2.max() <= FP32_S2_RTOL[potential]
+    assert dct.max() <= FP32_CT_ATOL[potential]
+    assert np.all(np.abs(st * st + ct * ct - 1.0) <= 1e-5)
+
+
 # ---------------------------------------------------------------- cascades vs oracle
 CASES = {
     "c1": lambda: configs.c1_he_fe(400, seed=5),
@@ -231,6 +286,42 @@ def test_sub_threshold_primaries(eng):
     o = O.run(prob, rng=1, per_history=True)
     assert np.array_equal(g["h_coll"], o["h_coll"])
     assert g["inter"] == o["inter"]
+
+
+def test_non_finite_energy_is_reported():
+    """A NaN stopping coefficient makes every flight's energy NaN: the run
+    returns MT_E_DEVICE "non-finite-energy" at once (not a runaway after 1e7
+    collisions per flight), every kernel exits, and the context runs
+    normally again once the tables are restored.  The single-flight path
+    reports the same."""
+    rows = gpu.default_tables()
+    bad = rows.copy()
+    bad[28, 7:15] = np.nan  # Cu (Z = 29): the eight stopping coefficients
+    e = gpu.Engine(0)
+    try:
+        prob = configs.c2_cu_cu(256, seed=5)
+        e.set_tables(bad)
+        e.set_target(prob)
+        with pytest.raises(gpu.MtError, match="non-finite-energy"):
+            e.run(prob)
+        ion = gpu.Ion(29, 63.546, 5.0e4, (0.0, 100.0, 100.0), (1.0, 0.0, 0.0), 0, -1, 0, 3.0)
+        with pytest.raises(gpu.MtError, match="non-finite-energy"):
+            e.trim_flight(ion, np.full(4096, 0.5))
+        e.set_tables(rows)
+        e.set_target(prob)
+        r = e.run(prob)
+        ref = eng_reference_run(prob)
+        assert r["n_coll"] == ref["n_coll"] and r["vac"] == ref["vac"]
+    finally:
+        e.close()
+
+
+def eng_reference_run(prob):
+    e = gpu.Engine(0)
+    try:
+        return e.run(prob)
+    finally:
+        e.close()
 
 
 def test_invalid_input_raises(eng):

@@ -7,7 +7,7 @@ tally_stats: vacancies per ion, collisions per history, Eel/E0, Enuc/E0 and
 the mean projected range (jointly, p >= 1e-3, each |z| <= 3.3, the count
 within 2 sigma printed), the range and lateral histograms (chi-square
 homogeneity), the vacancy and replacement depth histograms (Hotelling T^2).
-The FP32 kernel is held to the same criteria.  The GPU sample is several
+The FP32 kernel is held to the same criteria.  The GPU sample is two to eight
 times the reference's, so the reference sample's own error dominates sigma.
 """
 import pytest
@@ -19,8 +19,8 @@ from mytrim_amd import configs, gpu
 pytestmark = pytest.mark.gpu
 
 # GPU histories per workload (32 equal batches)
-N_GPU = {"c1": 128000, "c2": 38400, "c3": 25600, "c4mix": 51200, "c5": 51200,
-         "c2_primaries_kp": 128000, "vac_cu_10keV": 76800, "wire": 25600, "bwire": 7680}
+N_GPU = {"c1": 128000, "c2": 38400, "c3": 25600, "c4mix": 51200, "c5": 76800,
+         "c2_primaries_kp": 128000, "vac_cu_10keV": 128000, "wire": 128000, "bwire": 19200}
 K = 32
 
 
