@@ -255,6 +255,21 @@ int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* tuning);
 int mt_run_cascades(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts,
                     mt_tallies* out);
 
+/* mt_run_cascades without allocating: no counterpart in the reference.  It
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * device buffer of the run (tables, primaries, queue, control words,
+ * tallies) comes from one static array of 205 MiB that each device running
+ * the library holds until the process ends; the results are copied straight
+ * into the caller's mt_tallies arrays.  Same inputs and results as
+ * mt_run_cascades (the hot kernel only: per_history must be 0).  Limits: at
+ * most 2^22 histories, 2^18 per-history primaries, about 130 000 histogram
+ * bins in all, 16 384 resident waves, and 2^21 HBM queue slots (a run that
+ * needs more stops with "queue-full"); larger inputs return MT_E_LIMIT.
+ * Calls on one device share the array: calls that overlap on one GPU race. */
+int mt_run_cascades_noalloc(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts,
+                            mt_tallies* out);
+
 /* mt_run_cascades with event streams (TrimDefectLog, TrimHistory,
  * TrimPhononOut, TrimVacMap; trim.C:466-527, trim.h:139-227) written to
  * `log`.  Runs the per-history kernel. */

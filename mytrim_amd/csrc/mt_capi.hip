@@ -7,6 +7,7 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -27,6 +28,8 @@ const void* zero_spans_function();
 const void* flush_energy_function();
 hipError_t zero_spans_launch(const ZeroSpans& z, hipStream_t st);
 hipError_t flush_energy_launch(const double* we, int waves, double* out, hipStream_t st);
+hipError_t primary_species_launch(const SpeciesList& L, const int* Z, const double* m, long long n, int* out,
+                                  hipStream_t st);
 hipError_t stopping_launch(int arith, const unsigned char* tables, int species, int mat, const double* E, long long n,
                            double* out, hipStream_t stream);
 hipError_t scatter_launch(int arith, int potential, const MlpDev& mlp, const double* eps, const double* b, long long n,
@@ -156,8 +159,38 @@ uint64_t next_pow2(uint64_t x) {
   return p;
 }
 
+// Device memory of mt_run_cascades_noalloc, carved from one static array per
+// device (`scratch` below); limits of one call:
+constexpr size_t na_align(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr long long kNaMaxEntries = 1ll << 18;      // per-history primaries
+constexpr long long kNaMaxHist = 1ll << 22;         // histories
+constexpr size_t kNaResHistBytes = size_t(1) << 20;  // result block + the five histograms
+constexpr int kNaMaxWaves = 16384;                  // resident waves of the launch
+constexpr uint64_t kNaQueueSlots = uint64_t(1) << 21;
+constexpr size_t kNaSlotBytes = 9 * 8 + 4 + 8;      // one entry of each queue array
+// layout: tables | primaries (species, Z, m, E, pos, dir) | per-history
+// flight counts | result block + histograms | per-wave energies | queue
+constexpr size_t kNaOffTables = 0;
+constexpr size_t kNaOffPka = na_align(kNaOffTables + kMaxTableBytes);
+constexpr size_t kNaOffHout = na_align(kNaOffPka + size_t(kNaMaxEntries) * (2 * sizeof(int) + 8 * sizeof(double)));
+constexpr size_t kNaOffRes = na_align(kNaOffHout + size_t(kNaMaxHist) * sizeof(int));
+constexpr size_t kNaOffWave = na_align(kNaOffRes + kNaResHistBytes);
+constexpr size_t kNaOffQueue = na_align(kNaOffWave + size_t(kNaMaxWaves) * 2 * kNumEnergy * sizeof(double));
+constexpr size_t kNaBytes = na_align(kNaOffQueue + size_t(kNaQueueSlots) * kNaSlotBytes);
+static_assert(kNaBytes <= (size_t(224) << 20), "static scratch above its stated 224 MiB");
+
+// the array's device address on each device, resolved once per device at
+// context creation (hipGetSymbolAddress with that device current)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
 }  // namespace
 
+// Scratch of mt_run_cascades_noalloc: each device that runs the library holds
+// one copy until the process ends.  Host code never uses this name as an
+// address; it uses the pointer resolved for the device (g_scratch_dev).
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
 
 struct mt_ctx {
   int device = 0;
@@ -188,6 +221,10 @@ struct mt_ctx {
   int mlp_bf16 = 0;
   DBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
   DBuf log_rec, log_map, log_tr;  // event records, TrimVacMap map, layer transport
+  // mt_run_cascades_noalloc: the device's static scratch and a host buffer
+  // for the table blob, sized once here
+  void* na_scratch = nullptr;
+  std::vector<unsigned char> na_blob;
 
   ~mt_ctx() {
     graph.release();
@@ -242,6 +279,21 @@ int mt_ctx_create(int device, mt_ctx** out) {
     return fail(MT_E_HIP, std::string("context set-up: ") + hipGetErrorString(e));
   }
   if (khz > 0) c->clock_khz = khz;
+  // the static scratch's address on this device (resolved once per device)
+  if (device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+      void* p = nullptr;
+      const hipError_t se = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (se != hipSuccess) {
+        delete c;
+        return fail(MT_E_HIP, std::string("static scratch address: ") + hipGetErrorString(se));
+      }
+      g_scratch_dev[device] = p;
+    }
+    c->na_scratch = g_scratch_dev[device];
+  }
+  c->na_blob.assign(kMaxTableBytes, 0);
   *out = c;
   return MT_OK;
 }
@@ -505,7 +557,7 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
 static_assert(sizeof(LogRecord) == sizeof(mt_record) && sizeof(mt_record) == 80, "record layout");
 
 // ------------------------------------------------------------ run pieces
-// (shared by mt_run_cascades and mt_run_cascades_log)
+// (shared by mt_run_cascades, mt_run_cascades_log and mt_run_cascades_noalloc)
 
 // bytes per slot of each queue array (px py pz dx dy dz E key mt: 8, hist: 4, ready: 8)
 constexpr size_t kQueueFieldBytes[kQueueArrays] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 8};
@@ -1266,6 +1318,116 @@ int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* o, const float* para
   HIPCHECK(hipStreamSynchronize(st));
   *loss = lf;
   return MT_OK;
+}
+
+// mt_run_cascades with every device buffer carved from the device's static
+// scratch: no allocation of any kind (no device, pinned or graph memory); the
+// kernels are launched one by one and the results copied straight into the
+// caller's arrays.  Hot kernel only (no per-history records or streams).
+int mt_run_cascades_noalloc(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, mt_tallies* out) {
+  if (int rc = validate_run(ctx, pk, o, out)) return rc;
+  if (o->per_history) return fail(MT_E_ARG, "mt_run_cascades_noalloc keeps no per-history records");
+  if (pk->n_hist > kNaMaxHist) return fail(MT_E_LIMIT, "more than 2^22 histories for the static scratch");
+  const long long n_entries = pk->pka_uniform ? 1 : pk->n_hist;
+  if (n_entries > kNaMaxEntries)
+    return fail(MT_E_LIMIT, "more than 2^18 per-history primaries for the static scratch");
+  const int nb[5] = {o->range_nb + 2, o->lat_nb + 2, o->lat_nb + 2, o->vac_nb + 2, o->vac_nb + 2};
+  size_t hoff[6] = {0};
+  for (int i = 0; i < 5; ++i) hoff[i + 1] = hoff[i] + size_t(nb[i]);
+  if ((size_t(kResWords) + hoff[5]) * sizeof(unsigned long long) > kNaResHistBytes)
+    return fail(MT_E_LIMIT, "histograms exceed the static scratch (about 130 000 bins)");
+  if (!ctx->na_scratch) return fail(MT_E_STATE, "no static scratch on this device");
+  HIPCHECK(hipSetDevice(ctx->device));
+  double E0 = 0.0, e_max = 0.0;
+  if (int rc = check_primaries(pk, E0, e_max)) return rc;
+  Species sp[kMaxSpecies];
+  int ns = 0;
+  std::string msg = collect_species_into(ctx->tgt, pk->Z, pk->m, n_entries, sp, &ns);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  size_t tb = 0;
+  msg = build_tables_into(ctx->coef, ctx->tgt, sp, ns, ctx->na_blob.data(), ctx->na_blob.size(), &tb);
+  if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+  RunShape sh;
+  if (int rc = run_shape(ctx, o, tb, false, sh)) return rc;
+  if (sh.waves > kNaMaxWaves) return fail(MT_E_LIMIT, "more than 16384 waves for the static scratch");
+  const double f_init = flights_per_cascade0(ctx, e_max);
+
+  unsigned char* base = static_cast<unsigned char*>(ctx->na_scratch);
+  int* d_sp = reinterpret_cast<int*>(base + kNaOffPka);
+  int* d_Z = d_sp + kNaMaxEntries;
+  double* d_m = reinterpret_cast<double*>(d_Z + kNaMaxEntries);
+  double* d_E = d_m + kNaMaxEntries;
+  double* d_pos = d_E + kNaMaxEntries;
+  double* d_dir = d_pos + 3 * kNaMaxEntries;
+  RunMem M;
+  M.tables = base + kNaOffTables;
+  M.table_bytes = int(tb);
+  M.pka_sp = d_sp;
+  M.pka_E = d_E;
+  M.pka_pos = d_pos;
+  M.pka_dir = d_dir;
+  size_t qo = kNaOffQueue;
+  for (int i = 0; i < kQueueArrays; ++i) {
+    M.q[i] = base + qo;
+    qo += size_t(kNaQueueSlots) * kQueueFieldBytes[i];
+  }
+  M.qcap = kNaQueueSlots;
+  M.res = reinterpret_cast<unsigned long long*>(base + kNaOffRes);
+  M.hist = M.res + kResWords;
+  M.wave_energy = reinterpret_cast<double*>(base + kNaOffWave);
+  M.h_out = reinterpret_cast<int*>(base + kNaOffHout);
+
+  hipStream_t st = ctx->stream;
+  HIPCHECK(hipMemcpyAsync(base + kNaOffTables, ctx->na_blob.data(), tb, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(d_Z, pk->Z, size_t(n_entries) * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(d_m, pk->m, size_t(n_entries) * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(d_E, pk->E, size_t(n_entries) * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(d_pos, pk->pos, 3 * size_t(n_entries) * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(d_dir, pk->dir, 3 * size_t(n_entries) * sizeof(double), hipMemcpyHostToDevice, st));
+  SpeciesList L;
+  std::memset(&L, 0, sizeof(L));
+  L.n = ns;
+  for (int k = 0; k < ns; ++k) {
+    L.Z[k] = sp[k].Z;
+    L.m[k] = sp[k].m;
+  }
+  HIPCHECK(primary_species_launch(L, d_Z, d_m, n_entries, d_sp, st));
+
+  // what must start at zero: the previous call's contents are still there
+  ZeroSpans zs{};
+  auto span = [&zs](void* p, size_t bytes) {
+    zs.p[zs.n] = p;
+    zs.bytes[zs.n] = bytes;
+    ++zs.n;
+  };
+  span(M.q[kQueueReady], size_t(kNaQueueSlots) * sizeof(uint64_t));
+  span(M.res, (size_t(kResWords) + hoff[5]) * sizeof(unsigned long long));
+  span(M.wave_energy, size_t(sh.waves) * 2 * kNumEnergy * sizeof(double));
+  span(M.h_out, size_t(std::max<long long>(pk->n_hist, 1)) * sizeof(int));
+  Params P;
+  fill_params(P, ctx, pk, o, M, sh, f_init, hoff);
+  double* red = reinterpret_cast<double*>(M.res + kResReduced);
+  HIPCHECK(zero_spans_launch(zs, st));
+  HIPCHECK(hipEventRecord(ctx->ev0, st));
+  HIPCHECK(cascade_launch(P, o->arith, ctx->tgt.potential, false, sh.grid, sh.block, sh.lds, st));
+  HIPCHECK(hipEventRecord(ctx->ev1, st));
+  HIPCHECK(flush_energy_launch(M.wave_energy, sh.waves, red, st));
+  unsigned long long resh[kResWords];
+  HIPCHECK(hipMemcpyAsync(resh, M.res, sizeof(resh), hipMemcpyDeviceToHost, st));
+  long long* hdst[5] = {out->range_hist, out->lat_y_hist, out->lat_z_hist, out->vac_hist, out->repl_hist};
+  for (int i = 0; i < 5; ++i)
+    if (hdst[i])
+      HIPCHECK(hipMemcpyAsync(hdst[i], M.hist + hoff[i], size_t(nb[i]) * sizeof(long long), hipMemcpyDeviceToHost,
+                              st));
+  HIPCHECK(hipStreamSynchronize(st));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  read_results(resh, E0, out);
+  out->kernel_ms = ms;
+  out->grid = sh.grid;
+  out->block = sh.block;
+  out->waves = sh.waves;
+  return device_errors(resh + kResCtrl);
 }
 
 int mt_stopping_batch_arith(mt_ctx* ctx, int arith, int Z1, double m1, int mat, const double* E, long long n,

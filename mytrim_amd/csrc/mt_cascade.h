@@ -153,6 +153,14 @@ constexpr int kDefaultWavesPerCu = 8;
 
 // device spans zeroed before a run
 constexpr int kMaxSpans = 16;
+// the ion species of a run, for mapping primaries to species indices on the
+// device (primary_species)
+struct SpeciesList {
+  int n;
+  int Z[kMaxSpecies];
+  double m[kMaxSpecies];
+};
+
 struct ZeroSpans {
   void* p[kMaxSpans];
   size_t bytes[kMaxSpans];

@@ -1162,6 +1162,27 @@ int zero_spans_grid(const ZeroSpans& z) {
 const void* zero_spans_function() { return (const void*)zero_spans; }
 const void* flush_energy_function() { return (const void*)flush_energy; }
 
+// Species index of each primary: the first entry of the run's species list
+// with its (Z, m); -1 where none matches (the cascade kernel rejects such a
+// flight where it starts).
+__global__ void __launch_bounds__(256) primary_species(SpeciesList L, const int* Z, const double* m, long long n,
+                                                       int* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    int s = -1;
+    for (int k = L.n - 1; k >= 0; --k)
+      if (L.Z[k] == Z[i] && L.m[k] == m[i]) s = k;
+    out[i] = s;
+  }
+}
+
+hipError_t primary_species_launch(const SpeciesList& L, const int* Z, const double* m, long long n, int* out,
+                                  hipStream_t st) {
+  const long long g = (n + 255) / 256;
+  hipLaunchKernelGGL(primary_species, dim3((unsigned)(g < 4096 ? (g > 0 ? g : 1) : 4096)), dim3(256), 0, st, L, Z, m,
+                     n, out);
+  return hipGetLastError();
+}
+
 hipError_t zero_spans_launch(const ZeroSpans& z, hipStream_t st) {
   hipLaunchKernelGGL(zero_spans, dim3(zero_spans_grid(z)), dim3(256), 0, st, z);
   return hipGetLastError();

@@ -35,6 +35,7 @@ EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt
            "mt_default_tables",
            "mt_ctx_set_target", "mt_ctx_set_tuning", "mt_run_cascades", "mt_run_cascades_log", "mt_autotune",
            "mt_trim_flight",
+           "mt_run_cascades_noalloc",
            "mt_stopping_batch", "mt_scatter_batch", "mt_stopping_batch_arith", "mt_scatter_batch_arith",
            "mt_surrogate_norm", "mt_surrogate_train", "mt_surrogate_install",
            "mt_surrogate_eval", "mt_surrogate_grad"]
@@ -138,6 +139,8 @@ def lib():
                                           C.POINTER(LogSpec), C.POINTER(Tallies)]
         L.mt_run_cascades.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
                                       C.POINTER(Tallies)]
+        L.mt_run_cascades_noalloc.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts),
+                                              C.POINTER(Tallies)]
         L.mt_autotune.argtypes = [C.c_void_p, C.POINTER(Primaries), C.POINTER(RunOpts), C.c_longlong,
                                   C.c_int, C.POINTER(Tuning), dp]
         L.mt_trim_flight.argtypes = [C.c_void_p, C.POINTER(Ion), dp, C.c_longlong, lp,
@@ -264,11 +267,14 @@ class Engine:
         del keep
         return best, rate.value
 
-    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64, log=None) -> dict:
+    def run(self, prob: Problem, per_history: bool = False, arith: int = MT_FP64, log=None,
+            noalloc: bool = False) -> dict:
         """Full cascades for every history of `prob` (sets the target if needed).
         `log` (optional): dict(mask=MT_LOG_*, max_records=n, vacmap_Z=(z1, z2, z3));
         the result then holds "records" (RECORD array), "n_records", "vacmap" and
-        "transport" (n_mat x 93 net atom moves per layer and Z, MT_LOG_TRANSPORT)."""
+        "transport" (n_mat x 93 net atom moves per layer and Z, MT_LOG_TRANSPORT).
+        `noalloc`: through mt_run_cascades_noalloc (the device's static scratch;
+        no per-history records or streams)."""
         pk, o, keep = self._launch_args(prob, per_history, arith)
         out = Tallies()
         hists = {"range_hist": np.zeros(prob.range_bins.nb + 2, np.int64),
@@ -286,7 +292,12 @@ class Engine:
                    "h_z": np.zeros(n), "h_state": np.zeros(n, np.int32), "h_Eel": np.zeros(n)}
             for k, a in per.items():
                 setattr(out, k, _p(a, {np.int32: ip, np.int64: lp, np.float64: dp}[a.dtype.type]))
-        if log is None:
+        if noalloc:
+            if log is not None or per_history:
+                raise ValueError("the no-allocation run keeps no per-history records or streams")
+            _check(lib().mt_run_cascades_noalloc(self.h, C.byref(pk), C.byref(o), C.byref(out)),
+                   "mt_run_cascades_noalloc")
+        elif log is None:
             _check(lib().mt_run_cascades(self.h, C.byref(pk), C.byref(o), C.byref(out)),
                    "mt_run_cascades")
         else:
