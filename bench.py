@@ -79,14 +79,35 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}), flush=True)
 
 
-def train_surrogate(eng, gpu, prob, n_train: int, epochs: int):
-    opts = gpu.Engine.surrogate_opts(prob, n_train=n_train, epochs=epochs, bf16=True)
+def train_surrogate(eng, gpu, prob, n_train: int, epochs: int, **kw):
+    """Train on the device and install; kw: inputs, sampling, eps_min, eps_max."""
+    opts = gpu.Engine.surrogate_opts(prob, n_train=n_train, epochs=epochs, bf16=True, **kw)
     t0 = time.perf_counter()
     params, loss = eng.surrogate_train(opts, gpu.Engine.surrogate_init(1))
     t_train = time.perf_counter() - t0
     eng.surrogate_install(opts, params)
+    norm = eng.surrogate_norm(opts)
     return {"train_s": t_train, "final_loss": float(loss[-1]), "n_train": n_train, "epochs": epochs,
-            "domain": {k: v for k, v in eng.surrogate_norm(opts).items() if k in ("eps_lo", "eps_hi", "b_hi")}}
+            "inputs": opts.inputs, "sampling": ["as the cascades draw", "rectangle"][opts.sampling],
+            "domain": {k: v for k, v in norm.items() if k in ("eps_lo", "eps_hi", "b_hi")},
+            "accuracy": surrogate_accuracy(eng, gpu, norm)}
+
+
+def surrogate_accuracy(eng, gpu, norm, n=200000):
+    """|d ln s2| of the installed surrogate's scattering step against the
+    device's FP64 solve, on eps log-uniform and b uniform over the part of the
+    domain the surrogate answers (eps <= 10)."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    hi = min(norm["eps_hi"], 10.0)
+    eps = np.exp(rng.uniform(np.log(norm["eps_lo"]), np.log(hi), n))
+    b = rng.uniform(0.0, norm["b_hi"], n)
+    s2s, _, _, its = eng.scatter(eps, b, 0, arith=gpu.MT_FP32_SURROGATE)
+    s2d, _, _, _ = eng.scatter(eps, b, 0)
+    ok = (its < 0) & (s2d > 0) & (s2s > 0)
+    d = np.abs(np.log(s2s[ok]) - np.log(s2d[ok]))
+    return {"pairs": int(ok.sum()), "median_abs_dln_s2": float(np.median(d)),
+            "p99_abs_dln_s2": float(np.quantile(d, 0.99))}
 
 
 def tally_errors(ref, r):
@@ -107,7 +128,7 @@ def _tuned(eng, sample, arith, n_sample):
         return None
     best, rate = eng.autotune(sample, sample=sample.n_hist, arith=arith)
     return {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
-            "sample_rate": rate}
+            "primary_waves64": getattr(best, "primary_waves64", 0), "sample_rate": rate}
 
 
 def step_problem(prob, i, n_hist):
@@ -154,6 +175,7 @@ def main():
     ap.add_argument("--variant-histories", type=int, default=100000,
                     help="histories for the FP32 and FP32+surrogate lines (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=4_000_000)
+    ap.add_argument("--surrogate-train-configured", type=int, default=10_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=60)
     ap.add_argument("--cpu-histories", type=int, default=0, help="0 = size to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
@@ -191,6 +213,7 @@ def main():
         n_sample = int(min(max(args.autotune_sample, 3e9 / per_hist), args.histories))
         best, rate = eng.autotune(step_problem(prob, 0, n_sample).with_(hist0=spare + 2000), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
+                  "primary_waves64": getattr(best, "primary_waves64", 0),
                   "sample_histories": n_sample, "sample_rate": rate}
     wsize = max(1, min(20000, args.histories))
     for i in range(args.warmup):
@@ -209,6 +232,12 @@ def main():
         sample = step_problem(prob, 0, nv).with_(hist0=spare)
         t32 = _tuned(eng, sample, gpu.MT_FP32, args.autotune_sample)
         c32, w32, k32, r32 = timed(eng, prob, gpu.MT_FP32, nv, 1)
+        # the configured network: 2 -> 64 -> 64 -> 1 on 1e7 (log eps, b) pairs of a
+        # rectangle, eps in [1e-4, 1e4]
+        surc = train_surrogate(eng, gpu, prob, args.surrogate_train_configured, args.surrogate_epochs // 2,
+                               inputs=2, sampling=1, eps_min=1e-4, eps_max=1e4)
+        csc, wsc, ksc, rsc = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
+        # this build's network: 3 inputs, trained on the pairs the cascades draw
         sur = train_surrogate(eng, gpu, prob, args.surrogate_train, args.surrogate_epochs)
         tsur = _tuned(eng, sample, gpu.MT_FP32_SURROGATE, args.autotune_sample)
         cs, ws, ks, rs = timed(eng, prob, gpu.MT_FP32_SURROGATE, nv, 1)
@@ -229,6 +258,13 @@ def main():
                                "roofline": roofline(cs, 1, ks, w_eq, FP32_UNPACKED_PEAK_TFLOPS, pipe32,
                                                     "cascade_kernel<2,0,0,false>"),
                                "training": sur},
+            "fp32_surrogate_configured": {
+                "collisions_per_s": csc / wsc, "kernel_ms": ksc, "speedup_vs_fp64": (csc / wsc) / (c64 / w64),
+                "surrogate_share": rsc[0]["surrogate"] / max(rsc[0]["n_coll"], 1),
+                "tally_rel_err_vs_fp64": tally_errors(r64[0], rsc[0]),
+                "roofline": roofline(csc, 1, ksc, w_eq, FP32_UNPACKED_PEAK_TFLOPS, pipe32,
+                                     "cascade_kernel<2,0,0,false>"),
+                "training": surc},
         }
     eng.close()
 

@@ -145,6 +145,7 @@ typedef struct {
   long long queue_peak;  /* most recoils held in the HBM queue */
   long long queue_ops;   /* wave-level HBM queue transfers */
   long long surrogate;   /* collisions whose angle came from the surrogate */
+  long long coll_primary;/* collisions of gen-0 flights (the primaries' share of the work) */
 } mt_tallies;
 
 /* Tuning knobs of the persistent kernel (0 = default). */
@@ -156,6 +157,11 @@ typedef struct {
   int lds_hist;       /* bin vacancy/replacement depths in LDS first */
   int graph;          /* >= 0: replay the run sequence (reset, kernel, flush,
                          copies) from a cached hipGraph; < 0: launch directly */
+  int primary_waves64;/* 0: every wave admits primaries and keeps its recoils;
+                         k in 1..63: k waves of every 64 admit the primaries and
+                         hand every recoil to the HBM queue while primaries
+                         remain, the others run recoils only (waves of one ion
+                         class take one stopping branch) */
 } mt_tuning;
 
 /* One flight (TrimBase::trim) for the drop-in single-call path. */
@@ -236,6 +242,13 @@ typedef struct {
   double lr;               /* Adam step size, cosine-annealed to lr / 100 */
   uint32_t seed;
   int bf16;                /* inference with BF16 dot products (else FP32 FMA) */
+  int inputs;              /* 0 or 3: (log eps, log b, b); 2: (log eps, b), the log b
+                              input held at zero (a 2 -> 64 -> 64 -> 1 network) */
+  int sampling;            /* 0: pairs drawn as the target's cascades draw them;
+                              1: eps log-uniform on [eps_min, eps_max], b uniform on
+                              [0, b_max], b_max the widest reduced impact parameter
+                              the cascades draw */
+  double eps_min;          /* sampling 1: lower end of eps */
 } mt_surrogate_opts;
 
 const char* mt_last_error(void);

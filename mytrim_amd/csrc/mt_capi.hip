@@ -46,7 +46,7 @@ struct FlightParams {
   long long* n_used;
 };
 hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t stream);
-hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo,
+hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo, int rect,
                          const MlpNorm& nm, float* X, float* T, hipStream_t st);
 hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
                      float* rec, float* part, float* g, float* m, float* v, int* t, float* batch_loss,
@@ -203,7 +203,7 @@ struct mt_ctx {
   HostTarget tgt;
   bool has_target = false;
   mt_tuning tune{};
-  DBuf tables, sh, cl, cx, cy, cz, cr, occ;
+  DBuf tables, sh, cl, cx, cy, cz, cr, occ, occ_rank, occ_off, occ_idx;
   DBuf pka_sp, pka_E, pka_pos, pka_dir;
   DBuf res, wave_energy, hist_all, h_out;
   HBuf stage_res, stage_hist;
@@ -228,7 +228,7 @@ struct mt_ctx {
 
   ~mt_ctx() {
     graph.release();
-    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &occ, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
+    DBuf* all[] = {&tables, &sh, &cl, &cx, &cy, &cz, &cr, &occ, &occ_rank, &occ_off, &occ_idx, &pka_sp, &pka_E, &pka_pos, &pka_dir, &res,
                    &wave_energy, &hist_all, &h_out, &scratch_a, &scratch_b, &scratch_c, &scratch_d, &scratch_e,
                    &log_rec, &log_map, &log_tr};
     for (DBuf* b : all) b->release();
@@ -353,6 +353,7 @@ int mt_ctx_set_tuning(mt_ctx* ctx, const mt_tuning* t) {
     return fail(MT_E_ARG, "block must be a multiple of 64 and at most 256 (the kernel's launch bound)");
   if (t->queue_cap && (t->queue_cap & (t->queue_cap - 1))) return fail(MT_E_ARG, "queue_cap must be a power of two");
   if (t->stack_cap < 0 || t->stack_cap > 1024) return fail(MT_E_ARG, "stack_cap out of range");
+  if (t->primary_waves64 < 0 || t->primary_waves64 > 63) return fail(MT_E_ARG, "primary_waves64 out of range 0..63");
   ctx->tune = *t;
   return MT_OK;
 }
@@ -473,7 +474,9 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
       const long long nbits = on[0] * on[1] * on[2];
       std::vector<uint32_t> bits(size_t((nbits + 31) / 32), 0u);
       const double pad = 1e-9 * std::max(g.w[0], std::max(g.w[1], g.w[2]));
-      for (const HostCluster& c : tg.clusters) {
+      std::vector<std::pair<uint64_t, int>> meets;  // (cell, cluster) for the candidate lists
+      for (int ci = 0; ci < int(tg.clusters.size()); ++ci) {
+        const HostCluster& c = tg.clusters[size_t(ci)];
         const double cc[3] = {c.x, c.y, c.z}, rr = c.r + pad;
         long long lo[3], hi[3];
         for (int i = 0; i < 3; ++i) {
@@ -501,12 +504,65 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
                 b += (((j[i] % on[i]) + on[i]) % on[i]) * mul;
                 mul *= on[i];
               }
-              if (d2 <= rr * rr) bits[size_t(b >> 5)] |= 1u << (b & 31);
+              if (d2 <= rr * rr) {
+                bits[size_t(b >> 5)] |= 1u << (b & 31);
+                meets.emplace_back(uint64_t(b), ci);
+              }
             }
       }
       HIPCHECK(ctx->occ.ensure(bits.size() * sizeof(uint32_t)));
       HIPCHECK(hipMemcpy(ctx->occ.p, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
       g.occ = ctx->occ.as<uint32_t>();
+      // candidate lists (the same (cell, cluster) pairs as the bits)
+      std::vector<uint32_t> rank(bits.size() + 1, 0u);
+      for (size_t w = 0; w < bits.size(); ++w) rank[w + 1] = rank[w] + uint32_t(__builtin_popcount(bits[w]));
+      const size_t nset = rank[bits.size()];
+      auto rank_of = [&](uint64_t b) {
+        const uint32_t word = bits[size_t(b >> 5)];
+        return rank[size_t(b >> 5)] + uint32_t(__builtin_popcount(word & ((1u << (b & 31)) - 1u)));
+      };
+      std::sort(meets.begin(), meets.end());
+      std::vector<int> off(nset + 1, 0), idx;
+      idx.reserve(meets.size());
+      auto overlap = [&](int a, int b) {  // the lookup's minimum-image distance
+        const HostCluster &p = tg.clusters[size_t(a)], &q = tg.clusters[size_t(b)];
+        double d[3] = {p.x - q.x, p.y - q.y, p.z - q.z}, r2 = 0.0;
+        for (int i = 0; i < 3; ++i) {
+          if (g.bc[i] == MT_PBC) d[i] -= std::round(d[i] * g.inv_w[i]) * g.w[i];
+          r2 += d[i] * d[i];
+        }
+        // touching within rounding counts as overlapping
+        return r2 < (p.r + q.r + pad) * (p.r + q.r + pad);
+      };
+      for (size_t i = 0; i < meets.size();) {
+        size_t e = i;
+        while (e < meets.size() && meets[e].first == meets[i].first) ++e;
+        bool clash = false;
+        for (size_t a = i; a < e && !clash; ++a)
+          for (size_t b = a + 1; b < e && !clash; ++b) clash = overlap(meets[a].second, meets[b].second);
+        const uint32_t k = rank_of(meets[i].first);
+        off[k] = int(idx.size());
+        if (clash) {
+          idx.push_back(-1);
+        } else {
+          for (size_t a = i; a < e; ++a) idx.push_back(meets[a].second);
+        }
+        i = e;
+      }
+      // every set bit has at least one pair: off is non-decreasing; close it
+      off[nset] = int(idx.size());
+      for (size_t k = nset; k-- > 0;)
+        if (off[k] > off[k + 1]) off[k] = off[k + 1];
+      if (idx.empty()) idx.push_back(-1);
+      HIPCHECK(ctx->occ_rank.ensure(rank.size() * sizeof(uint32_t)));
+      HIPCHECK(ctx->occ_off.ensure(off.size() * sizeof(int)));
+      HIPCHECK(ctx->occ_idx.ensure(idx.size() * sizeof(int)));
+      HIPCHECK(hipMemcpy(ctx->occ_rank.p, rank.data(), rank.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(ctx->occ_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(ctx->occ_idx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+      g.occ_rank = ctx->occ_rank.as<uint32_t>();
+      g.occ_off = ctx->occ_off.as<int>();
+      g.occ_idx = ctx->occ_idx.as<int>();
     }
     g.sh = ctx->sh.as<int>();
     g.cl = ctx->cl.as<int>();
@@ -724,6 +780,7 @@ static void fill_params(Params& P, const mt_ctx* ctx, const mt_primaries* pk, co
   P.vac_bw = o->vac_bw;
   P.vac_nb = o->vac_nb;
   P.lds_hist = sh.lds_hist;
+  P.primary_waves64 = ctx->tune.primary_waves64;
   if (o->arith == MT_FP32_SURROGATE) {
     P.mlp.p = ctx->mlp_p.as<float>();
     P.mlp.w2bf = ctx->mlp_w2bf.as<uint32_t>();
@@ -751,6 +808,7 @@ static void read_results(const unsigned long long* sr, double E0, mt_tallies* ou
   out->disp = (long long)cnt[cDisp];
   out->range_n = (long long)cnt[cRangeN];
   out->surrogate = (long long)cnt[cSurr];
+  out->coll_primary = (long long)cnt[cCollPrimary];
   out->E0 = E0;
   out->Eel = rsum[0] + rsum[1];
   out->Enuc = rsum[2] + rsum[3];
@@ -1042,6 +1100,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
     t.queue_cap = keep.queue_cap;
     t.lds_hist = keep.lds_hist;
     t.graph = keep.graph;
+    t.primary_waves64 = 0;
     cand.push_back(t);
   }
   double best_r = -1.0;
@@ -1071,6 +1130,44 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   if (best_r < 0.0) {
     ctx->tune = keep;
     return fail(MT_E_LIMIT, "no candidate launch shape fits");
+  }
+  // wave roles at the best shape: feeder shares around the primaries' share
+  // of the collisions (from a run at that shape), kept if faster
+  {
+    ctx->tune = best_t;
+    mt_tallies probe{};
+    const int rc = mt_run_cascades(ctx, &s, &ro, &probe);
+    if (rc != MT_OK) {
+      ctx->tune = keep;
+      return rc;
+    }
+    const double share = probe.n_coll > 0 ? double(probe.coll_primary) / double(probe.n_coll) : 0.0;
+    int tried[4] = {0, 0, 0, 0};
+    int n_tried = 0;
+    for (double f : {1.0, 1.5, 2.0}) {
+      const int k = std::min(63, std::max(1, int(std::lround(64.0 * std::min(1.0, f * share)))));
+      bool dup = false;
+      for (int i = 0; i < n_tried; ++i) dup = dup || tried[i] == k;
+      if (dup) continue;
+      tried[n_tried++] = k;
+      mt_tuning t = best_t;
+      t.primary_waves64 = k;
+      ctx->tune = t;
+      double r = 0.0;
+      for (int i = 0; i < reps; ++i) {
+        mt_tallies out{};
+        const int rc2 = mt_run_cascades(ctx, &s, &ro, &out);
+        if (rc2 != MT_OK) {
+          ctx->tune = keep;
+          return rc2;
+        }
+        if (out.kernel_ms > 0.0) r = std::max(r, double(out.n_coll) / (1e-3 * out.kernel_ms));
+      }
+      if (r > best_r) {
+        best_r = r;
+        best_t = t;
+      }
+    }
   }
   ctx->tune = best_t;
   *best = best_t;
@@ -1148,6 +1245,9 @@ static std::string surrogate_setup(const mt_ctx* ctx, const mt_surrogate_opts* o
   if (o->n_ions < 1 || !o->ion_Z || !o->ion_m) return "need at least one ion species";
   if (!(o->E_min > 0.0) || !(o->E_max > o->E_min)) return "need 0 < E_min < E_max";
   if (!(o->eps_max > 0.0)) return "need eps_max > 0";
+  if (o->inputs != 0 && o->inputs != 2 && o->inputs != 3) return "inputs must be 2 or 3";
+  if (o->sampling != 0 && o->sampling != 1) return "sampling must be 0 or 1";
+  if (o->sampling == 1 && !(o->eps_min > 0.0 && o->eps_min < o->eps_max)) return "need 0 < eps_min < eps_max";
   std::vector<Species> species;
   std::vector<int> ps;
   std::string msg = collect_species(ctx->tgt, o->ion_Z, o->ion_m, o->n_ions, species, ps);
@@ -1176,13 +1276,18 @@ static std::string surrogate_setup(const mt_ctx* ctx, const mt_surrogate_opts* o
       }
     }
   if (tri.empty()) return "no collision of these ions reaches the surrogate's domain";
+  if (o->sampling == 1) {  // the rectangle: its own eps range, the cascades' widest b
+    eps_lo = o->eps_min;
+    eps_hi = o->eps_max;
+  }
   const double l0 = std::log10(eps_lo), l1 = std::log10(eps_hi);
   const double b_floor = 1e-4 * b_hi;
   const double m0 = std::log10(b_floor), m1 = std::log10(b_hi);
   nm.e_mid = float(0.5 * (l0 + l1));
   nm.e_inv_half = float(2.0 / std::max(l1 - l0, 1e-6));
   nm.lb_mid = float(0.5 * (m0 + m1));
-  nm.lb_inv_half = float(2.0 / (m1 - m0));
+  // two inputs: the log b input is held at zero (its weights get no gradient)
+  nm.lb_inv_half = o->inputs == 2 ? 0.0f : float(2.0 / (m1 - m0));
   nm.b_mid = float(0.5 * b_hi);
   nm.b_inv_half = float(2.0 / b_hi);
   nm.b_floor = float(b_floor);
@@ -1235,7 +1340,7 @@ int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* o, float* params, d
   HIPCHECK(hipMemsetAsync(ctx->mlp_m.p, 0, kMlpParams * sizeof(float), st));
   HIPCHECK(hipMemsetAsync(ctx->mlp_v.p, 0, kMlpParams * sizeof(float), st));
   HIPCHECK(hipMemsetAsync(ctx->mlp_t.p, 0, sizeof(int), st));
-  HIPCHECK(mlp_make_data(o->seed, n, ctx->mlp_tri.as<MlpTriple>(), int(tri.size()), o->E_min, nm,
+  HIPCHECK(mlp_make_data(o->seed, n, ctx->mlp_tri.as<MlpTriple>(), int(tri.size()), o->E_min, o->sampling, nm,
                          ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), st));
   HIPCHECK(mlp_train(ctx->mlp_p.as<float>(), ctx->mlp_X.as<float>(), ctx->mlp_T.as<float>(), n, B, o->epochs,
                      float(o->lr), ctx->mlp_rec.as<float>(), ctx->mlp_part.as<float>(), ctx->mlp_g.as<float>(),

@@ -64,7 +64,7 @@ enum ErrBits {
 
 enum Counter {
   cColl = 0, cFlight, cVac, cRepl, cSubst, cInter, cLost, cVacuum, cRuth, cNewton, cDisp,
-  cRangeN, cSurr, kNumCounters = 16
+  cRangeN, cSurr, cCollPrimary, kNumCounters = 16
 };
 
 enum Energy { eEel = 0, eEnuc, eEunf, eElost, eEdrop, eEvac, eRangeSum, eRangeSum2, kNumEnergy = 8 };
@@ -119,6 +119,7 @@ struct Params {
   double budget, f_init;
   long long c_floor;
   int* h_out;                   // per history: flights queued or running
+  int primary_waves64;          // waves per 64 that feed primaries (0: all waves do everything)
   long long max_ic;             // collisions per flight before kErrRunaway
   unsigned long long max_wait_ticks;  // wall-clock ticks before kErrTimeout
   // tallies

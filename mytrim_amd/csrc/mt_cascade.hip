@@ -278,6 +278,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
   const int home = (P.q.nq >= 8 ? (int)(blockIdx.x & 7u) * per_xcd + wid % per_xcd
                                 : (int)(blockIdx.x * nwb + wid)) & (P.q.nq - 1);
   int victim = home;       // last shard this wave stole from
+  // wave roles (P.primary_waves64 > 0): feeder waves admit the primaries and
+  // send every recoil to the HBM queue while primaries remain; the others
+  // never admit primaries.  A feeder whose primaries are used up works like
+  // any other wave.
+  const bool roles = P.primary_waves64 > 0;
+  const bool feeder_wave = roles && ((int)((blockIdx.x * nwb + wid) & 63u) < P.primary_waves64);
+  bool feeding = feeder_wave;  // wave-uniform: primaries were left at the last refill
   int poll_skip = 0;       // iterations before the shared queue is polled again
   long long peak_seen = 0; // largest queue fill this wave has reported
   unsigned long long fe_pending = 0;  // ended flights not yet added to kFlightsEnded
@@ -303,7 +310,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
   // counted when the flight ends (its collision index), displacements are
   // vacancies + replacements + substitutions.
   unsigned l_newton32 = 0, l_kpvac32 = 0, l_coll32 = 0, l_ruth32 = 0, l_surr32 = 0, l_vac32 = 0;
-  unsigned l_flight32 = 0, l_fate32[4] = {0u, 0u, 0u, 0u}, l_vacuum32 = 0, l_rangen32 = 0;
+  unsigned l_flight32 = 0, l_fate32[4] = {0u, 0u, 0u, 0u}, l_vacuum32 = 0, l_rangen32 = 0, l_coll0_32 = 0;
   double eel_s = 0.0, eel_e = 0.0, enuc_s = 0.0, enuc_e = 0.0;
   unsigned err = 0;
 
@@ -327,19 +334,26 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         const ArgPtr A = cold_args();
         // one lane loads one entry, so a batch holds at most 64
         const int want = min(64, min(cap - scount, max(n_idle, cap / 2) - scount));
+        // a feeder wave takes primaries only, until they are used up
+        if (feeding) feeding = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
         // the home shard first, then up to three others (rotating victim)
         int qs = home;
-        uint64_t h0 = ld_relaxed(&ctrl[kQHead(qs)]);
-        uint64_t t0 = ld_relaxed(&ctrl[kQTail(qs)]);
-        for (int v = 0; v < 3 && t0 <= h0 && A->q.nq > 1; ++v) {
-          victim = (victim + 1) & (A->q.nq - 1);
-          if (victim == home) victim = (victim + 1) & (A->q.nq - 1);
-          qs = victim;
+        uint64_t h0 = 0, t0 = 0;
+        if (!feeding) {
           h0 = ld_relaxed(&ctrl[kQHead(qs)]);
           t0 = ld_relaxed(&ctrl[kQTail(qs)]);
+          for (int v = 0; v < 3 && t0 <= h0 && A->q.nq > 1; ++v) {
+            victim = (victim + 1) & (A->q.nq - 1);
+            if (victim == home) victim = (victim + 1) & (A->q.nq - 1);
+            qs = victim;
+            h0 = ld_relaxed(&ctrl[kQHead(qs)]);
+            t0 = ld_relaxed(&ctrl[kQTail(qs)]);
+          }
         }
         const bool may_pop = t0 > h0;
-        bool may_pka = !may_pop && ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
+        // with roles, only feeder waves admit primaries
+        bool may_pka = !may_pop && (!roles || feeder_wave) &&
+                       ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
         long long n_adm = 0;
         if (may_pka) {  // cascade budget (see Params)
           const uint64_t live = ld_relaxed(&ctrl[kLive]);
@@ -807,7 +821,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     const uint64_t push_m = __ballot(push);
     if (push_m) {
       const int n_push = __popcll(push_m);
-      const int room = cap - scount;
+      // a feeding wave keeps no recoils: they go to the waves that run recoils
+      const int room = feeding ? 0 : cap - scount;
       const int r = lane_rank(push_m);
       const bool over = push && r >= room;  // goes to the HBM queue
       const uint32_t meta = (uint32_t)rsp | ((uint32_t)(F.gen + 1) << 8);
@@ -941,6 +956,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       if (fin) {
         ++l_flight32;
         l_coll32 += (unsigned)(vacuum ? F.ic - 1 : F.ic);
+        l_coll0_32 += F.gen == 0 ? (unsigned)(vacuum ? F.ic - 1 : F.ic) : 0u;
         l_vacuum32 += vacuum ? 1u : 0u;
 #pragma unroll
         for (int f = 0; f < 4; ++f) l_fate32[f] += (state == f + 1) ? 1u : 0u;
@@ -998,7 +1014,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
                            n_repl = wsum(l_fate32[0]), n_subst = wsum(l_fate32[1]), n_inter = wsum(l_fate32[2]),
                            n_lost = wsum(l_fate32[3]), n_vacuum = wsum(l_vacuum32), n_ruth = wsum(l_ruth32),
                            n_newton = wsum(l_newton32), n_kpvac = wsum(l_kpvac32), n_rangen = wsum(l_rangen32),
-                           n_surr = wsum(l_surr32);
+                           n_surr = wsum(l_surr32), n_coll0 = wsum(l_coll0_32);
   for (int off = 32; off > 0; off >>= 1) {
     const double s_a = __shfl_xor(eel_s, off), e_a = __shfl_xor(eel_e, off);
     kadd(eel_s, eel_e, s_a);
@@ -1022,6 +1038,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
     atomicAdd(&C[cDisp], n_vac + n_repl + n_subst);
     atomicAdd(&C[cRangeN], n_rangen);
     atomicAdd(&C[cSurr], n_surr);
+    atomicAdd(&C[cCollPrimary], n_coll0);
     atomicAdd(&ctrl[kQueueOps], c_qops);
     if (fe_pending) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
     const int gw = blockIdx.x * nwb + wid;

@@ -40,7 +40,30 @@ __device__ __forceinline__ int lookup_cluster(const Geometry& g, const double po
   // would find none); this skips the (2 ks + 1)^3-cell scan for most positions
   {
     const uint64_t bit = (uint64_t)oc[0] + (uint64_t)g.on[0] * ((uint64_t)oc[1] + (uint64_t)g.on[1] * (uint64_t)oc[2]);
-    if (((g.occ[bit >> 5] >> (bit & 31u)) & 1u) == 0u) return -1;
+    const uint32_t word = g.occ[bit >> 5];
+    const uint32_t m = 1u << (bit & 31u);
+    if ((word & m) == 0u) return -1;
+    // occupied cell: its listed clusters are every cluster that can contain
+    // the position.  When they do not overlap at most one contains it, and
+    // the reference's scan returns that one (or none): test them with the
+    // scan's own distance.  A list of overlapping clusters is the single
+    // entry -1: scan as the reference does.
+    const uint32_t k = g.occ_rank[bit >> 5] + (uint32_t)__popc(word & (m - 1u));
+    const int o = g.occ_off[k], e = g.occ_off[k + 1];
+    if (g.occ_idx[o] >= 0) {
+      for (int q = o; q < e; ++q) {
+        const int l = g.occ_idx[q];
+        double d0 = pos[0] - g.cx[l];
+        double d1 = pos[1] - g.cy[l];
+        double d2 = pos[2] - g.cz[l];
+        if (g.bc[0] == 0) d0 -= round(d0 * g.inv_w[0]) * g.w[0];
+        if (g.bc[1] == 0) d1 -= round(d1 * g.inv_w[1]) * g.w[1];
+        if (g.bc[2] == 0) d2 -= round(d2 * g.inv_w[2]) * g.w[2];
+        const double rc = g.cr[l];
+        if (d0 * d0 + d1 * d1 + d2 * d2 < rc * rc) return l;
+      }
+      return -1;
+    }
   }
   // hash cell as the reference computes it (sample_clusters.C:59-81)
   for (int i = 0; i < 3; ++i) {

@@ -50,9 +50,12 @@ struct MlpDev {
   int bf16;
 };
 
+// tanh(x) = (1 - e) / (1 + e) with e = exp(-2|x|) in (0, 1], the sign of x
+// restored; 1 / (1 + e) from the hardware reciprocal (about 1 ulp), not an
+// IEEE division (a Newton-refined sequence of a dozen instructions)
 __device__ __forceinline__ float mlp_tanh(float x) {
   const float e = __expf(-2.0f * fabsf(x));
-  const float r = (1.0f - e) / (1.0f + e);
+  const float r = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
   return copysignf(r, x);
 }
 

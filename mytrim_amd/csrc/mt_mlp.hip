@@ -1,8 +1,9 @@
 // Training and evaluation kernels of the scattering surrogate (mt_mlp.h).
 //
 //   mlp_data      training pairs sampled as the cascade samples (eps, b),
-//                 target ln s2 from scatter_stable<double> (universal
-//                 potential), one Philox stream per sample
+//                 or on a rectangle in (log eps, b); target ln s2 from
+//                 scatter_stable<double> (universal potential), one Philox
+//                 stream per sample
 //   mlp_fwdbwd    one lane per sample: forward pass and the per-sample
 //                 back-propagated deltas (stored for the gradient kernel)
 //   mlp_grad_*    per parameter: batch sum of delta x activation in fixed
@@ -31,21 +32,31 @@ __device__ __forceinline__ double mlp_pmax(const MlpTriple& t, double E) {
   return t.a / (eeg + sqrt(eeg) + 0.125 * pow(eeg, 0.1));
 }
 
-// Training pairs drawn the way the cascade draws them: a random triple, a
-// flight energy (half log-uniform, half uniform on [E_lo, E_hi of the
-// triple]), eps = fi E and b = pmax(E) sqrt(u) / ai; labels from the FP64
-// MAGIC solve.  One Philox stream per sample.
+// Training pairs, one Philox stream per sample; labels from the FP64 MAGIC
+// solve.  Two samplings:
+//   rect = 0: the way the cascade draws them: a random triple, a flight
+//             energy (half log-uniform, half uniform on [E_lo, E_hi of the
+//             triple]), eps = fi E and b = pmax(E) sqrt(u) / ai;
+//   rect = 1: a rectangle: eps log-uniform on [eps_lo, eps_hi], b uniform on
+//             [0, b_hi].
 __global__ void __launch_bounds__(256) mlp_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri,
-                                                double E_lo, MlpNorm nm, float* X, float* T) {
+                                                double E_lo, int rect, double eps_lo, double eps_hi, double b_hi,
+                                                MlpNorm nm, float* X, float* T) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const P4 r = philox10((uint32_t)i, (uint32_t)(i >> 32), 0x5a17u, 0u, seed, 0x6d6c70u);
     const P4 r2 = philox10((uint32_t)i, (uint32_t)(i >> 32), 0x5a18u, 0u, seed, 0x6d6c70u);
-    const int k = min((int)(u32d(r.x) * n_tri), n_tri - 1);
-    const MlpTriple t = tri[k];
     const double u = u53(r.y, r.z);
-    const double E = (r2.x & 1u) ? E_lo * pow(t.E_hi / E_lo, u) : E_lo + (t.E_hi - E_lo) * u;
-    const double eps = t.fi * E;
-    const double b = mlp_pmax(t, E) * sqrt(u32d(r.w)) / t.ai;
+    double eps, b;
+    if (rect) {
+      eps = eps_lo * pow(eps_hi / eps_lo, u);
+      b = b_hi * u32d(r.w);
+    } else {
+      const int k = min((int)(u32d(r.x) * n_tri), n_tri - 1);
+      const MlpTriple t = tri[k];
+      const double E = (r2.x & 1u) ? E_lo * pow(t.E_hi / E_lo, u) : E_lo + (t.E_hi - E_lo) * u;
+      eps = t.fi * E;
+      b = mlp_pmax(t, E) * sqrt(u32d(r.w)) / t.ai;
+    }
     double s2, ct, st;
     int it;
     scatter_stable<double, 0>(eps, b, s2, ct, st, it);
@@ -199,9 +210,10 @@ __global__ void mlp_pack_bf16(const float* p, uint32_t* w2bf) {
 // ------------------------------------------------------------------ host side
 static int grid_of(long long n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
 
-hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo,
+hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo, int rect,
                          const MlpNorm& nm, float* X, float* T, hipStream_t st) {
-  hipLaunchKernelGGL(mlp_data, dim3(grid_of(n)), dim3(256), 0, st, seed, n, tri, n_tri, E_lo, nm, X, T);
+  hipLaunchKernelGGL(mlp_data, dim3(grid_of(n)), dim3(256), 0, st, seed, n, tri, n_tri, E_lo, rect,
+                     (double)nm.eps_lo, (double)nm.eps_hi, (double)nm.b_hi, nm, X, T);
   return hipGetLastError();
 }
 

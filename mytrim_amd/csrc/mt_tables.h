@@ -106,6 +106,14 @@ struct Geometry {
   int on[3];
   double on_w[3];  // on[i] / w[i]
   const uint32_t* occ;
+  // candidate lists of the occupied cells: the k-th set bit of occ (k =
+  // occ_rank[word] + the set bits below it in its word) owns clusters
+  // occ_idx[occ_off[k] .. occ_off[k + 1]), every cluster whose padded sphere
+  // meets the cell.  When two of them overlap, the list is the single entry
+  // -1 and the lookup scans the hash as the reference does.
+  const uint32_t* occ_rank;
+  const int* occ_off;
+  const int* occ_idx;
 };
 
 }  // namespace mt

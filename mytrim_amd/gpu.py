@@ -82,19 +82,21 @@ class Tallies(C.Structure):
         ("repl_hist", lp), ("h_vac", ip), ("h_coll", lp), ("h_flights", ip), ("h_x", dp),
         ("h_y", dp), ("h_z", dp), ("h_state", ip), ("h_Eel", dp), ("kernel_ms", C.c_double),
         ("grid", C.c_int), ("block", C.c_int), ("waves", C.c_int), ("queue_peak", C.c_longlong),
-        ("queue_ops", C.c_longlong), ("surrogate", C.c_longlong)])
+        ("queue_ops", C.c_longlong), ("surrogate", C.c_longlong), ("coll_primary", C.c_longlong)])
 
 
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("stack_cap", C.c_int),
-                ("queue_cap", C.c_longlong), ("lds_hist", C.c_int), ("graph", C.c_int)]
+                ("queue_cap", C.c_longlong), ("lds_hist", C.c_int), ("graph", C.c_int),
+                ("primary_waves64", C.c_int)]
 
 
 class SurrogateOpts(C.Structure):
     _fields_ = [("n_ions", C.c_int), ("ion_Z", C.POINTER(C.c_int)), ("ion_m", C.POINTER(C.c_double)),
                 ("E_min", C.c_double), ("E_max", C.c_double), ("eps_max", C.c_double),
                 ("n_train", C.c_longlong), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
-                ("seed", C.c_uint32), ("bf16", C.c_int)]
+                ("seed", C.c_uint32), ("bf16", C.c_int), ("inputs", C.c_int), ("sampling", C.c_int),
+                ("eps_min", C.c_double)]
 
 
 N_SURROGATE_PARAMS = 4481
@@ -204,8 +206,8 @@ class Engine:
         except Exception:
             pass
 
-    def tune(self, block=0, blocks_per_cu=0, stack_cap=0, queue_cap=0, lds_hist=0, graph=0):
-        t = Tuning(block, blocks_per_cu, stack_cap, queue_cap, lds_hist, graph)
+    def tune(self, block=0, blocks_per_cu=0, stack_cap=0, queue_cap=0, lds_hist=0, graph=0, primary_waves64=0):
+        t = Tuning(block, blocks_per_cu, stack_cap, queue_cap, lds_hist, graph, primary_waves64)
         _check(lib().mt_ctx_set_tuning(self.h, C.byref(t)), "mt_ctx_set_tuning")
 
     def set_tables(self, rows: np.ndarray):
@@ -321,7 +323,8 @@ class Engine:
                        transport=trans)
         res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
                    kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
-                   queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate)
+                   queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate,
+                   coll_primary=out.coll_primary)
         res.update(hists)
         res.update(per)
         del keep
@@ -359,10 +362,13 @@ class Engine:
     # ---------------------------------------------------------- surrogate
     @staticmethod
     def surrogate_opts(prob: Problem, E_min: float = 3.0, eps_max: float = 1.0, n_train=4_000_000, epochs=60,
-                       batch=4096, lr=4e-3, seed=7, bf16=False) -> SurrogateOpts:
+                       batch=4096, lr=4e-3, seed=7, bf16=False, inputs=3, sampling=0,
+                       eps_min=1e-4) -> SurrogateOpts:
         """Options for the cascades of `prob`: its primary species (target atoms
         are added by the library) and flight energies up to its largest primary
-        energy.  The returned object keeps its arrays alive."""
+        energy.  inputs=2, sampling=1 with eps_min=1e-4, eps_max=1e4 is the
+        2 -> 64 -> 64 -> 1 network on a (log eps, b) rectangle.  The returned
+        object keeps its arrays alive."""
         p = prob.primaries
         Zs, ms = [], []
         for Z, m in zip(np.atleast_1d(p.Z), np.atleast_1d(p.m)):
@@ -372,7 +378,8 @@ class Engine:
         Za = np.array(Zs, np.int32)
         ma = np.array(ms, np.float64)
         o = SurrogateOpts(len(Zs), _p(Za, ip), _p(ma, dp), float(E_min), float(np.max(p.E)), float(eps_max),
-                          int(n_train), int(epochs), int(batch), float(lr), int(seed) & 0xFFFFFFFF, int(bf16))
+                          int(n_train), int(epochs), int(batch), float(lr), int(seed) & 0xFFFFFFFF, int(bf16),
+                          int(inputs), int(sampling), float(eps_min))
         o._keep = (Za, ma)
         return o
 

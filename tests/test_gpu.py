@@ -178,6 +178,31 @@ def test_cascade_histories_match_oracle(eng, case):
     assert tot == pytest.approx(g["E0"], rel=1e-11)
 
 
+@pytest.mark.parametrize("case", ["c1", "c4mix", "c5"])
+def test_wave_roles_leave_results_unchanged(eng, case):
+    """Feeder waves (primaries only, every recoil handed to the HBM queue)
+    change which wave runs a flight, not what the flight computes: integer
+    tallies and histograms equal those without roles, energies to 1e-12."""
+    prob = CASES[case]()
+    eng.tune()
+    a = eng.run(prob)
+    try:
+        for k64 in (8, 32):
+            eng.tune(primary_waves64=k64)
+            b = eng.run(prob)
+            print("%s roles %d/64: %.2f ms (without roles %.2f ms), primaries' share of collisions %.3f" % (
+                case, k64, b["kernel_ms"], a["kernel_ms"], b["coll_primary"] / b["n_coll"]))
+            for k in ["n_coll", "n_flight", "vac", "repl", "subst", "inter", "lost", "vacuum", "disp", "range_n",
+                      "coll_primary"]:
+                assert a[k] == b[k], k
+            for k in ["range_hist", "lat_y_hist", "lat_z_hist", "vac_hist", "repl_hist"]:
+                assert np.array_equal(a[k], b[k]), k
+            for k in ["Eel", "Enuc", "Eunf", "Elost", "Edrop", "Evac"]:
+                assert b[k] == pytest.approx(a[k], rel=1e-12, abs=1e-9), k
+    finally:
+        eng.tune()
+
+
 def test_cascade_bitwise_totals_when_histories_agree(eng):
     """Where every history agrees, the tallies must agree exactly."""
     prob = configs.c1_he_fe(300, seed=9)

@@ -168,33 +168,71 @@ def test_device_inference_matches_numpy(eng, trained, bf16):
     assert np.all(out == -1.0)
 
 
+@pytest.fixture(scope="module")
+def trained_configured(eng):
+    """The configured network: 2 -> 64 -> 64 -> 1 (inputs log eps and b), 1e7
+    pairs with eps log-uniform on [1e-4, 1e4] and b uniform on [0, b_max]."""
+    opts = gpu.Engine.surrogate_opts(PROB, n_train=10_000_000, epochs=30, seed=12, inputs=2, sampling=1,
+                                     eps_min=1e-4, eps_max=1e4)
+    p, loss = eng.surrogate_train(opts, gpu.Engine.surrogate_init(4))
+    return opts, p, loss
+
+
+def tally_check(name, a, s, n):
+    """Vacancies/ion, mean range and Eel/E0 of `s` within 1 % and within 2 sigma
+    (the statistical error of the two estimates, as if independent) of the
+    FP64 run `a` on the same histories."""
+    sig_v = math.sqrt((np.var(a["h_vac"]) + np.var(s["h_vac"])) / n)
+    dv = abs(s["vac"] - a["vac"]) / n
+    assert dv <= 0.01 * a["vac"] / n and dv <= 2 * sig_v, (name, "vac", dv, sig_v)
+    ma, ms = a["range_sum"] / a["range_n"], s["range_sum"] / s["range_n"]
+    va = a["range_sum2"] / a["range_n"] - ma * ma
+    vs = s["range_sum2"] / s["range_n"] - ms * ms
+    sig_r = math.sqrt(va / a["range_n"] + vs / s["range_n"])
+    assert abs(ms - ma) <= 0.01 * ma and abs(ms - ma) <= 2 * sig_r, (name, "range", ms, ma, sig_r)
+    ea, es = a["Eel"] / a["E0"], s["Eel"] / s["E0"]
+    sig_e = math.sqrt((np.var(a["h_Eel"]) + np.var(s["h_Eel"])) / n) / (a["E0"] / n)
+    assert abs(es - ea) <= 0.01 * ea and abs(es - ea) <= 2 * sig_e, (name, "Eel/E0", es, ea, sig_e)
+
+
 @pytest.mark.gpu
-def test_surrogate_cascade_ablation(eng, trained):
-    """C -> W at 2 MeV: FP32 + surrogate against FP32 MAGIC and FP64 MAGIC on
-    the same histories (vacancies/ion, electronic share, mean range)."""
-    opts, p, _ = trained
-    res = {}
-    for bf16 in (0, 1):
+def test_surrogate_cascade_ablation(eng, trained, trained_configured):
+    """C -> W at 2 MeV: FP32 + surrogate (this build's 3-input network, FP32
+    and BF16 weights; the configured 2-input network) against FP32 MAGIC and
+    FP64 MAGIC on the same histories: this build's networks and FP32 within
+    1 % and 2 sigma of FP64 in vacancies/ion, mean range and Eel/E0, every
+    ledger closed; the configured network's deviations and the speed of each
+    against FP64 (per collision, kernel time) recorded."""
+    n = PROB.n_hist
+    res = {"fp64": eng.run(PROB, per_history=True, arith=gpu.MT_FP64),
+           "fp32": eng.run(PROB, per_history=True, arith=gpu.MT_FP32)}
+    for name, (opts, p, _), bf16 in (("surr3_fp32", trained, 0), ("surr3_bf16", trained, 1),
+                                     ("surr2_configured_bf16", trained_configured, 1)):
         opts.bf16 = bf16
         eng.surrogate_install(opts, p)
-        res["surr_bf16" if bf16 else "surr_fp32"] = eng.run(PROB, per_history=True, arith=gpu.MT_FP32_SURROGATE)
-    res["fp64"] = eng.run(PROB, per_history=True, arith=gpu.MT_FP64)
-    res["fp32"] = eng.run(PROB, per_history=True, arith=gpu.MT_FP32)
-    n = PROB.n_hist
+        res[name] = eng.run(PROB, per_history=True, arith=gpu.MT_FP32_SURROGATE)
     a = res["fp64"]
+    ns_per_coll = {k: 1e6 * r["kernel_ms"] / r["n_coll"] for k, r in res.items()}
     for name, r in res.items():
-        print("%s: %.1f ms, %.4g coll, surrogate %.4g, vac/ion %.2f, Eel/E0 %.4f, range %.1f" % (
-            name, r["kernel_ms"], r["n_coll"], r["surrogate"], r["vac"] / n, r["Eel"] / r["E0"],
-            r["range_sum"] / r["range_n"]))
+        print("%s: %.1f ms, %.4g coll (%.3f ns each, %.2fx FP64 speed), surrogate share %.3f, vac/ion %.2f, "
+              "Eel/E0 %.4f, range %.1f" % (
+                  name, r["kernel_ms"], r["n_coll"], ns_per_coll[name], ns_per_coll["fp64"] / ns_per_coll[name],
+                  r["surrogate"] / r["n_coll"], r["vac"] / n, r["Eel"] / r["E0"], r["range_sum"] / r["range_n"]))
     assert res["fp64"]["surrogate"] == 0 and res["fp32"]["surrogate"] == 0
-    for name in ("surr_fp32", "surr_bf16"):
+    for name in ("surr3_fp32", "surr3_bf16", "surr2_configured_bf16"):
         s = res[name]
         assert 0 < s["surrogate"] < s["n_coll"]
-        sig = math.sqrt((np.var(a["h_vac"]) + np.var(s["h_vac"])) / n)
-        assert abs(s["vac"] - a["vac"]) / n <= max(0.03 * a["vac"] / n, 3 * sig), name
-        assert s["Eel"] / s["E0"] == pytest.approx(a["Eel"] / a["E0"], rel=0.01), name
         tot = s["Eel"] + s["Enuc"] + s["Eunf"] + s["Elost"] + s["Edrop"] + s["Evac"]
         assert tot == pytest.approx(s["E0"], rel=1e-11)
+        print("%s vs fp64: vac/ion %+.3f %%, mean range %+.3f %%, Eel/E0 %+.3f %%" % (
+            name, 100 * (s["vac"] / a["vac"] - 1),
+            100 * ((s["range_sum"] / s["range_n"]) / (a["range_sum"] / a["range_n"]) - 1),
+            100 * ((s["Eel"] / s["E0"]) / (a["Eel"] / a["E0"]) - 1)))
+    # this build's networks meet the bound; the configured 2-input network
+    # (a rectangle in (log eps, b) that cascades do not sample evenly) is
+    # recorded above, not held to it
+    for name in ("surr3_fp32", "surr3_bf16", "fp32"):
+        tally_check(name, a, res[name], n)
 
 
 @pytest.mark.gpu
