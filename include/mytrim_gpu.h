@@ -291,9 +291,13 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts
 
 /* Autotuner (no counterpart in the reference: the reference's tuning knob is
  * its thread count, apps/runmytrim.C:59-68).  Runs the first `sample`
- * histories of `pkas` under each candidate launch shape (block size, LDS stack
- * depth, resident blocks per CU), `reps` times each, keeps the fastest
- * (collisions per kernel second) in the context and returns it in *best. */
+ * histories of `pkas` once at the default shape (for the primaries' share of
+ * the collisions), then under each of 16 candidate launch shapes (block size,
+ * LDS stack depth, resident blocks per CU) without wave roles and with feeder
+ * waves for 2.5 times that share, then at the fastest shape with roles with
+ * three other feeder counts, `reps` times each; keeps the fastest
+ * (collisions per kernel second; roles only when 3 % faster than the fastest
+ * shape without them) in the context and returns it in *best. */
 int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, long long sample,
                 int reps, mt_tuning* best, double* best_rate);
 

@@ -1082,96 +1082,110 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   mt_run_opts ro = *o;
   ro.per_history = 0;
   const mt_tuning keep = ctx->tune;
-  // candidate shapes (threads per block, resident blocks per CU, LDS stack
-  // entries per wave) around the region measured best on MI355X: one or two
-  // waves per SIMD with deep stacks, three for the single-precision kernels
-  // (fewer registers; a kernel whose registers do not allow the requested
-  // blocks per CU runs at the most that fit); queue capacity stays at its
-  // default
-  static const int kCand[][3] = {{256, 2, 256}, {256, 2, 192}, {256, 2, 160}, {256, 2, 128}, {256, 2, 96},
-                                 {256, 2, 64},  {128, 4, 128}, {128, 4, 96},  {64, 8, 128},  {256, 1, 384},
-                                 {256, 1, 256}, {128, 3, 256}, {256, 3, 128}, {256, 3, 96},  {128, 6, 96}};
-  std::vector<mt_tuning> cand;
-  for (const auto& c : kCand) {
-    mt_tuning t{};
-    t.block = c[0];
-    t.blocks_per_cu = c[1];
-    t.stack_cap = c[2];
-    t.queue_cap = keep.queue_cap;
-    t.lds_hist = keep.lds_hist;
-    t.graph = keep.graph;
-    t.primary_waves64 = 0;
-    cand.push_back(t);
-  }
-  double best_r = -1.0;
-  mt_tuning best_t = keep;
-  std::string first_err;
-  for (const mt_tuning& t : cand) {
+  // one timed pass of the sample under tuning t: collisions per kernel
+  // second (the best of `reps`), 0 when the shape does not fit (LDS,
+  // occupancy), < 0 on any other error (rc set)
+  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last) -> double {
     ctx->tune = t;
     double r = 0.0;
-    bool ok = true;
-    for (int i = 0; i < reps && ok; ++i) {
+    for (int i = 0; i < reps; ++i) {
       mt_tallies out{};
-      const int rc = mt_run_cascades(ctx, &s, &ro, &out);
-      if (rc == MT_E_LIMIT) {  // shape does not fit (LDS, occupancy): skip it
-        ok = false;
-      } else if (rc != MT_OK) {
+      rc = mt_run_cascades(ctx, &s, &ro, &out);
+      if (rc == MT_E_LIMIT) {
+        rc = MT_OK;
+        return 0.0;
+      }
+      if (rc != MT_OK) return -1.0;
+      if (out.kernel_ms > 0.0) r = std::max(r, double(out.n_coll) / (1e-3 * out.kernel_ms));
+      if (last) *last = out;
+    }
+    return r;
+  };
+  // feeder waves per 64 for a multiple f of the primaries' share of the
+  // collisions (0: no roles)
+  auto feeders = [](double share, double f) {
+    const double k = 64.0 * std::min(1.0, f * share);
+    return k < 0.5 ? 0 : std::min(63, std::max(1, int(std::lround(k))));
+  };
+  // the primaries' share of the collisions, from a run at the default shape
+  mt_tuning t0{};
+  t0.queue_cap = keep.queue_cap;
+  t0.lds_hist = keep.lds_hist;
+  t0.graph = keep.graph;
+  int rc = MT_OK;
+  mt_tallies probe{};
+  if (rate_of(t0, rc, &probe) < 0.0) {
+    ctx->tune = keep;
+    return rc;
+  }
+  const double share = probe.n_coll > 0 ? double(probe.coll_primary) / double(probe.n_coll) : 0.0;
+  // candidate shapes (threads per block, resident blocks per CU, LDS stack
+  // entries per wave) around the region measured best on MI355X: two waves
+  // per SIMD (the FP64 kernels' registers allow no more; a kernel whose
+  // registers do not allow the requested blocks per CU runs at the most that
+  // fit) with 64 to 256 stack entries, one with deep stacks; queue capacity
+  // stays at its default.  Each runs without wave roles and with feeder
+  // waves for 2.5 times the primaries' share, the region where roles
+  // measured within a few per cent of their best on every workload; the
+  // feeder count is then refined at the fastest shape with roles.  Roles are
+  // kept only when they beat the fastest shape without them by 3 %: a sample
+  // is smaller than the launches it tunes, and where roles gain little on
+  // the sample (cascades of thousands of flights) they have lost at full
+  // size.
+  static const int kCand[][3] = {{256, 2, 256}, {256, 2, 192}, {256, 2, 128}, {256, 2, 96}, {256, 2, 64},
+                                 {128, 4, 128}, {128, 6, 128}, {128, 6, 96},  {128, 6, 64}, {64, 8, 128},
+                                 {256, 1, 384}, {256, 1, 256}, {128, 3, 256}, {128, 3, 128}, {256, 3, 128},
+                                 {256, 3, 96}};
+  const int k_roles = feeders(share, 2.5);
+  double best_r[2] = {0.0, 0.0};  // without, with roles
+  mt_tuning best_t[2] = {keep, keep};
+  for (const auto& c : kCand) {
+    for (int with = 0; with < (k_roles > 0 ? 2 : 1); ++with) {
+      mt_tuning t = t0;
+      t.block = c[0];
+      t.blocks_per_cu = c[1];
+      t.stack_cap = c[2];
+      t.primary_waves64 = with ? k_roles : 0;
+      const double r = rate_of(t, rc, nullptr);
+      if (r < 0.0) {
         ctx->tune = keep;
         return rc;
-      } else if (out.kernel_ms > 0.0) {
-        r = std::max(r, double(out.n_coll) / (1e-3 * out.kernel_ms));
+      }
+      if (r > best_r[with]) {
+        best_r[with] = r;
+        best_t[with] = t;
       }
     }
-    if (ok && r > best_r) {
-      best_r = r;
-      best_t = t;
-    }
   }
-  if (best_r < 0.0) {
+  if (best_r[0] <= 0.0 && best_r[1] <= 0.0) {
     ctx->tune = keep;
     return fail(MT_E_LIMIT, "no candidate launch shape fits");
   }
-  // wave roles at the best shape: feeder shares around the primaries' share
-  // of the collisions (from a run at that shape), kept if faster
-  {
-    ctx->tune = best_t;
-    mt_tallies probe{};
-    const int rc = mt_run_cascades(ctx, &s, &ro, &probe);
-    if (rc != MT_OK) {
-      ctx->tune = keep;
-      return rc;
-    }
-    const double share = probe.n_coll > 0 ? double(probe.coll_primary) / double(probe.n_coll) : 0.0;
-    int tried[4] = {0, 0, 0, 0};
-    int n_tried = 0;
-    for (double f : {1.0, 1.5, 2.0}) {
-      const int k = std::min(63, std::max(1, int(std::lround(64.0 * std::min(1.0, f * share)))));
-      bool dup = false;
-      for (int i = 0; i < n_tried; ++i) dup = dup || tried[i] == k;
+  if (best_r[1] > 0.0) {
+    const int ks[3] = {feeders(share, 1.5), feeders(share, 4.0), 63};
+    for (int i = 0; i < 3; ++i) {
+      bool dup = ks[i] == k_roles;
+      for (int j = 0; j < i; ++j) dup = dup || ks[j] == ks[i];
       if (dup) continue;
-      tried[n_tried++] = k;
-      mt_tuning t = best_t;
-      t.primary_waves64 = k;
-      ctx->tune = t;
-      double r = 0.0;
-      for (int i = 0; i < reps; ++i) {
-        mt_tallies out{};
-        const int rc2 = mt_run_cascades(ctx, &s, &ro, &out);
-        if (rc2 != MT_OK) {
-          ctx->tune = keep;
-          return rc2;
-        }
-        if (out.kernel_ms > 0.0) r = std::max(r, double(out.n_coll) / (1e-3 * out.kernel_ms));
+      mt_tuning t = best_t[1];
+      t.primary_waves64 = ks[i];
+      const double r = rate_of(t, rc, nullptr);
+      if (r < 0.0) {
+        ctx->tune = keep;
+        return rc;
       }
-      if (r > best_r) {
-        best_r = r;
-        best_t = t;
+      if (r > best_r[1]) {
+        best_r[1] = r;
+        best_t[1] = t;
       }
     }
   }
-  ctx->tune = best_t;
-  *best = best_t;
-  if (best_rate) *best_rate = best_r;
+  const int pick = best_r[1] > 1.03 * best_r[0] ? 1 : 0;
+  const double best_rate_v = best_r[pick];
+  const mt_tuning best_v = best_t[pick];
+  ctx->tune = best_v;
+  *best = best_v;
+  if (best_rate) *best_rate = best_rate_v;
   return MT_OK;
 }
 

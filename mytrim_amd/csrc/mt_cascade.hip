@@ -85,6 +85,7 @@ struct Flight {
   int sp, gen, tag;
   uint32_t hist;
   uint64_t key;
+  CellCache cc;  // clusters samples: the occupancy cell of the last lookup
   int ic;
   double eel;  // electronic loss of this flight (per-history tally)
 };
@@ -125,14 +126,25 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// checksum of a queue entry and its lap (multiply / xor-shift mixing),
-// accumulated one field at a time
-__device__ __forceinline__ uint64_t sum_init(uint32_t lap) { return 0x9E3779B97F4A7C15ull * (uint64_t)(lap + 1u); }
-__device__ __forceinline__ uint64_t sum_add(uint64_t h, uint64_t w) {
-  h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
-  return h ^ (h >> 31);
+// checksum of a queue entry and its lap, accumulated one field at a time in
+// two 32-bit words mixed by additions, rotations and exclusive-ors: every
+// step is a full-rate vector instruction (a 64-bit multiply costs three
+// quarter-rate ones, and an entry is summed by its writer and its reader)
+struct QSum {
+  uint32_t a, b;
+};
+__device__ __forceinline__ QSum sum_init(uint32_t lap) { return QSum{0x9E3779B9u ^ lap, 0x7F4A7C15u + (lap << 16)}; }
+__device__ __forceinline__ QSum sum_add(QSum h, uint64_t w) {
+  h.a += (uint32_t)w;
+  h.b ^= (uint32_t)(w >> 32);
+  h.a = __builtin_rotateleft32(h.a, 7) ^ h.b;
+  h.b = __builtin_rotateleft32(h.b, 13) + h.a;
+  return h;
 }
-__device__ __forceinline__ uint32_t sum_end(uint64_t h) { return (uint32_t)(h >> 32); }
+__device__ __forceinline__ uint32_t sum_end(QSum h) {
+  const uint32_t a = h.a + __builtin_rotateleft32(h.b, 16);
+  return a ^ __builtin_rotateleft32(a, 11) ^ h.b;
+}
 
 // Kernel arguments are read from the argument segment where they are used
 // (scalar loads, which the scalar cache serves) rather than held in scalar
@@ -278,13 +290,14 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
   const int home = (P.q.nq >= 8 ? (int)(blockIdx.x & 7u) * per_xcd + wid % per_xcd
                                 : (int)(blockIdx.x * nwb + wid)) & (P.q.nq - 1);
   int victim = home;       // last shard this wave stole from
-  // wave roles (P.primary_waves64 > 0): feeder waves admit the primaries and
-  // send every recoil to the HBM queue while primaries remain; the others
-  // never admit primaries.  A feeder whose primaries are used up works like
-  // any other wave.
+  // wave roles (P.primary_waves64 > 0): feeder waves admit the primaries in
+  // bulk and send every recoil to the HBM queue; the others run what the
+  // queue holds and admit primaries only for their idle lanes when it is
+  // empty.  A feeder that may not admit (primaries used up, cascade budget
+  // reached) works like any other wave until it may again.
   const bool roles = P.primary_waves64 > 0;
   const bool feeder_wave = roles && ((int)((blockIdx.x * nwb + wid) & 63u) < P.primary_waves64);
-  bool feeding = feeder_wave;  // wave-uniform: primaries were left at the last refill
+  bool feeding = feeder_wave;  // wave-uniform: admitted primaries at its last refill
   int poll_skip = 0;       // iterations before the shared queue is polled again
   long long peak_seen = 0; // largest queue fill this wave has reported
   unsigned long long fe_pending = 0;  // ended flights not yet added to kFlightsEnded
@@ -334,8 +347,24 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
         const ArgPtr A = cold_args();
         // one lane loads one entry, so a batch holds at most 64
         const int want = min(64, min(cap - scount, max(n_idle, cap / 2) - scount));
-        // a feeder wave takes primaries only, until they are used up
-        if (feeding) feeding = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
+        // cascade budget (see Params): cascades that may be admitted now
+        auto budget_room = [&]() -> long long {
+          const uint64_t live = ld_relaxed(&ctrl[kLive]);
+          const uint64_t done = ld_relaxed(&ctrl[kCascDone]);
+          const double fhat =
+              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : A->f_init;
+          const long long cmax = max((long long)(A->budget / fmax(fhat, 1.0)), A->c_floor);
+          return cmax - (long long)live;
+        };
+        const bool pka_left = ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
+        long long n_adm = 0;
+        // a feeder wave admits primaries while they last and the budget lets
+        // it, leaving the shared queue to the others; otherwise it works like
+        // any other wave
+        if (feeder_wave) {
+          if (pka_left) n_adm = min((long long)want, budget_room());
+          feeding = n_adm > 0;
+        }
         // the home shard first, then up to three others (rotating victim)
         int qs = home;
         uint64_t h0 = 0, t0 = 0;
@@ -351,19 +380,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           }
         }
         const bool may_pop = t0 > h0;
-        // with roles, only feeder waves admit primaries
-        bool may_pka = !may_pop && (!roles || feeder_wave) &&
-                       ld_relaxed(&ctrl[kPkaNext]) < (unsigned long long)A->n_hist;
-        long long n_adm = 0;
-        if (may_pka) {  // cascade budget (see Params)
-          const uint64_t live = ld_relaxed(&ctrl[kLive]);
-          const uint64_t done = ld_relaxed(&ctrl[kCascDone]);
-          const double fhat =
-              done >= 64ull ? (double)ld_relaxed(&ctrl[kFlightsEnded]) / (double)done : A->f_init;
-          const long long cmax = max((long long)(A->budget / fmax(fhat, 1.0)), A->c_floor);
-          n_adm = min((long long)want, cmax - (long long)live);
-          may_pka = n_adm > 0;
+        // a wave that is not feeding admits primaries when the queue has
+        // nothing for it; with roles, only as many as it has idle lanes
+        if (!feeding && !may_pop && pka_left) {
+          const int w = roles ? min(want, n_idle - scount) : want;
+          n_adm = min((long long)w, budget_room());
         }
+        const bool may_pka = n_adm > 0;
         if (wave_idle && (may_pop || may_pka)) {
           if (lane == 0) atomicAdd(&ctrl[kState], (1ull << 32) - 1ull);  // epoch += 1, idle -= 1
           wave_idle = false;
@@ -403,7 +426,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
               while (true) {
                 const uint64_t rw = ld_agent(&A->q.ready[slot]);
                 if ((uint32_t)(rw >> 32) == lap) {
-                  uint64_t h = sum_init(lap), w;
+                  QSum h = sum_init(lap);
+                  uint64_t w;
                   w = ld_agent(&A->q.px[slot]); S.px[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
                   w = ld_agent(&A->q.py[slot]); S.py[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
                   w = ld_agent(&A->q.pz[slot]); S.pz[j] = __longlong_as_double((long long)w); h = sum_add(h, w);
@@ -495,6 +519,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           F.hist = S.hist[j];
           F.ic = 0;
           F.eel = 0.0;
+          F.cc.cell = kNoCell;
           // every index a flight uses is checked here, where all flights start
           if (F.sp < T.n_species && F.hist - (uint32_t)P.hist0 < (uint32_t)P.n_hist)
             active = true;
@@ -549,7 +574,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
       ++F.ic;
       int mtag;
       const double pos[3] = {F.px, F.py, F.pz};
-      const int k = lookup_material<GEO>(G, pos, mtag);
+      const int k = lookup_material<GEO>(G, pos, mtag, F.cc);
       if constexpr (PERHIST && GEO == 1) {
         // layer transport: a recoil's atom leaves the layer its flight starts
         // in (apps/mytrim_dynamic.C:160-164)
@@ -904,7 +929,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           if (lane == __ffsll((long long)full_m) - 1) atomicOr((unsigned int*)&ctrl[kError], (unsigned)kErrQueueFull);
         }
         if (over && !full) {
-          uint64_t h = sum_init(lap), w;
+          QSum h = sum_init(lap);
+          uint64_t w;
           w = (uint64_t)__double_as_longlong(rx); st_agent(&A->q.px[slot], w); h = sum_add(h, w);
           w = (uint64_t)__double_as_longlong(ry); st_agent(&A->q.py[slot], w); h = sum_add(h, w);
           w = (uint64_t)__double_as_longlong(rz); st_agent(&A->q.pz[slot], w); h = sum_add(h, w);

@@ -19,7 +19,19 @@
 
 namespace mt {
 
-__device__ __forceinline__ int lookup_cluster(const Geometry& g, const double pos[3]) {
+// The occupancy cell of a flight's last cluster lookup and what it holds:
+// o < 0 a clear cell, e < 0 a cell whose clusters overlap (the neighbourhood
+// scan decides), else its candidate list occ_idx[o, e).  Consecutive
+// collisions of a flight are a few A apart, mostly inside one cell, so the
+// bitmap word, rank and list offsets are loaded once per cell, not per
+// collision.  kNoCell marks an empty cache (a flight's start).
+constexpr uint32_t kNoCell = 0xffffffffu;
+struct CellCache {
+  uint32_t cell = kNoCell;
+  int o = -1, e = -1;
+};
+
+__device__ __forceinline__ int lookup_cluster(const Geometry& g, const double pos[3], CellCache& cc) {
   int k[3], k1[3], k2[3];
   int oc[3];
   bool out[3];
@@ -39,19 +51,29 @@ __device__ __forceinline__ int lookup_cluster(const Geometry& g, const double po
   // clear occupancy cell: no cluster contains the position (the scan below
   // would find none); this skips the (2 ks + 1)^3-cell scan for most positions
   {
-    const uint64_t bit = (uint64_t)oc[0] + (uint64_t)g.on[0] * ((uint64_t)oc[1] + (uint64_t)g.on[1] * (uint64_t)oc[2]);
-    const uint32_t word = g.occ[bit >> 5];
-    const uint32_t m = 1u << (bit & 31u);
-    if ((word & m) == 0u) return -1;
-    // occupied cell: its listed clusters are every cluster that can contain
-    // the position.  When they do not overlap at most one contains it, and
-    // the reference's scan returns that one (or none): test them with the
-    // scan's own distance.  A list of overlapping clusters is the single
-    // entry -1: scan as the reference does.
-    const uint32_t k = g.occ_rank[bit >> 5] + (uint32_t)__popc(word & (m - 1u));
-    const int o = g.occ_off[k], e = g.occ_off[k + 1];
-    if (g.occ_idx[o] >= 0) {
-      for (int q = o; q < e; ++q) {
+    // the cell count is at most 2^30, so a cell index fits 32 bits
+    const uint32_t bit = (uint32_t)oc[0] + (uint32_t)g.on[0] * ((uint32_t)oc[1] + (uint32_t)g.on[1] * (uint32_t)oc[2]);
+    if (bit != cc.cell) {
+      cc.cell = bit;
+      const uint32_t word = g.occ[bit >> 5];
+      const uint32_t m = 1u << (bit & 31u);
+      if ((word & m) == 0u) {
+        cc.o = -1;
+      } else {
+        // occupied cell: its listed clusters are every cluster that can
+        // contain a position in it.  When they do not overlap at most one
+        // contains the position, and the reference's scan returns that one
+        // (or none): test them with the scan's own distance.  A list of
+        // overlapping clusters is the single entry -1: scan as the reference
+        // does.
+        const uint32_t k = g.occ_rank[bit >> 5] + (uint32_t)__popc(word & (m - 1u));
+        cc.o = g.occ_off[k];
+        cc.e = g.occ_idx[cc.o] >= 0 ? g.occ_off[k + 1] : -1;
+      }
+    }
+    if (cc.o < 0) return -1;
+    if (cc.e >= 0) {
+      for (int q = cc.o; q < cc.e; ++q) {
         const int l = g.occ_idx[q];
         double d0 = pos[0] - g.cx[l];
         double d1 = pos[1] - g.cy[l];
@@ -129,7 +151,7 @@ __device__ __forceinline__ int lookup_wire(const Geometry& g, const double pos[3
 }
 
 template <int GEO>
-__device__ __forceinline__ int lookup_material(const Geometry& g, const double pos[3], int& tag) {
+__device__ __forceinline__ int lookup_material(const Geometry& g, const double pos[3], int& tag, CellCache& cc) {
   tag = -1;
   if (GEO == 0) return g.kind >= 3 ? lookup_wire(g, pos) : 0;
   if (GEO == 1) {
@@ -139,11 +161,18 @@ __device__ __forceinline__ int lookup_material(const Geometry& g, const double p
     for (int j = 0; j < g.n_layers; ++j) i += !(pos[0] < g.layer_end[j]);
     return i >= g.n_mat ? g.n_mat - 1 : i;
   }
-  const int l = lookup_cluster(g, pos);
+  const int l = lookup_cluster(g, pos, cc);
   if (l == -2) return -1;  // vacuum beyond a CUT face
   if (l == -1) return 0;
   tag = l;
   return 1;
+}
+
+// a single lookup (no flight to keep the cell for)
+template <int GEO>
+__device__ __forceinline__ int lookup_material(const Geometry& g, const double pos[3], int& tag) {
+  CellCache cc;
+  return lookup_material<GEO>(g, pos, tag, cc);
 }
 
 }  // namespace mt
