@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="fixed launch shape (with --autotune-sample 0)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--stack-cap", type=int, default=0)
+    ap.add_argument("--primary-waves64", type=int, default=0,
+                    help="feeder waves per 64 with a fixed launch shape (0: no wave roles)")
     ap.add_argument("--variant-histories", type=int, default=100000,
                     help="histories for the FP32 and FP32+surrogate lines (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=4_000_000)
@@ -197,10 +199,11 @@ def main():
     eng = gpu.Engine(0)
     eng.set_target(prob)
     tuning = None
-    if args.block or args.blocks_per_cu or args.stack_cap:
-        eng.tune(block=args.block, blocks_per_cu=args.blocks_per_cu, stack_cap=args.stack_cap)
+    if args.block or args.blocks_per_cu or args.stack_cap or args.primary_waves64:
+        eng.tune(block=args.block, blocks_per_cu=args.blocks_per_cu, stack_cap=args.stack_cap,
+                 primary_waves64=args.primary_waves64)
         tuning = {"block": args.block, "blocks_per_cu": args.blocks_per_cu, "stack_cap": args.stack_cap,
-                  "fixed": True}
+                  "primary_waves64": args.primary_waves64, "fixed": True}
     elif args.autotune_sample > 0:
         # the tuning sample holds at least --autotune-sample histories and about
         # 3e9 collisions, at most one timed step: small cascades need more
