@@ -90,7 +90,33 @@ def train_surrogate(eng, gpu, prob, n_train: int, epochs: int, **kw):
     return {"train_s": t_train, "final_loss": float(loss[-1]), "n_train": n_train, "epochs": epochs,
             "inputs": opts.inputs, "sampling": ["as the cascades draw", "rectangle"][opts.sampling],
             "domain": {k: v for k, v in norm.items() if k in ("eps_lo", "eps_hi", "b_hi")},
-            "accuracy": surrogate_accuracy(eng, gpu, norm)}
+            "accuracy": surrogate_accuracy(eng, gpu, norm),
+            "inference_ablation": inference_ablation(eng, gpu, norm)}
+
+
+def inference_ablation(eng, gpu, norm, n=1 << 21):
+    """Time per scattering evaluation of the FP32 MAGIC solve and of the installed
+    network with layer 2 in FP32 FMAs, BF16 dot products and BF16 matrix-core
+    products (mt_surrogate_ablate), on the same pairs of the surrogate's domain
+    (eps log-uniform up to min(eps_hi, 10), b uniform), each with its |d ln s2|
+    against the device's FP64 solve."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    hi = min(norm["eps_hi"], 10.0)
+    eps = np.exp(rng.uniform(np.log(norm["eps_lo"]), np.log(hi), n))
+    b = rng.uniform(0.0, norm["b_hi"], n)
+    ref, _, _, _ = eng.scatter(eps, b, 0)
+    out = {"pairs": n}
+    t_magic = None
+    for mode, name in gpu.Engine.ABLATE_MODES.items():
+        s2, ms = eng.surrogate_ablate(mode, eps, b, reps=5)
+        ok = (s2 > 0) & (ref > 0)
+        d = np.abs(np.log(s2[ok]) - np.log(ref[ok]))
+        t_magic = ms if mode == 0 else t_magic
+        out[name] = {"evaluations_per_s": n / (ms * 1e-3), "ns_per_evaluation": ms * 1e6 / n,
+                     "time_vs_magic_fp32": ms / t_magic, "median_abs_dln_s2": float(np.median(d)),
+                     "p99_abs_dln_s2": float(np.quantile(d, 0.99))}
+    return out
 
 
 def surrogate_accuracy(eng, gpu, norm, n=200000):

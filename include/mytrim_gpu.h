@@ -323,6 +323,17 @@ int mt_surrogate_train(mt_ctx* ctx, const mt_surrogate_opts* opts, float* params
 int mt_surrogate_install(mt_ctx* ctx, const mt_surrogate_opts* opts, const float* params);
 /* Installed surrogate at (eps, b): s2 = sin^2(theta/2), -1 outside its domain. */
 int mt_surrogate_eval(mt_ctx* ctx, const double* eps, const double* b, long long n, double* s2);
+/* Inference ablation of the learned scattering (no counterpart in the
+ * reference: it measures whether the network can replace trim.C:181-272).
+ * Evaluates n (eps, b) pairs `reps` times after one untimed pass, in one of
+ * four ways: mode 0 the FP32 MAGIC solve of the universal potential (no
+ * network needed), 1 the installed network with layer 2 in FP32 FMAs, 2 in
+ * BF16 dot products (v_dot2_f32_bf16), 3 on the matrix cores
+ * (v_mfma_f32_32x32x16_bf16, one 64 x 64 x 64 product per wave).  s2 receives
+ * sin^2(theta/2) (the network's without its domain check); *kernel_ms the mean
+ * time of one pass. */
+int mt_surrogate_ablate(mt_ctx* ctx, int mode, const double* eps, const double* b, long long n, int reps,
+                        double* s2, double* kernel_ms);
 /* Loss and gradient of one batch (targets ln s2), FP32 back-propagation. */
 int mt_surrogate_grad(mt_ctx* ctx, const mt_surrogate_opts* opts, const float* params, const double* eps,
                       const double* b, const double* ln_s2, int n, float* grad, double* loss);

@@ -98,4 +98,54 @@ hipError_t scatter_launch(int arith, int potential, const MlpDev& mlp, const dou
   return hipGetLastError();
 }
 
+// Inference ablation of the learned scattering: the FP32 MAGIC solve
+// (universal potential) it replaces (MODE 0), and the installed network with
+// layer 2 in FP32 FMAs (1), BF16 dot products (2) or on the matrix cores (3),
+// over the same (eps, b) pairs, without the domain check.  The loop runs
+// whole waves (base is a multiple of 64, lanes past n evaluate a dummy pair and
+// store nothing), as mlp_eval_mfma requires.
+template <int MODE>
+__global__ void __launch_bounds__(256) ablate_kernel(MlpDev S, const double* eps, const double* b, long long n,
+                                                     double* s2) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = blockIdx.x * (long long)blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    const long long i = base + (threadIdx.x & 63);
+    const bool in = i < n;
+    const float e = in ? (float)eps[i] : 1.0f, bb = in ? (float)b[i] : 1.0f;
+    float r;
+    if constexpr (MODE == 0) {
+      float ct, st;
+      int it;
+      scatter_f32<0>(e, bb, m_log(e), r, ct, st, it);
+    } else {
+      float x0, x1, x2;
+      mlp_inputs(S.n, e, bb, x0, x1, x2);
+      MlpDev T = S;
+      float y;
+      if constexpr (MODE == 3) {
+        y = mlp_eval_mfma(T, x0, x1, x2);
+      } else {
+        T.bf16 = MODE == 2;
+        y = mlp_eval(T, x0, x1, x2);
+      }
+      r = fminf(__expf(y * S.n.y_scale + S.n.y_off), 1.0f);
+    }
+    if (in) s2[i] = r;
+  }
+}
+
+hipError_t ablate_launch(int mode, const MlpDev& S, const double* eps, const double* b, long long n, double* s2,
+                         hipStream_t stream) {
+  const dim3 g(grid_for(n)), blk(256);
+  if (mode == 0)
+    hipLaunchKernelGGL(ablate_kernel<0>, g, blk, 0, stream, S, eps, b, n, s2);
+  else if (mode == 1)
+    hipLaunchKernelGGL(ablate_kernel<1>, g, blk, 0, stream, S, eps, b, n, s2);
+  else if (mode == 2)
+    hipLaunchKernelGGL(ablate_kernel<2>, g, blk, 0, stream, S, eps, b, n, s2);
+  else
+    hipLaunchKernelGGL(ablate_kernel<3>, g, blk, 0, stream, S, eps, b, n, s2);
+  return hipGetLastError();
+}
+
 }  // namespace mt

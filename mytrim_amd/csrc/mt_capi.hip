@@ -34,6 +34,8 @@ hipError_t stopping_launch(int arith, const unsigned char* tables, int species, 
                            double* out, hipStream_t stream);
 hipError_t scatter_launch(int arith, int potential, const MlpDev& mlp, const double* eps, const double* b, long long n,
                           double* s2, double* ct, double* st, int* iters, hipStream_t stream);
+hipError_t ablate_launch(int mode, const MlpDev& S, const double* eps, const double* b, long long n, double* s2,
+                         hipStream_t stream);
 struct FlightParams {
   const unsigned char* tables;
   Geometry geo;
@@ -1400,6 +1402,50 @@ int mt_surrogate_eval(mt_ctx* ctx, const double* eps, const double* b, long long
   S.bf16 = ctx->mlp_bf16;
   HIPCHECK(mlp_infer_launch(S, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n,
                             ctx->scratch_c.as<double>(), st));
+  HIPCHECK(hipMemcpyAsync(s2, ctx->scratch_c.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return MT_OK;
+}
+
+int mt_surrogate_ablate(mt_ctx* ctx, int mode, const double* eps, const double* b, long long n, int reps,
+                        double* s2, double* kernel_ms) {
+  if (!ctx || !kernel_ms || (n > 0 && (!eps || !b || !s2))) return fail(MT_E_ARG, "NULL argument");
+  if (mode < 0 || mode > 3) return fail(MT_E_ARG, "mode must be 0 (MAGIC), 1 (FP32), 2 (BF16 dot) or 3 (MFMA)");
+  if (reps < 1) return fail(MT_E_ARG, "reps must be >= 1");
+  if (mode > 0 && !ctx->mlp_ready) return fail(MT_E_STATE, "no surrogate installed");
+  *kernel_ms = 0.0;
+  if (n <= 0) return MT_OK;
+  HIPCHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  HIPCHECK(ctx->scratch_a.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_b.ensure(n * sizeof(double)));
+  HIPCHECK(ctx->scratch_c.ensure(n * sizeof(double)));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_a.p, eps, n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(ctx->scratch_b.p, b, n * sizeof(double), hipMemcpyHostToDevice, st));
+  MlpDev S{};
+  if (ctx->mlp_ready) {
+    S.p = ctx->mlp_p.as<float>();
+    S.w2bf = ctx->mlp_w2bf.as<uint32_t>();
+    S.n = ctx->mlp_norm;
+  }
+  // one untimed launch, then `reps` timed back to back
+  HIPCHECK(ablate_launch(mode, S, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n,
+                         ctx->scratch_c.as<double>(), st));
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  hipError_t e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    e = ablate_launch(mode, S, ctx->scratch_a.as<double>(), ctx->scratch_b.as<double>(), n,
+                      ctx->scratch_c.as<double>(), st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHECK(e);
+  *kernel_ms = double(ms) / reps;
   HIPCHECK(hipMemcpyAsync(s2, ctx->scratch_c.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   return MT_OK;

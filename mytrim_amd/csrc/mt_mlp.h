@@ -2,9 +2,10 @@
 // mapping (log10 eps, log10 b, b) to ln sin^2(theta/2), trained on the
 // engine's own universal-potential scattering (scatter_stable<double>) over
 // the (eps, b) pairs a cascade in the current target actually draws.
-// Inference runs per lane on the VALU: layer 2 either in FP32 FMAs or in BF16
-// dot products (v_dot2_f32_bf16, no MFMA), weights read with wave-uniform
-// (scalar) loads.
+// Inference in the cascades runs per lane on the VALU: layer 2 either in FP32
+// FMAs or in BF16 dot products (v_dot2_f32_bf16), weights read with
+// wave-uniform (scalar) loads.  A matrix-core form of layer 2 (mlp_eval_mfma)
+// needs the whole wave in step and serves the inference ablation.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -104,6 +105,74 @@ __device__ __forceinline__ float mlp_eval(const MlpDev& S, float x0, float x1, f
     }
   }
   return y;
+}
+
+typedef __bf16 mt_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float mt_f32x16 __attribute__((ext_vector_type(16)));
+
+// The same network with layer 2 on the matrix cores: the wave's 64 evaluations
+// form one 64 x 64 by 64 x 64 product, H2^T = W2 H1^T, as 2 x 2 tiles of
+// v_mfma_f32_32x32x16_bf16 over 4 K-steps (16 instructions).  Tile layouts of
+// that instruction (lane l, element j): A[i][k] holds A[l % 32][8 (l / 32) + j],
+// B[k][n] holds B[8 (l / 32) + j][l % 32], D[i][n] holds element r at
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  Column n of an M-tile t is
+// evaluation (lane) n + 32 t, so lane l computes the layer-1 activations of its
+// own evaluation and of lane l ^ 32's for its half of K, and after layer 2 holds
+// 16 of the 64 rows of those two columns; lane l ^ 32 holds the other 16 of
+// each tile, and one exchange completes layer 3.  W2 is read as A in BF16
+// (the w2bf pairs), the layer-1 activations are rounded to BF16, accumulation
+// is FP32.  All 64 lanes of the wave must call it together (uniform control
+// flow): every evaluation's product uses operands from two lanes.
+__device__ __forceinline__ float mlp_eval_mfma(const MlpDev& S, float x0, float x1, float x2) {
+  const int l = threadIdx.x & 63;
+  const int half = l >> 5, col = l & 31;
+  const float q0 = __shfl_xor(x0, 32), q1 = __shfl_xor(x1, 32), q2 = __shfl_xor(x2, 32);
+  float y_own = 0.0f;
+  // one M-tile at a time, and within it one K-step at a time: the step's 8
+  // layer-1 activations feed both output tiles at once, so the live set is
+  // the two accumulators (32 registers) and one step's operands
+  const float* p = S.p;
+  const uint32_t* w2 = S.w2bf;
+#pragma unroll 1
+  for (int t = 0; t < 2; ++t) {
+    // the weights are reloaded (from cache) for each M-tile: opaque copies of
+    // the pointers keep the compiler from holding all 224 per-lane weights of
+    // both tiles in registers (300 VGPRs, one wave per SIMD)
+    asm volatile("" : "+s"(p), "+s"(w2));
+    // evaluation n + 32 t of column n = col: this lane's own when t == half
+    const float a0 = t == half ? x0 : q0, a1 = t == half ? x1 : q1, a2 = t == half ? x2 : q2;
+    mt_f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r >> 2) + 4 * half + (r & 3);
+      acc0[r] = p[kOffB2 + row];
+      acc1[r] = p[kOffB2 + 32 + row];
+    }
+    const uint4* w0 = (const uint4*)(w2 + col * 32);
+    const uint4* w1 = (const uint4*)(w2 + (32 + col) * 32);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint32_t w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 16 * c + 8 * half + 2 * j;
+        w[j] = bf16_bits(mlp_h1(p, k, a0, a1, a2)) | (bf16_bits(mlp_h1(p, k + 1, a0, a1, a2)) << 16);
+      }
+      const mt_bf16x8 hb = __builtin_bit_cast(mt_bf16x8, w);
+      // W2[32 nt + col][16 c + 8 half .. + 7]
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mt_bf16x8, w0[2 * c + half]), hb, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mt_bf16x8, w1[2 * c + half]), hb, acc1, 0, 0, 0);
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r >> 2) + 4 * half + (r & 3);
+      part += p[kOffW3 + row] * mlp_tanh(acc0[r]) + p[kOffW3 + 32 + row] * mlp_tanh(acc1[r]);
+    }
+    const float y = part + __shfl_xor(part, 32);
+    if (t == half) y_own = y;
+  }
+  return S.p[kOffB3] + y_own;
 }
 
 __host__ __device__ __forceinline__ void mlp_inputs(const MlpNorm& n, float eps, float b, float& x0, float& x1,

@@ -38,7 +38,7 @@ EXPORTS = ["mt_last_error", "mt_version", "mt_ctx_create", "mt_ctx_destroy", "mt
            "mt_run_cascades_noalloc",
            "mt_stopping_batch", "mt_scatter_batch", "mt_stopping_batch_arith", "mt_scatter_batch_arith",
            "mt_surrogate_norm", "mt_surrogate_train", "mt_surrogate_install",
-           "mt_surrogate_eval", "mt_surrogate_grad"]
+           "mt_surrogate_eval", "mt_surrogate_grad", "mt_surrogate_ablate"]
 
 
 class Target(C.Structure):
@@ -161,6 +161,7 @@ def lib():
         L.mt_surrogate_install.argtypes = [C.c_void_p, so, fp]
         L.mt_surrogate_eval.argtypes = [C.c_void_p, dp, dp, C.c_longlong, dp]
         L.mt_surrogate_grad.argtypes = [C.c_void_p, so, fp, dp, dp, dp, C.c_int, fp, dp]
+        L.mt_surrogate_ablate.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_longlong, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -421,6 +422,20 @@ class Engine:
         _check(lib().mt_surrogate_eval(self.h, _p(eps, dp), _p(b, dp), len(eps), _p(out, dp)),
                "mt_surrogate_eval")
         return out
+
+    ABLATE_MODES = {0: "magic_fp32", 1: "mlp_fp32", 2: "mlp_bf16_dot", 3: "mlp_bf16_mfma"}
+
+    def surrogate_ablate(self, mode: int, eps, b, reps: int = 5):
+        """s2 at (eps, b) and the mean kernel time of one pass: mode 0 the FP32
+        MAGIC solve, 1-3 the installed network with layer 2 in FP32 FMAs, BF16
+        dot products or BF16 matrix-core products (no domain check)."""
+        eps = np.ascontiguousarray(eps, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        out = np.zeros(len(eps))
+        ms = C.c_double(0.0)
+        _check(lib().mt_surrogate_ablate(self.h, int(mode), _p(eps, dp), _p(b, dp), len(eps), int(reps),
+                                         _p(out, dp), C.byref(ms)), "mt_surrogate_ablate")
+        return out, ms.value
 
     def surrogate_grad(self, opts: SurrogateOpts, params, eps, b, ln_s2):
         params = np.ascontiguousarray(params, np.float32)

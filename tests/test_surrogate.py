@@ -168,6 +168,41 @@ def test_device_inference_matches_numpy(eng, trained, bf16):
     assert np.all(out == -1.0)
 
 
+@pytest.mark.gpu
+def test_inference_ablation_matrix_cores(eng, trained):
+    """Layer 2 on the matrix cores (one 64 x 64 x 64 BF16 product per wave,
+    mlp_eval_mfma) evaluates the same network as numpy, to the accuracy of the
+    BF16 dot-product form; records the time per evaluation of the three
+    network forms against the FP32 MAGIC solve they would replace."""
+    opts, p, _ = trained
+    opts.bf16 = 1
+    eng.surrogate_install(opts, p)
+    norm = eng.surrogate_norm(opts)
+    rng = np.random.default_rng(17)
+    n = (1 << 20) + 37  # a last wave with 37 lanes in use
+    eps = 10 ** rng.uniform(math.log10(norm["eps_lo"]), math.log10(norm["eps_hi"]), n)
+    b = rng.uniform(0, norm["b_hi"], n)
+    ref, _, _, _ = eng.scatter(eps, b, 0)  # FP64 MAGIC
+    want = M.s2_of(p.astype(np.float64), norm, eps[-3000:], b[-3000:])
+    res = {}
+    for mode, name in gpu.Engine.ABLATE_MODES.items():
+        s2, ms = eng.surrogate_ablate(mode, eps, b, reps=5)
+        ok = (s2 > 0) & (ref > 0)
+        d = np.abs(np.log(s2[ok]) - np.log(ref[ok]))
+        res[name] = (s2, ms)
+        print("%-14s %.3f ns per evaluation; vs FP64 MAGIC |d ln s2| median %.2e p99 %.2e"
+              % (name, ms * 1e6 / n, np.median(d), np.quantile(d, 0.99)))
+    t0 = res["magic_fp32"][1]
+    print("time relative to the FP32 MAGIC solve: " + ", ".join(
+        "%s %.2fx" % (k, v[1] / t0) for k, v in res.items() if k != "magic_fp32"))
+    for name, bound in (("mlp_fp32", (2e-3, 2e-3)), ("mlp_bf16_dot", (0.02, 0.5)), ("mlp_bf16_mfma", (0.02, 0.5))):
+        d = np.abs(np.log(res[name][0][-3000:]) - np.log(want))
+        assert np.median(d) < bound[0] and d.max() < bound[1], (name, np.median(d), d.max())
+    # the two BF16 forms round the same operands; they differ in summation order only
+    d = np.abs(np.log(res["mlp_bf16_mfma"][0]) - np.log(res["mlp_bf16_dot"][0]))
+    assert np.median(d) < 2e-3, np.median(d)
+
+
 @pytest.fixture(scope="module")
 def trained_configured(eng):
     """The configured network: 2 -> 64 -> 64 -> 1 (inputs log eps and b), 1e7
