@@ -1,7 +1,17 @@
 // Point and Real of the MyTRIM API (shim/point.h, simconf.h:39): a 3-vector
-// with component access and component-wise arithmetic.
+// with component access and component-wise arithmetic.  Built inside
+// MOOSE/Magpie (MYTRIM_ENABLED) they, and Utility::pow, come from MOOSE and
+// libMesh instead, as in the reference (simconf.h:31-36); the API uses only
+// the members both provide.
 #ifndef MYTRIM_AMD_POINT_H
 #define MYTRIM_AMD_POINT_H
+
+#ifdef MYTRIM_ENABLED
+#include "MooseError.h"
+#include "MooseTypes.h"
+#include "libmesh/point.h"
+#include "libmesh/utility.h"
+#else
 
 #include <cmath>
 
@@ -60,5 +70,7 @@ inline Real pow(const Real& x) {
   }
 }
 }  // namespace Utility
+
+#endif  // MYTRIM_ENABLED
 
 #endif

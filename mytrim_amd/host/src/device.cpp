@@ -1,5 +1,6 @@
 // Internal: device contexts of the C++ API (see device.h).
 #include "device.h"
+#include "api_error.h"
 
 #include <cstdlib>
 #include <map>
@@ -20,7 +21,7 @@ int ordinal() {
 }
 
 void check(int rc, const char* what) {
-  if (rc != MT_OK) throw std::runtime_error(std::string(what) + ": " + mt_last_error());
+  if (rc != MT_OK) api_error(std::string(what) + ": " + mt_last_error());
 }
 
 mt_ctx* create_context(SimconfType* simconf) {
@@ -109,7 +110,7 @@ std::vector<double> target_signature(SampleBase* s, SimconfType* simconf, int po
 void upload_target(mt_ctx* ctx, SampleBase* s, SimconfType* simconf, int potential) {
   const int kind = s->geometry();
   if (kind < 0)
-    throw std::runtime_error(
+    api_error(
         "sample kind has no device geometry (use SampleSolid, SampleLayers, sampleClusters, SampleWire or "
         "SampleBurriedWire)");
   std::vector<double> rho;

@@ -1,5 +1,6 @@
 // MaterialBase (material.C): prepare/average on the host, getrstop on the GPU.
 #include "material.h"
+#include "api_error.h"
 
 #include <cmath>
 #include <memory>
@@ -21,7 +22,7 @@ void MaterialBase::prepare() {  // material.C:37-74
     if (_element[i]._t < 0.0) _element[i]._t = 0.0;
     tt += _element[i]._t;
   }
-  if (tt == 0.0) throw std::runtime_error("Stoichiometry invalid, all elements zero.");
+  if (tt == 0.0) api_error("Stoichiometry invalid, all elements zero.");
   for (unsigned i = 0; i < end; ++i) _element[i]._t /= tt;
   _am = 0.0;
   _az = 0.0;
@@ -29,7 +30,7 @@ void MaterialBase::prepare() {  // material.C:37-74
     _am += _element[i]._m * _element[i]._t;
     _az += Real(_element[i]._Z) * _element[i]._t;
   }
-  if (_am == 0.0) throw std::runtime_error("Stoichiometry invalid, atomic density is zero.");
+  if (_am == 0.0) api_error("Stoichiometry invalid, atomic density is zero.");
   _arho = _rho * 0.6022 / _am;
 }
 
@@ -62,7 +63,7 @@ std::vector<Real> MaterialBase::getrstop(const IonBase* pka, const std::vector<R
   std::lock_guard<std::mutex> lock(sc.mu);
   sc.upload(*this);
   if (mt_stopping_batch(sc.ctx, pka->_Z, pka->_m, 0, energies.data(), (long long)energies.size(), out.data()) != MT_OK)
-    throw std::runtime_error(std::string("getrstop: ") + mt_last_error());
+    api_error(std::string("getrstop: ") + mt_last_error());
   return out;
 }
 

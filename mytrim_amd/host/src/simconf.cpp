@@ -1,5 +1,6 @@
 // SimconfType (simconf.C): defaults, seeding, length scale, tables.
 #include "simconf.h"
+#include "api_error.h"
 
 #include <cstdlib>
 #include <fstream>
@@ -31,7 +32,7 @@ SimconfType::SimconfType(unsigned int s)
     readDataFiles(dir);
   } else {
     std::vector<double> rows(92 * 16);
-    if (mt_default_tables(rows.data(), 92, 16) != MT_OK) throw std::runtime_error(mt_last_error());
+    if (mt_default_tables(rows.data(), 92, 16) != MT_OK) api_error(mt_last_error());
     for (unsigned i = 0; i < _rows; ++i) {
       const double* v = &rows[16 * i];
       ScoefLine& c = scoef[i];
@@ -84,10 +85,10 @@ std::vector<double> SimconfType::tableRows() const {
 void SimconfType::readDataFiles(const std::string& dir) {
   auto open = [&](const char* name, int skip) {
     std::ifstream* in = new std::ifstream((dir + "/" + name).c_str());
-    if (!*in) throw std::runtime_error("Error reading " + dir + "/" + name);
+    if (!*in) api_error("Error reading " + dir + "/" + name);
     std::string line;
     for (int i = 0; i < skip; ++i)
-      if (!std::getline(*in, line)) throw std::runtime_error("Error reading file");
+      if (!std::getline(*in, line)) api_error("Error reading file");
     return std::unique_ptr<std::ifstream>(in);
   };
   auto a = open("SCOEF.95A", 2);
@@ -97,8 +98,8 @@ void SimconfType::readDataFiles(const std::string& dir) {
     ScoefLine& c = scoef[i];
     *a >> dummy >> c.mm1 >> c.m1 >> c.mnat >> c.rho >> c.atrho >> c.vfermi >> c.heat;
     for (int j = 0; j < 8; ++j) *a >> c.pcoef[j];
-    if (!*a) throw std::runtime_error("contents of SCOEF.95A");
-    if (!(*l >> dummy >> c.lfctr)) throw std::runtime_error("contents of SLFCTR.dat");
+    if (!*a) api_error("contents of SCOEF.95A");
+    if (!(*l >> dummy >> c.lfctr)) api_error("contents of SLFCTR.dat");
   }
 }
 

@@ -1,5 +1,6 @@
 // TrimGPU::runCascades: the batched device path (mt_run_cascades).
 #include "trim_gpu.h"
+#include "api_error.h"
 
 #include <climits>
 #include <ostream>
@@ -149,7 +150,7 @@ TrimGPU::Tallies TrimGPU::runCascades(const std::vector<IonBase*>& pkas, uint32_
   std::vector<double> m, E, pos, dir;
   double Ef = pkas.empty() ? 3.0 : pkas[0]->_Ef;
   for (IonBase* p : pkas) {
-    if (p->_Ef != Ef) throw std::runtime_error("runCascades: all primaries must share _Ef");
+    if (p->_Ef != Ef) api_error("runCascades: all primaries must share _Ef");
     Z.push_back(p->_Z);
     m.push_back(p->_m);
     E.push_back(p->_E);
@@ -168,12 +169,12 @@ TrimGPU::Tallies TrimGPU::runCascades(const std::vector<IonBase*>& pkas, uint32_
 
 void TrimGPU::applyTransport(SampleDynamic& sample, const Tallies& t) {
   const size_t nl = sample.material.size();
-  if (t.transport.size() != nl * 93) throw std::runtime_error("applyTransport: no transport tally for this sample");
+  if (t.transport.size() != nl * 93) api_error("applyTransport: no transport tally for this sample");
   for (size_t k = 0; k < nl; ++k)
     for (int Z = 1; Z <= 92; ++Z) {
       const long long n = t.transport[k * 93 + size_t(Z)];
       if (n == 0) continue;
-      if (n > INT_MAX || n < INT_MIN) throw std::runtime_error("applyTransport: count out of int range");
+      if (n > INT_MAX || n < INT_MIN) api_error("applyTransport: count out of int range");
       sample.addAtomsToLayer(int(k), int(n), Z);
     }
 }
