@@ -296,7 +296,8 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts
  * LDS stack depth, resident blocks per CU) without wave roles and with feeder
  * waves for 2.5 times that share, then at the fastest shape with roles with
  * three other feeder counts, `reps` times each; keeps the fastest
- * (collisions per kernel second; roles only when 3 % faster than the fastest
+ * (collisions per kernel second; roles only when the primaries make at least
+ * 2 % of the collisions and, timed once more, run 3 % faster than the fastest
  * shape without them) in the context and returns it in *best. */
 int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, long long sample,
                 int reps, mt_tuning* best, double* best_rate);

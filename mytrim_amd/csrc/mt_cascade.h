@@ -131,7 +131,7 @@ struct Params {
   int range_nb;
   double lat_y0, lat_z0, lat_bw;
   int lat_nb;
-  double vac_x0, vac_bw;
+  double vac_x0, vac_bw, vac_ibw;  // vac_ibw = 1 / vac_bw
   int vac_nb;
   int lds_hist;                   // bin vac/repl depths in LDS, flush at exit
   // per history (index hist - hist0)

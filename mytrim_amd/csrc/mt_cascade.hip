@@ -79,6 +79,19 @@ __device__ __forceinline__ int bin_of(double x, double x0, double bw, int nb) {
   return 1 + (int)t;
 }
 
+// bin_of with the quotient from the stored reciprocal ibw = 1 / bw: d ibw is
+// within two ulps of the correctly rounded d / bw, so both have the same floor
+// and the same comparisons with 0 and nb unless d ibw lies within a few ulps
+// of an integer; there (a lane in some 1e12) the division decides
+__device__ __forceinline__ int bin_of_r(double x, double x0, double bw, double ibw, int nb) {
+  const double d = x - x0;
+  double t = d * ibw;
+  if (fabs(t - rint(t)) <= 1e-12 * fmax(fabs(t), 1.0)) t = d / bw;
+  if (!(t >= 0.0)) return 0;
+  if (t >= (double)nb) return nb + 1;
+  return 1 + (int)t;
+}
+
 // one flight's state, held in registers
 struct Flight {
   double px, py, pz, dx, dy, dz, E;
@@ -707,13 +720,18 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
           } else {
             atomicAdd(&cold_args()->energy[eEunf], den);
           }
-          if (F.E > ec.Edisp) {
-            // vacancyCreation (trim.C:439-443, TrimVacCount.C:31-42, KP trim.C:445-464)
-            const int bn = bin_of(rx, C->vac_x0, C->vac_bw, C->vac_nb);
+          // a vacancy or a replacement at the recoil's site: one depth bin,
+          // one count in the vacancy or the replacement histogram
+          const bool vac = F.E > ec.Edisp;
+          {
+            const int bn = bin_of_r(rx, C->vac_x0, C->vac_bw, C->vac_ibw, C->vac_nb);
             if (C->lds_hist)
-              atomicAdd(&lvac[bn], 1u);
+              atomicAdd(&(vac ? lvac : lrepl)[bn], 1u);
             else
-              atomicAdd(&cold_args()->vac_hist[bn], 1ull);
+              atomicAdd(&(vac ? cold_args()->vac_hist : cold_args()->repl_hist)[bn], 1ull);
+          }
+          if (vac) {
+            // vacancyCreation (trim.C:439-443, TrimVacCount.C:31-42, KP trim.C:445-464)
             int extra = 0;
             if (C->kp && F.gen + 1 == C->follow_max_gen) {
               const double ed = 0.0115 * m_pow(mc.az, -7.0 / 3.0) * rE;
@@ -726,11 +744,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) 
             if (PERHIST) atomicAdd(&cold_args()->h_vac[F.hist - (uint32_t)P.hist0], 1 + extra);
             state = -1;  // vacancy marker, ion keeps moving
           } else {
-            const int bn = bin_of(rx, C->vac_x0, C->vac_bw, C->vac_nb);
-            if (C->lds_hist)
-              atomicAdd(&lrepl[bn], 1u);
-            else
-              atomicAdd(&cold_args()->repl_hist[bn], 1ull);
+            // replacementCollision (trim.C:388-395)
             state = (H->species[F.sp].Z == ec.Z) ? 1 : 2;
             kadd(enuc_s, enuc_e, F.E);
           }

@@ -195,6 +195,7 @@ class Engine:
         _check(lib().mt_ctx_create(device, C.byref(h)), "mt_ctx_create")
         self.h = h
         self.problem: Optional[Problem] = None
+        self._target_key = None
 
     def close(self):
         if self.h:
@@ -242,9 +243,24 @@ class Engine:
         _check(lib().mt_ctx_set_target(self.h, C.byref(tg)), "mt_ctx_set_target")
         del keep
         self.problem = prob
+        self._target_key = self.target_key(prob)
+
+    @staticmethod
+    def target_key(prob: Problem):
+        """What mt_ctx_set_target receives from a problem: materials, geometry
+        and the collision knobs.  Batches of one problem (Problem.batch) share
+        it; their primaries and history range go with each run instead.  The
+        bubble array is keyed by identity (the engine keeps the last problem,
+        and with it the array, alive)."""
+        g = prob.geometry
+        mats = tuple((m.rho, tuple((e.Z, e.m, e.t, e.Edisp, e.Elbind) for e in m.elements)) for m in prob.materials)
+        cl = None if g.clusters is None else (id(g.clusters), g.clusters.shape)
+        return (mats, g.kind, tuple(float(x) for x in g.w), tuple(int(x) for x in g.bc),
+                tuple(float(x) for x in g.layers), tuple(int(x) for x in g.kn), cl,
+                prob.tmin, prob.tau, prob.cw, prob.scale, prob.potential)
 
     def _launch_args(self, prob: Problem, per_history: bool, arith: int):
-        if self.problem is not prob:
+        if self.problem is not prob and self._target_key != self.target_key(prob):
             self.set_target(prob)
         p = prob.primaries
         keep = [np.ascontiguousarray(p.Z, np.int32), np.ascontiguousarray(p.m, np.float64),
