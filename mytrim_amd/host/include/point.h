@@ -14,6 +14,7 @@
 #else
 
 #include <cmath>
+#include <type_traits>
 
 typedef double Real;
 
@@ -52,22 +53,36 @@ class Point {
 };
 
 namespace Utility {
-// integer powers by repeated squaring (shim/pow.h)
+namespace detail {
+// x^N in the order of shim/pow.h: an odd exponent takes one factor of x off,
+// an even one squares x^(N/2), and x^6 is x^4 x^2 (C++11: overloads on the
+// exponent instead of if constexpr, so that the API's C++11 callers build)
+inline Real ipow(const Real&, std::integral_constant<int, 0>) { return 1.0; }
+inline Real ipow(const Real& x, std::integral_constant<int, 1>) { return x; }
+inline Real ipow(const Real& x, std::integral_constant<int, 6>) {
+  const Real x2 = x * x, x4 = x2 * x2;
+  return x4 * x2;
+}
+template <int N>
+inline Real ipow(const Real& x, std::integral_constant<int, N>);
+template <int N>
+inline Real ipow_step(const Real& x, std::true_type) {  // N odd
+  return x * ipow(x, std::integral_constant<int, N - 1>());
+}
+template <int N>
+inline Real ipow_step(const Real& x, std::false_type) {  // N even
+  const Real h = ipow(x, std::integral_constant<int, N / 2>());
+  return h * h;
+}
+template <int N>
+inline Real ipow(const Real& x, std::integral_constant<int, N>) {
+  return ipow_step<N>(x, std::integral_constant<bool, (N % 2 == 1)>());
+}
+}  // namespace detail
+
 template <int N>
 inline Real pow(const Real& x) {
-  if constexpr (N == 0) {
-    return 1.0;
-  } else if constexpr (N == 1) {
-    return x;
-  } else if constexpr (N == 6) {
-    const Real x2 = x * x, x4 = x2 * x2;
-    return x4 * x2;
-  } else if constexpr (N % 2) {
-    return x * pow<N - 1>(x);
-  } else {
-    const Real h = pow<N / 2>(x);
-    return h * h;
-  }
+  return detail::ipow(x, std::integral_constant<int, N>());
 }
 }  // namespace Utility
 
