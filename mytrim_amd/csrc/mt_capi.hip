@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "mt_cascade.h"
+#include "mt_flight.h"
 #include "mt_host.h"
 #include "mytrim_gpu.h"
 
@@ -36,18 +37,6 @@ hipError_t scatter_launch(int arith, int potential, const MlpDev& mlp, const dou
                           double* s2, double* ct, double* st, int* iters, hipStream_t stream);
 hipError_t ablate_launch(int mode, const MlpDev& S, const double* eps, const double* b, long long n, double* s2,
                          hipStream_t stream);
-struct FlightParams {
-  const unsigned char* tables;
-  Geometry geo;
-  double tau, cw, invscale;
-  mt_ion* ion;
-  const double* u;
-  long long n_u;
-  mt_event* ev;
-  int max_ev;
-  long long* n_used;
-};
-hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t stream);
 hipError_t mlp_make_data(uint32_t seed, long long n, const MlpTriple* tri, int n_tri, double E_lo, int rect,
                          const MlpNorm& nm, float* X, float* T, hipStream_t st);
 hipError_t mlp_train(float* p, const float* X, const float* T, long long n, int B, int epochs, float lr0,
@@ -116,6 +105,32 @@ struct HBuf {
   void release() {
     if (p) (void)hipHostFree(p);
     p = nullptr;
+    bytes = 0;
+  }
+};
+
+// pinned host memory that kernels write directly: mapped and coherent, at the
+// device address hipHostGetDevicePointer gives
+struct MBuf {
+  void* p = nullptr;
+  void* dev = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    const size_t want = std::max<size_t>(n, 256);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&dev, p, 0);
+    if (e == hipSuccess)
+      bytes = want;
+    else
+      release();
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    dev = nullptr;
     bytes = 0;
   }
 };
@@ -215,7 +230,9 @@ struct mt_ctx {
   uint64_t qcap = 0;
   // single-flight path: device tables per ion species (Z, m), event buffers
   std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
-  DBuf f_ion, f_u, f_ev, f_used;
+  DBuf f_in;     // single flight: the ion and the caller's uniforms
+  HBuf f_stage;  // their pinned staging
+  MBuf f_out;    // what the flight kernel writes: status words, the ion, events
   // scattering surrogate
   DBuf mlp_p, mlp_w2bf, mlp_X, mlp_T, mlp_rec, mlp_part, mlp_g, mlp_m, mlp_v, mlp_t, mlp_bl, mlp_tri;
   MlpNorm mlp_norm{};
@@ -239,10 +256,9 @@ struct mt_ctx {
     for (auto& b : per) b.release();
     for (auto& b : q) b.release();
     for (auto& kv : flight_tables) kv.second->release();
-    f_ion.release();
-    f_u.release();
-    f_ev.release();
-    f_used.release();
+    f_in.release();
+    f_stage.release();
+    f_out.release();
     DBuf* ml[] = {&mlp_p,     &mlp_w2bf, &mlp_X, &mlp_T, &mlp_rec, &mlp_part,
                   &mlp_g,     &mlp_m,    &mlp_v, &mlp_t, &mlp_bl,  &mlp_tri};
     for (DBuf* b : ml) b->release();
@@ -1246,12 +1262,24 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
     HIPCHECK(hipMemcpy(b->p, blob.data(), blob.size(), hipMemcpyHostToDevice));
     it = ctx->flight_tables.emplace(key, std::move(b)).first;
   }
-  HIPCHECK(ctx->f_ion.ensure(sizeof(mt_ion)));
-  HIPCHECK(ctx->f_u.ensure(size_t(n_u) * sizeof(double)));
-  HIPCHECK(ctx->f_ev.ensure(size_t(max_events) * sizeof(mt_event)));
-  HIPCHECK(ctx->f_used.ensure(3 * sizeof(long long)));
-  HIPCHECK(hipMemcpyAsync(ctx->f_ion.p, ion, sizeof(mt_ion), hipMemcpyHostToDevice, st));
-  HIPCHECK(hipMemcpyAsync(ctx->f_u.p, u, size_t(n_u) * sizeof(double), hipMemcpyHostToDevice, st));
+  // one copy in (the ion and the uniforms, staged in pinned memory) and one
+  // synchronisation: the kernel writes the status words, the ion and the
+  // events straight into mapped pinned memory
+  const size_t u_off = (sizeof(mt_ion) + 15) & ~size_t(15);
+  const size_t in_bytes = u_off + size_t(n_u) * sizeof(double);
+  const size_t ion_off = 32, ev_off = (ion_off + sizeof(mt_ion) + 15) & ~size_t(15);
+  const size_t out_bytes = ev_off + size_t(max_events) * sizeof(mt_event);
+  HIPCHECK(ctx->f_in.ensure(in_bytes));
+  HIPCHECK(ctx->f_stage.ensure(in_bytes));
+  HIPCHECK(ctx->f_out.ensure(out_bytes));
+  unsigned char* stage = static_cast<unsigned char*>(ctx->f_stage.p);
+  std::memcpy(stage, ion, sizeof(mt_ion));
+  std::memcpy(stage + u_off, u, size_t(n_u) * sizeof(double));
+  HIPCHECK(hipMemcpyAsync(ctx->f_in.p, stage, in_bytes, hipMemcpyHostToDevice, st));
+  unsigned char* out_h = static_cast<unsigned char*>(ctx->f_out.p);
+  unsigned char* out_d = static_cast<unsigned char*>(ctx->f_out.dev);
+  long long* used = reinterpret_cast<long long*>(out_h);
+  used[2] = -1;  // overwritten by the kernel
   FlightParams P;
   std::memset(&P, 0, sizeof(P));
   P.tables = it->second->as<unsigned char>();
@@ -1259,15 +1287,14 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
   P.tau = ctx->tgt.tau;
   P.cw = ctx->tgt.cw;
   P.invscale = 1.0 / ctx->tgt.scale;
-  P.ion = ctx->f_ion.as<mt_ion>();
-  P.u = ctx->f_u.as<double>();
+  P.ion = ctx->f_in.as<mt_ion>();
+  P.u = reinterpret_cast<const double*>(ctx->f_in.as<unsigned char>() + u_off);
   P.n_u = n_u;
-  P.ev = ctx->f_ev.as<mt_event>();
+  P.ion_out = reinterpret_cast<mt_ion*>(out_d + ion_off);
+  P.ev = reinterpret_cast<mt_event*>(out_d + ev_off);
   P.max_ev = max_events;
-  P.n_used = ctx->f_used.as<long long>();
+  P.n_used = reinterpret_cast<long long*>(out_d);
   HIPCHECK(flight_launch(P, ctx->tgt.potential, st));
-  long long used[3];
-  HIPCHECK(hipMemcpyAsync(used, ctx->f_used.p, sizeof(used), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   *n_used = used[0];
   *n_events = int(used[1]);
@@ -1275,10 +1302,9 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
   if (used[2] == 2) return fail(MT_E_MORE, "event buffer full");
   if (used[2] == 3) return fail(MT_E_DEVICE, "runaway flight");
   if (used[2] == 4) return fail(MT_E_DEVICE, "non-finite-energy");
-  HIPCHECK(hipMemcpyAsync(ion, ctx->f_ion.p, sizeof(mt_ion), hipMemcpyDeviceToHost, st));
-  if (used[1] > 0)
-    HIPCHECK(hipMemcpyAsync(events, ctx->f_ev.p, size_t(used[1]) * sizeof(mt_event), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipStreamSynchronize(st));
+  if (used[2] != 0) return fail(MT_E_DEVICE, "single-flight kernel left no status");
+  std::memcpy(ion, out_h + ion_off, sizeof(mt_ion));
+  if (used[1] > 0) std::memcpy(events, out_h + ev_off, size_t(used[1]) * sizeof(mt_event));
   return MT_OK;
 }
 

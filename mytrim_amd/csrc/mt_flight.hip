@@ -7,24 +7,13 @@
 // that drive trim() one ion at a time.
 #include <hip/hip_runtime.h>
 
+#include "mt_flight.h"
 #include "mt_geometry.h"
 #include "mt_physics.h"
 #include "mt_tables.h"
 #include "mytrim_gpu.h"
 
 namespace mt {
-
-struct FlightParams {
-  const unsigned char* tables;
-  Geometry geo;
-  double tau, cw, invscale;
-  mt_ion* ion;
-  const double* u;
-  long long n_u;
-  mt_event* ev;
-  int max_ev;
-  long long* n_used;  // [0] draws used, [1] events written, [2] status
-};
 
 enum { kFlightDone = 0, kFlightNeedDraws = 1, kFlightNeedEvents = 2, kFlightRunaway = 3, kFlightNonFinite = 4 };
 
@@ -173,7 +162,7 @@ __global__ void __launch_bounds__(64) flight_kernel(FlightParams P) {
       break;
     }
   } while (ion.state == 0);
-  if (status == kFlightDone) *P.ion = ion;
+  if (status == kFlightDone) *P.ion_out = ion;
   P.n_used[0] = iu;
   P.n_used[1] = nev;
   P.n_used[2] = status;
