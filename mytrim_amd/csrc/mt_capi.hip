@@ -230,6 +230,7 @@ struct mt_ctx {
   uint64_t qcap = 0;
   // single-flight path: device tables per ion species (Z, m), event buffers
   std::map<std::pair<int, double>, std::unique_ptr<DBuf>> flight_tables;
+  std::map<std::pair<int, double>, int> flight_table_bytes;
   DBuf f_in;     // single flight: the ion and the caller's uniforms
   HBuf f_stage;  // their pinned staging
   MBuf f_out;    // what the flight kernel writes: status words, the ion, events
@@ -343,6 +344,7 @@ int mt_ctx_set_tables(mt_ctx* ctx, const double* rows, int n_rows, int n_cols) {
   // single-flight tables are cached per ion; the next flight rebuilds them
   for (auto& kv : ctx->flight_tables) kv.second->release();
   ctx->flight_tables.clear();
+  ctx->flight_table_bytes.clear();
   return MT_OK;
 }
 
@@ -604,6 +606,7 @@ int mt_ctx_set_target(mt_ctx* ctx, const mt_target* t) {
   ctx->has_target = true;
   for (auto& kv : ctx->flight_tables) kv.second->release();
   ctx->flight_tables.clear();
+  ctx->flight_table_bytes.clear();
   return MT_OK;
 }
 
@@ -1257,10 +1260,12 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
     std::vector<unsigned char> blob;
     msg = build_tables(ctx->coef, ctx->tgt, species, blob);
     if (!msg.empty()) return fail(MT_E_LIMIT, msg);
+    if (blob.size() > size_t(ctx->lds_max)) return fail(MT_E_LIMIT, "the ion's tables exceed the LDS of a workgroup");
     std::unique_ptr<DBuf> b(new DBuf);
     HIPCHECK(b->ensure(blob.size()));
     HIPCHECK(hipMemcpy(b->p, blob.data(), blob.size(), hipMemcpyHostToDevice));
     it = ctx->flight_tables.emplace(key, std::move(b)).first;
+    ctx->flight_table_bytes[key] = int(blob.size());
   }
   // one copy in (the ion and the uniforms, staged in pinned memory) and one
   // synchronisation: the kernel writes the status words, the ion and the
@@ -1283,6 +1288,7 @@ int mt_trim_flight(mt_ctx* ctx, mt_ion* ion, const double* u, long long n_u, lon
   FlightParams P;
   std::memset(&P, 0, sizeof(P));
   P.tables = it->second->as<unsigned char>();
+  P.table_bytes = ctx->flight_table_bytes[key];
   P.geo = ctx->tgt.geo;
   P.tau = ctx->tgt.tau;
   P.cw = ctx->tgt.cw;

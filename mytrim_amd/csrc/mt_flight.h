@@ -9,7 +9,8 @@
 namespace mt {
 
 struct FlightParams {
-  const unsigned char* tables;
+  const unsigned char* tables;  // the ion's table blob (a multiple of 16 bytes)
+  int table_bytes;
   Geometry geo;
   double tau, cw, invscale;
   const mt_ion* ion;  // the flight's ion

@@ -19,8 +19,17 @@ enum { kFlightDone = 0, kFlightNeedDraws = 1, kFlightNeedEvents = 2, kFlightRuna
 
 template <int POT>
 __global__ void __launch_bounds__(64) flight_kernel(FlightParams P) {
+  // the wave copies the ion's tables to LDS; each collision then reads its
+  // constants there (about a hundred cycles) instead of from L2
+  extern __shared__ __align__(16) unsigned char lds_tables[];
+  {
+    const uint4* src = (const uint4*)P.tables;
+    uint4* dst = (uint4*)lds_tables;
+    for (int i = threadIdx.x; i < P.table_bytes / 16; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  const unsigned char* blob = P.tables;
+  const unsigned char* blob = lds_tables;
   const TableHeader* H = (const TableHeader*)blob;
   TableView T;
   T.n_species = H->n_species;
@@ -169,12 +178,13 @@ __global__ void __launch_bounds__(64) flight_kernel(FlightParams P) {
 }
 
 hipError_t flight_launch(const FlightParams& P, int potential, hipStream_t stream) {
+  const unsigned lds = unsigned(P.table_bytes);
   if (potential == 1)
-    hipLaunchKernelGGL(flight_kernel<1>, dim3(1), dim3(64), 0, stream, P);
+    hipLaunchKernelGGL(flight_kernel<1>, dim3(1), dim3(64), lds, stream, P);
   else if (potential == 2)
-    hipLaunchKernelGGL(flight_kernel<2>, dim3(1), dim3(64), 0, stream, P);
+    hipLaunchKernelGGL(flight_kernel<2>, dim3(1), dim3(64), lds, stream, P);
   else
-    hipLaunchKernelGGL(flight_kernel<0>, dim3(1), dim3(64), 0, stream, P);
+    hipLaunchKernelGGL(flight_kernel<0>, dim3(1), dim3(64), lds, stream, P);
   return hipGetLastError();
 }
 
