@@ -1142,11 +1142,10 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   }
   const double share = probe.n_coll > 0 ? double(probe.coll_primary) / double(probe.n_coll) : 0.0;
   // candidate shapes (threads per block, resident blocks per CU, LDS stack
-  // entries per wave) around the region measured best on MI355X: two waves
-  // per SIMD (the FP64 kernels' registers allow no more; a kernel whose
-  // registers do not allow the requested blocks per CU runs at the most that
-  // fit) with 64 to 256 stack entries, one with deep stacks; queue capacity
-  // stays at its default.  Each runs without wave roles and with feeder
+  // entries per wave) around the region measured best on MI355X: two or three
+  // waves per SIMD (a kernel whose registers or LDS do not allow the requested
+  // blocks per CU runs at the most that fit) with 64 to 256 stack entries, one
+  // with deep stacks; queue capacity stays at its default.  Each runs without wave roles and with feeder
   // waves for 2.5 times the primaries' share, the region where roles
   // measured within a few per cent of their best on every workload; the
   // feeder count is then refined at the fastest shape with roles.  Roles are

@@ -241,8 +241,11 @@ __device__ __forceinline__ void physics(const TableView& T, int sp, int k, doubl
 
 }  // namespace
 
+// Three waves per SIMD (168 VGPRs): the FP64 and FP32 kernels fit without
+// spills (the per-history FP64 variants spill a few dwords); the surrogate
+// kernels, whose network needs more registers, keep two.
 template <int ARITH, int POT, int GEO, bool PERHIST>
-__global__ void __launch_bounds__(kCascadeMaxBlock, 2) cascade_kernel(Params P) {
+__global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascade_kernel(Params P) {
   extern __shared__ __align__(16) unsigned char lds[];
   {
     const uint4* src = (const uint4*)P.tables;
