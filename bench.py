@@ -240,10 +240,12 @@ def main():
         pilot = eng.run(step_problem(prob, 0, 2000).with_(hist0=spare))
         per_hist = max(pilot["n_coll"] / 2000.0, 1.0)
         n_sample = int(min(max(args.autotune_sample, 3e9 / per_hist), args.histories))
-        best, rate = eng.autotune(step_problem(prob, 0, n_sample).with_(hist0=spare + 2000), sample=n_sample)
+        # the tuner times its finalists again on up to four times the sample
+        n_final = int(min(4 * n_sample, args.histories))
+        best, rate = eng.autotune(step_problem(prob, 0, n_final).with_(hist0=spare + 2000), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
                   "primary_waves64": getattr(best, "primary_waves64", 0),
-                  "sample_histories": n_sample, "sample_rate": rate}
+                  "sample_histories": n_sample, "final_histories": n_final, "sample_rate": rate}
     wsize = max(1, min(20000, args.histories))
     for i in range(args.warmup):
         eng.run(step_problem(prob, 0, wsize).with_(hist0=args.steps * args.histories + i * wsize))

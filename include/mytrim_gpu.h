@@ -146,6 +146,8 @@ typedef struct {
   long long queue_ops;   /* wave-level HBM queue transfers */
   long long surrogate;   /* collisions whose angle came from the surrogate */
   long long coll_primary;/* collisions of gen-0 flights (the primaries' share of the work) */
+  double sclk_mhz;       /* mean shader clock of the launch's waves, start to exit (their cycle
+                            counters over the constant-rate wall clock) */
 } mt_tallies;
 
 /* Tuning knobs of the persistent kernel (0 = default). */
@@ -295,10 +297,17 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts
  * the collisions), then under each of 16 candidate launch shapes (block size,
  * LDS stack depth, resident blocks per CU) without wave roles and with feeder
  * waves for 2.5 times that share, then at the fastest shape with roles with
- * three other feeder counts, `reps` times each; keeps the fastest
- * (collisions per kernel second; roles only when the primaries make at least
- * 2 % of the collisions and, timed once more, run 3 % faster than the fastest
- * shape without them) in the context and returns it in *best. */
+ * three other feeder counts, `reps` times each.  A shape's rate is its
+ * launch's steady state: collisions from the start to the admission of the
+ * last primary over that time (the drain after it is left out).  When `pkas`
+ * holds at least twice `sample` histories, the finalists (the fastest shape
+ * of each occupancy within 10 % of the fastest without roles, and the
+ * fastest with roles) are timed again on up to four times `sample`
+ * histories, and those timings decide; otherwise the two fastest (without
+ * and with roles) are timed once more on the sample.  Roles are kept only
+ * when the primaries make at least 2 % of the collisions and they run 3 %
+ * faster than the fastest shape without them.  The pick stays in the context
+ * and is returned in *best, its rate in *best_rate (collisions per second). */
 int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, long long sample,
                 int reps, mt_tuning* best, double* best_rate);
 

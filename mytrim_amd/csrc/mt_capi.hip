@@ -220,6 +220,9 @@ struct mt_ctx {
   HostTarget tgt;
   bool has_target = false;
   mt_tuning tune{};
+  // the last cascade launch's wall-clock stamps (kT0, kTAdmit) and the
+  // flights ended at its last admission (kFlightsAtAdmit)
+  unsigned long long last_t0 = 0, last_tadm = 0, last_fadm = 0;
   DBuf tables, sh, cl, cx, cy, cz, cr, occ, occ_rank, occ_off, occ_idx;
   DBuf pka_sp, pka_E, pka_pos, pka_dir;
   DBuf res, wave_energy, hist_all, h_out;
@@ -811,8 +814,9 @@ static void fill_params(Params& P, const mt_ctx* ctx, const mt_primaries* pk, co
   }
 }
 
-// counters, energies and queue figures of a finished run's result block
-static void read_results(const unsigned long long* sr, double E0, mt_tallies* out) {
+// counters, energies, queue figures and the mean shader clock of a finished
+// run's result block (clock_khz: the wall clock's rate)
+static void read_results(const unsigned long long* sr, double E0, double clock_khz, mt_tallies* out) {
   const unsigned long long* ctrl = sr + kResCtrl;
   const unsigned long long* cnt = sr + kResCounters;
   const double* en = reinterpret_cast<const double*>(sr + kResEnergy);
@@ -842,6 +846,7 @@ static void read_results(const unsigned long long* sr, double E0, mt_tallies* ou
   out->range_sum2 = en[eRangeSum2];
   out->queue_peak = (long long)ctrl[kQueuePeak];
   out->queue_ops = (long long)ctrl[kQueueOps];
+  out->sclk_mhz = ctrl[kWallSum] > 0 ? double(ctrl[kClkSum]) / double(ctrl[kWallSum]) * clock_khz * 1e-3 : 0.0;
 }
 
 // MT_E_DEVICE with the names of the device error flags, if any were raised
@@ -1058,7 +1063,10 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
 
   const unsigned long long* sr = static_cast<const unsigned long long*>(ctx->stage_res.p);
   const unsigned long long* ctrl = sr + kResCtrl;
-  read_results(sr, E0, out);
+  read_results(sr, E0, ctx->clock_khz, out);
+  ctx->last_t0 = ctrl[kT0];
+  ctx->last_tadm = ctrl[kTAdmit];
+  ctx->last_fadm = ctrl[kFlightsAtAdmit];
   long long* hdst[5] = {out->range_hist, out->lat_y_hist, out->lat_z_hist, out->vac_hist, out->repl_hist};
   for (int i = 0; i < 5; ++i)
     if (hdst[i])
@@ -1104,21 +1112,36 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   mt_run_opts ro = *o;
   ro.per_history = 0;
   const mt_tuning keep = ctx->tune;
-  // one timed pass of the sample under tuning t: collisions per kernel
-  // second (the best of `reps`), 0 when the shape does not fit (LDS,
-  // occupancy), < 0 on any other error (rc set)
-  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last) -> double {
+  // one timed pass of the sample under tuning t: collisions per second (the
+  // best of `reps`), 0 when the shape does not fit (LDS, occupancy), < 0 on
+  // any other error (rc set).  The rate is the launch's steady state: the
+  // flights ended from its start to the admission of its last primary over
+  // that time, in collisions at the launch's collisions per flight.  After
+  // the last admission the launch drains: the last cascades' longest flight
+  // chains run on an emptying device.  The drain is a few tens of ms for any
+  // shape and varies by as much from one launch to the next, a third of a
+  // sample launch of 3e9 collisions but 2 % of a full one, so it is left out.
+  // Where the admissions end before half the flights (a sample of fewer
+  // histories than the cascades the device holds at once), the whole
+  // launch's kernel time is used.
+  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last, const mt_primaries* ps = nullptr) -> double {
     ctx->tune = t;
     double r = 0.0;
     for (int i = 0; i < reps; ++i) {
       mt_tallies out{};
-      rc = mt_run_cascades(ctx, &s, &ro, &out);
+      rc = mt_run_cascades(ctx, ps ? ps : &s, &ro, &out);
       if (rc == MT_E_LIMIT) {
         rc = MT_OK;
         return 0.0;
       }
       if (rc != MT_OK) return -1.0;
-      if (out.kernel_ms > 0.0) r = std::max(r, double(out.n_coll) / (1e-3 * out.kernel_ms));
+      double ri = out.kernel_ms > 0.0 ? double(out.n_coll) / (1e-3 * out.kernel_ms) : 0.0;
+      const double sec = ctx->last_tadm > ctx->last_t0 && ctx->last_t0 > 0
+                             ? double(ctx->last_tadm - ctx->last_t0) / (ctx->clock_khz * 1e3)
+                             : 0.0;
+      if (sec > 0.0 && out.n_flight > 0 && 2 * ctx->last_fadm >= (unsigned long long)out.n_flight)
+        ri = double(ctx->last_fadm) * (double(out.n_coll) / double(out.n_flight)) / sec;
+      r = std::max(r, ri);
       if (last) *last = out;
     }
     return r;
@@ -1177,24 +1200,39 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       if (r > 0.0) timed[with].emplace_back(t, r);
     }
   }
-  // per group the fastest shape, except that among shapes within 2 % of it
-  // (the spread of one timing) the deepest LDS stack wins: on the full
+  // the fastest of a set of timed shapes, except that among shapes within 2 %
+  // of it (the spread of one timing) the deepest LDS stack wins: on the full
   // launches shallower stacks than the sample's pick measured slower (C3,
   // 96 entries picked over 128: -8 %)
-  double best_r[2] = {0.0, 0.0};
-  mt_tuning best_t[2] = {keep, keep};
-  for (int w = 0; w < 2; ++w) {
+  auto pick_of = [](const std::vector<std::pair<mt_tuning, double>>& v, mt_tuning& bt, double& br) {
     double top = 0.0;
-    for (const auto& e : timed[w]) top = std::max(top, e.second);
-    for (const auto& e : timed[w]) {
+    for (const auto& e : v) top = std::max(top, e.second);
+    br = 0.0;
+    for (const auto& e : v) {
       if (e.second < 0.98 * top) continue;
-      const bool deeper = e.first.stack_cap > best_t[w].stack_cap;
-      const bool same = e.first.stack_cap == best_t[w].stack_cap;
-      if (best_r[w] == 0.0 || deeper || (same && e.second > best_r[w])) {
-        best_r[w] = e.second;
-        best_t[w] = e.first;
+      const bool deeper = e.first.stack_cap > bt.stack_cap;
+      const bool same = e.first.stack_cap == bt.stack_cap;
+      if (br == 0.0 || deeper || (same && e.second > br)) {
+        br = e.second;
+        bt = e.first;
       }
     }
+  };
+  double best_r[2] = {0.0, 0.0};
+  mt_tuning best_t[2] = {keep, keep};
+  for (int w = 0; w < 2; ++w) pick_of(timed[w], best_t[w], best_r[w]);
+  // without roles, the same pick per occupancy (waves per SIMD requested:
+  // threads per block x blocks per CU / 256)
+  std::vector<std::pair<mt_tuning, double>> per_occ;
+  for (int occ = 1; occ <= 4; ++occ) {
+    std::vector<std::pair<mt_tuning, double>> v;
+    for (const auto& e : timed[0])
+      if (std::max(1, e.first.block * e.first.blocks_per_cu / 256) == occ) v.push_back(e);
+    if (v.empty()) continue;
+    mt_tuning bt = keep;
+    double br = 0.0;
+    pick_of(v, bt, br);
+    if (br >= 0.9 * best_r[0]) per_occ.emplace_back(bt, br);
   }
   if (best_r[0] <= 0.0 && best_r[1] <= 0.0) {
     ctx->tune = keep;
@@ -1219,9 +1257,42 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       }
     }
   }
-  // the finalists once more each, alternately (a single timing varies by a
-  // few per cent)
-  if (best_r[0] > 0.0 && best_r[1] > 0.0) {
+  // Second stage, where the primaries hold at least twice the sample: the
+  // finalists are timed on up to four times as many histories, and those
+  // timings decide.  A launch's queue keeps growing past the sample's size,
+  // and shapes that spill more of their recoils to it lose ground there: on
+  // C2 (Cu -> Cu 500 keV), three waves per SIMD led two by 10 % on a
+  // 5e4-history sample and trailed by 6-12 % from 2e5 histories on, with
+  // 2.6e6 entries per queue shard against 1.5e6, on some MI355X devices
+  // (others ran both as on the sample).  The finalists are the pick of each
+  // occupancy within 10 % of the fastest shape without roles, and the roles
+  // pick.  Otherwise the two picks are timed once more each on the sample (a
+  // single timing varies by a few per cent).
+  mt_primaries s2 = *pk;
+  s2.n_hist = std::min(4 * s.n_hist, pk->n_hist);
+  const bool stage2 = s2.n_hist >= 2 * s.n_hist && (per_occ.size() > 1 || best_r[1] > 0.0);
+  if (stage2) {
+    double r0 = 0.0;
+    for (const auto& e : per_occ) {
+      const double r = rate_of(e.first, rc, nullptr, &s2);
+      if (r < 0.0) {
+        ctx->tune = keep;
+        return rc;
+      }
+      if (r > r0) {
+        r0 = r;
+        best_t[0] = e.first;
+      }
+    }
+    best_r[0] = r0;
+    if (best_r[1] > 0.0) {
+      best_r[1] = rate_of(best_t[1], rc, nullptr, &s2);
+      if (best_r[1] < 0.0) {
+        ctx->tune = keep;
+        return rc;
+      }
+    }
+  } else if (best_r[0] > 0.0 && best_r[1] > 0.0) {
     for (int w = 0; w < 2; ++w) {
       const double r = rate_of(best_t[w], rc, nullptr);
       if (r < 0.0) {
@@ -1649,7 +1720,7 @@ int mt_run_cascades_noalloc(mt_ctx* ctx, const mt_primaries* pk, const mt_run_op
   HIPCHECK(hipStreamSynchronize(st));
   float ms = 0.0f;
   HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  read_results(resh, E0, out);
+  read_results(resh, E0, ctx->clock_khz, out);
   out->kernel_ms = ms;
   out->grid = sh.grid;
   out->block = sh.block;

@@ -37,6 +37,17 @@ enum Ctrl {
   kLogCount = 32,       // event records produced (diagnostic kernels)
   kError = 64,          // error bits
   kQueueOps = 65,       // wave-level queue transfers
+  // steady-state throughput of a launch, read by the launch-shape tuner:
+  // wall-clock stamp of the first wave's start, of the admission of the last
+  // primary, and the flights ended by then (kFlightsEnded at that moment; each
+  // wave adds its ended flights in batches of up to 256)
+  kT0 = 66,
+  kTAdmit = 67,
+  kFlightsAtAdmit = 68,
+  // the waves' shader cycles and wall-clock ticks from start to exit, summed:
+  // their ratio is the launch's mean shader clock
+  kClkSum = 69,
+  kWallSum = 70,
   kQueuePeak = 80,      // largest tail - head seen at a push
   kLive = 96,           // admitted cascades not yet complete
   kCascDone = 97,       // completed cascades

@@ -276,7 +276,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
   const Geometry& G = P.geo;
 
   unsigned long long* ctrl = P.ctrl;
-  if (lane == 0) atomicAdd(&ctrl[kState], 1ull << 16);  // live += 1
+  const unsigned long long clk0 = (unsigned long long)clock64();
+  const unsigned long long wall0 = (unsigned long long)wall_clock64();
+  if (lane == 0) {
+    atomicAdd(&ctrl[kState], 1ull << 16);  // live += 1
+    // the first wave to start stamps the launch (the words start at zero)
+    atomicCAS(&ctrl[kT0], 0ull, wall0);
+  }
 
   // wave-uniform state
   int scount = 0;          // entries in this wave's LDS stack
@@ -474,7 +480,15 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
           if (lane == 0) first = atomicAdd(&ctrl[kPkaNext], (unsigned long long)n_adm);
           first = bcast64(first, 0);
           const long long got = min(n_adm, max(0ll, A->n_hist - (long long)first));
-          if (lane == 0 && got > 0) atomicAdd(&ctrl[kLive], (unsigned long long)got);
+          if (lane == 0 && got > 0) {
+            atomicAdd(&ctrl[kLive], (unsigned long long)got);
+            if ((long long)first + got == A->n_hist) {  // the last primary: the drain starts
+              __hip_atomic_store(&ctrl[kFlightsAtAdmit], (unsigned long long)ld_relaxed(&ctrl[kFlightsEnded]), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_store(&ctrl[kTAdmit], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+            }
+          }
           const long long jh = (long long)first + lane;
           if (lane < got) {
             __hip_atomic_store(&A->h_out[jh], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1066,6 +1080,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
     atomicAdd(&C[cCollPrimary], n_coll0);
     atomicAdd(&ctrl[kQueueOps], c_qops);
     if (fe_pending) atomicAdd(&ctrl[kFlightsEnded], fe_pending);
+    atomicAdd(&ctrl[kClkSum], (unsigned long long)clock64() - clk0);
+    atomicAdd(&ctrl[kWallSum], (unsigned long long)wall_clock64() - wall0);
     const int gw = blockIdx.x * nwb + wid;
     double* we = A->wave_energy + (size_t)gw * (2 * kNumEnergy);
     we[2 * eEel] = eel_s;

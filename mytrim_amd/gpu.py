@@ -82,7 +82,8 @@ class Tallies(C.Structure):
         ("repl_hist", lp), ("h_vac", ip), ("h_coll", lp), ("h_flights", ip), ("h_x", dp),
         ("h_y", dp), ("h_z", dp), ("h_state", ip), ("h_Eel", dp), ("kernel_ms", C.c_double),
         ("grid", C.c_int), ("block", C.c_int), ("waves", C.c_int), ("queue_peak", C.c_longlong),
-        ("queue_ops", C.c_longlong), ("surrogate", C.c_longlong), ("coll_primary", C.c_longlong)])
+        ("queue_ops", C.c_longlong), ("surrogate", C.c_longlong), ("coll_primary", C.c_longlong),
+        ("sclk_mhz", C.c_double)])
 
 
 class Tuning(C.Structure):
@@ -277,7 +278,9 @@ class Engine:
 
     def autotune(self, prob: Problem, sample: int = 2000, reps: int = 1, arith: int = MT_FP64):
         """Pick the fastest launch shape on the first `sample` histories of `prob`
-        (kept in the context for later runs); returns (Tuning, collisions/s)."""
+        (kept in the context for later runs); when `prob` holds at least twice
+        `sample` histories, the finalists are timed again on up to four times
+        `sample`.  Returns (Tuning, collisions/s)."""
         pk, o, keep = self._launch_args(prob, False, arith)
         best = Tuning()
         rate = C.c_double()
@@ -341,7 +344,7 @@ class Engine:
         res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
                    kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
                    queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate,
-                   coll_primary=out.coll_primary)
+                   coll_primary=out.coll_primary, sclk_mhz=out.sclk_mhz)
         res.update(hists)
         res.update(per)
         del keep
