@@ -331,6 +331,11 @@ def main():
                              "cascade_kernel<0,0,0,false>", bpc),
         "variants": variants,
     }
+    # the waves' mean shader clock over the timed launches: the published FP64
+    # peak assumes the 2400 MHz maximum
+    sclk = sum(r["sclk_mhz"] for r in steps) / len(steps)
+    out["roofline"]["sclk_mhz"] = sclk
+    out["roofline"]["peak_at_sclk"] = FP64_PEAK_TFLOPS * sclk / 2400.0
     if not args.no_cpu:
         # one thread per core of the GPU's host-core share (at most 16 worker
         # threads per GPU on the benchmark machines, whatever os.cpu_count()

@@ -452,6 +452,8 @@ def test_autotune_picks_a_working_shape(eng):
     eng.tune()
     s = eng.run(prob.with_(n_hist=300), per_history=True)
     assert np.array_equal(r["h_coll"], s["h_coll"]) and r["vac"] == s["vac"]
+    # the launch record's mean shader clock (MI355X: up to 2.4 GHz)
+    assert 300.0 < r["sclk_mhz"] < 3000.0 and 300.0 < s["sclk_mhz"] < 3000.0
 
 
 @pytest.mark.parametrize("block,cap", [(64, 256), (64, 512), (128, 256)])
