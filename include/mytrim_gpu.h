@@ -294,7 +294,7 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts
 /* Autotuner (no counterpart in the reference: the reference's tuning knob is
  * its thread count, apps/runmytrim.C:59-68).  Runs the first `sample`
  * histories of `pkas` once at the default shape (for the primaries' share of
- * the collisions), then under each of 16 candidate launch shapes (block size,
+ * the collisions), then under each of 17 candidate launch shapes (block size,
  * LDS stack depth, resident blocks per CU) without wave roles and with feeder
  * waves for 2.5 times that share, then at the fastest shape with roles with
  * three other feeder counts, `reps` times each.  A shape's rate is its

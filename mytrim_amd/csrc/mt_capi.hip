@@ -1179,12 +1179,13 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   static const int kCand[][3] = {{256, 2, 256}, {256, 2, 192}, {256, 2, 128}, {256, 2, 96}, {256, 2, 64},
                                  {128, 4, 128}, {128, 6, 128}, {128, 6, 96},  {128, 6, 64}, {64, 8, 128},
                                  {256, 1, 384}, {256, 1, 256}, {128, 3, 256}, {128, 3, 128}, {256, 3, 128},
-                                 {256, 3, 96}};
+                                 {256, 3, 96},  {128, 5, 128}};
   // below a 2 % share (heavy-ion cascades of thousands of flights) feeders only
   // add queue traffic: on C2 (1.2 %) roles measured within noise on the sample
   // and 1-4 % slower on full launches, three times out of three
   const int k_roles = share >= 0.02 ? feeders(share, 2.5) : 0;
   std::vector<std::pair<mt_tuning, double>> timed[2];  // without, with roles
+  std::vector<int> occ0;  // waves per CU each shape without roles ran at
   for (const auto& c : kCand) {
     for (int with = 0; with < (k_roles > 0 ? 2 : 1); ++with) {
       mt_tuning t = t0;
@@ -1192,12 +1193,16 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       t.blocks_per_cu = c[1];
       t.stack_cap = c[2];
       t.primary_waves64 = with ? k_roles : 0;
-      const double r = rate_of(t, rc, nullptr);
+      mt_tallies last{};
+      const double r = rate_of(t, rc, &last);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
       }
-      if (r > 0.0) timed[with].emplace_back(t, r);
+      if (r > 0.0) {
+        timed[with].emplace_back(t, r);
+        if (!with) occ0.push_back(ctx->cus > 0 ? last.waves / ctx->cus : 0);
+      }
     }
   }
   // the fastest of a set of timed shapes, except that among shapes within 2 %
@@ -1221,13 +1226,17 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   double best_r[2] = {0.0, 0.0};
   mt_tuning best_t[2] = {keep, keep};
   for (int w = 0; w < 2; ++w) pick_of(timed[w], best_t[w], best_r[w]);
-  // without roles, the same pick per occupancy (waves per SIMD requested:
-  // threads per block x blocks per CU / 256)
+  // without roles, the same pick per occupancy: the waves per CU a shape ran
+  // at, which the registers and the LDS (stacks and tables) may hold below
+  // the requested threads per block x blocks per CU
   std::vector<std::pair<mt_tuning, double>> per_occ;
-  for (int occ = 1; occ <= 4; ++occ) {
+  std::vector<int> occs(occ0);
+  std::sort(occs.begin(), occs.end());
+  occs.erase(std::unique(occs.begin(), occs.end()), occs.end());
+  for (int occ : occs) {
     std::vector<std::pair<mt_tuning, double>> v;
-    for (const auto& e : timed[0])
-      if (std::max(1, e.first.block * e.first.blocks_per_cu / 256) == occ) v.push_back(e);
+    for (size_t i = 0; i < timed[0].size(); ++i)
+      if (occ0[i] == occ) v.push_back(timed[0][i]);
     if (v.empty()) continue;
     mt_tuning bt = keep;
     double br = 0.0;
