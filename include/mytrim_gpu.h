@@ -156,7 +156,10 @@ typedef struct {
   int blocks_per_cu;  /* resident workgroups per CU (0: two waves per SIMD) */
   int stack_cap;      /* per-wave LDS recoil stack entries (0: 128) */
   long long queue_cap;/* HBM queue entries (power of two) */
-  int lds_hist;       /* bin vacancy/replacement depths in LDS first */
+  int lds_hist;       /* vacancy/replacement depth bins counted in LDS (the
+                         leading ones; the rest in HBM): 0 as many as fit
+                         without costing a resident workgroup, < 0 none,
+                         n > 0 at most n */
   int graph;          /* >= 0: replay the run sequence (reset, kernel, flush,
                          copies) from a cached hipGraph; < 0: launch directly */
   int primary_waves64;/* 0: every wave admits primaries and keeps its recoils;

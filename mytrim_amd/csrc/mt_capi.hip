@@ -687,21 +687,48 @@ struct RunShape {
 
 // launch shape of the persistent kernel: the context's tuning, else the shape
 // measured best for Cu -> Cu cascades (two waves per SIMD, 128-entry LDS
-// stacks); depth bins in LDS where they fit
+// stacks).  The leading vacancy/replacement depth bins are counted in LDS, as
+// many as fit without costing a resident block: the requested blocks per CU
+// (or what the registers, tables and stacks allow, if fewer) decide, and bins
+// beyond those in LDS are counted in HBM.
 static int run_shape(const mt_ctx* ctx, const mt_run_opts* o, size_t table_bytes, bool perhist, RunShape& s) {
   s.block = ctx->tune.block ? ctx->tune.block : 256;
   s.cap = ctx->tune.stack_cap ? ctx->tune.stack_cap : 128;
   const int nwb = s.block / 64;
-  const size_t lds_hist_bytes = 2 * size_t(o->vac_nb + 2) * sizeof(uint32_t);
-  s.lds = table_bytes + size_t(nwb) * cascade_stack_bytes(s.cap);
-  s.lds_hist = ctx->tune.lds_hist >= 0 ? 1 : 0;
+  const size_t base = table_bytes + size_t(nwb) * cascade_stack_bytes(s.cap);
   const size_t lds_max = size_t(ctx->lds_max);
-  if (s.lds + lds_hist_bytes > lds_max) s.lds_hist = 0;
-  if (s.lds_hist) s.lds += lds_hist_bytes;
-  if (s.lds > lds_max) return fail(MT_E_LIMIT, "tables and stacks exceed the LDS of a workgroup");
+  if (base > lds_max) return fail(MT_E_LIMIT, "tables and stacks exceed the LDS of a workgroup");
+  const int arith = o->arith, pot = ctx->tgt.potential, geo = ctx->tgt.geo.kind;
+  const int want = ctx->tune.blocks_per_cu > 0 ? ctx->tune.blocks_per_cu : kDefaultWavesPerCu * 64 / s.block;
+  // LDS bytes of nl bins of each histogram (16-byte multiples)
+  auto hist_bytes = [](int nl) { return (2 * size_t(nl) * sizeof(uint32_t) + 15) & ~size_t(15); };
+  int nl = ctx->tune.lds_hist < 0 ? 0 : o->vac_nb + 2;
+  if (ctx->tune.lds_hist > 0) nl = std::min(nl, ctx->tune.lds_hist);
+  while (nl > 0 && base + hist_bytes(nl) > lds_max) nl /= 2;
+  if (nl > 0) {
+    int top = 0, with = 0;
+    HIPCHECK(cascade_occupancy(arith, pot, geo, perhist, s.block, base, &top));
+    const int target = std::min(top, want);
+    HIPCHECK(cascade_occupancy(arith, pot, geo, perhist, s.block, base + hist_bytes(nl), &with));
+    if (with < target) {
+      // the most bins that keep `target` blocks: occupancy falls as LDS grows
+      int lo = 0, hi = nl;
+      while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        HIPCHECK(cascade_occupancy(arith, pot, geo, perhist, s.block, base + hist_bytes(mid), &with));
+        if (with >= target)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      nl = lo;
+    }
+  }
+  s.lds_hist = nl;
+  s.lds = base + (nl > 0 ? hist_bytes(nl) : 0);
   int bpc = 0;
-  HIPCHECK(cascade_occupancy(o->arith, ctx->tgt.potential, ctx->tgt.geo.kind, perhist, s.block, s.lds, &bpc));
-  bpc = std::min(bpc, ctx->tune.blocks_per_cu > 0 ? ctx->tune.blocks_per_cu : kDefaultWavesPerCu * 64 / s.block);
+  HIPCHECK(cascade_occupancy(arith, pot, geo, perhist, s.block, s.lds, &bpc));
+  bpc = std::min(bpc, want);
   if (bpc < 1) return fail(MT_E_LIMIT, "cascade kernel does not fit on a CU");
   s.grid = bpc * ctx->cus;
   s.waves = s.grid * nwb;

@@ -144,7 +144,7 @@ struct Params {
   int lat_nb;
   double vac_x0, vac_bw, vac_ibw;  // vac_ibw = 1 / vac_bw
   int vac_nb;
-  int lds_hist;                   // bin vac/repl depths in LDS, flush at exit
+  int lds_hist;                   // leading vac/repl depth bins counted in LDS (flushed at exit)
   // per history (index hist - hist0)
   int* h_vac;
   unsigned long long* h_coll;

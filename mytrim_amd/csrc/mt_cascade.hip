@@ -258,10 +258,10 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
   const int cap = P.stack_cap;
   StackView S;
   S.bind(lds + P.table_bytes + wid * stack_bytes(cap), cap);
+  // depth bins [0, lds_hist) of the vacancy and replacement histograms
   uint32_t* lvac = (uint32_t*)(lds + P.table_bytes + nwb * stack_bytes(cap));
-  uint32_t* lrepl = lvac + (P.vac_nb + 2);
-  if (P.lds_hist)
-    for (int i = threadIdx.x; i < 2 * (P.vac_nb + 2); i += blockDim.x) lvac[i] = 0u;
+  uint32_t* lrepl = lvac + P.lds_hist;
+  for (int i = threadIdx.x; i < 2 * P.lds_hist; i += blockDim.x) lvac[i] = 0u;
   __syncthreads();
 
   const TableHeader* H = (const TableHeader*)lds;
@@ -724,7 +724,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
           const bool vac = F.E > ec.Edisp;
           {
             const int bn = bin_of_r(rx, C->vac_x0, C->vac_bw, C->vac_ibw, C->vac_nb);
-            if (C->lds_hist)
+            if (bn < C->lds_hist)
               atomicAdd(&(vac ? lvac : lrepl)[bn], 1u);
             else
               atomicAdd(&(vac ? cold_args()->vac_hist : cold_args()->repl_hist)[bn], 1ull);
@@ -1092,7 +1092,7 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
   if (err) atomicOr((unsigned int*)&ctrl[kError], err);
   if (P.lds_hist) {
     __syncthreads();
-    for (int i = threadIdx.x; i < P.vac_nb + 2; i += blockDim.x) {
+    for (int i = threadIdx.x; i < P.lds_hist; i += blockDim.x) {
       if (lvac[i]) atomicAdd(&A->vac_hist[i], (unsigned long long)lvac[i]);
       if (lrepl[i]) atomicAdd(&A->repl_hist[i], (unsigned long long)lrepl[i]);
     }

@@ -456,6 +456,25 @@ def test_autotune_picks_a_working_shape(eng):
     assert 300.0 < r["sclk_mhz"] < 3000.0 and 300.0 < s["sclk_mhz"] < 3000.0
 
 
+def test_depth_bins_split_between_lds_and_hbm(eng):
+    """Vacancy/replacement depth bins: all in HBM, the first few in LDS and the
+    rest in HBM, or as many in LDS as fit (three blocks per CU asked for):
+    identical histograms."""
+    prob = configs.c2_cu_cu(200, seed=5)
+    out = []
+    try:
+        for lh, bpc in ((-1, 0), (7, 0), (300, 0), (0, 3), (0, 0)):
+            eng.tune(block=256, blocks_per_cu=bpc, stack_cap=128, lds_hist=lh)
+            out.append(eng.run(prob))
+    finally:
+        eng.tune()
+    assert out[0]["vac_hist"].sum() > 0 and out[0]["repl_hist"].sum() > 0
+    for r in out[1:]:
+        assert np.array_equal(r["vac_hist"], out[0]["vac_hist"])
+        assert np.array_equal(r["repl_hist"], out[0]["repl_hist"])
+        assert r["vac"] == out[0]["vac"] and r["repl"] == out[0]["repl"]
+
+
 @pytest.mark.parametrize("block,cap", [(64, 256), (64, 512), (128, 256)])
 def test_deep_lds_stacks_match_oracle(eng, block, cap):
     """LDS stacks deeper than a wave: refills still move at most one entry per
