@@ -154,7 +154,7 @@ def _tuned(eng, sample, arith, n_sample):
         return None
     best, rate = eng.autotune(sample, sample=sample.n_hist, arith=arith)
     return {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
-            "primary_waves64": getattr(best, "primary_waves64", 0), "sample_rate": rate}
+            "primary_waves64": getattr(best, "primary_waves64", 0), "lds_hist": best.lds_hist, "sample_rate": rate}
 
 
 def step_problem(prob, i, n_hist):
@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--stack-cap", type=int, default=0)
     ap.add_argument("--primary-waves64", type=int, default=0,
                     help="feeder waves per 64 with a fixed launch shape (0: no wave roles)")
+    ap.add_argument("--lds-hist", type=int, default=0,
+                    help="depth bins in LDS with a fixed launch shape (0: as many as fit, -1: none)")
     ap.add_argument("--variant-histories", type=int, default=100000,
                     help="histories for the FP32 and FP32+surrogate lines (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=4_000_000)
@@ -225,11 +227,11 @@ def main():
     eng = gpu.Engine(0)
     eng.set_target(prob)
     tuning = None
-    if args.block or args.blocks_per_cu or args.stack_cap or args.primary_waves64:
+    if args.block or args.blocks_per_cu or args.stack_cap or args.primary_waves64 or args.lds_hist:
         eng.tune(block=args.block, blocks_per_cu=args.blocks_per_cu, stack_cap=args.stack_cap,
-                 primary_waves64=args.primary_waves64)
+                 primary_waves64=args.primary_waves64, lds_hist=args.lds_hist)
         tuning = {"block": args.block, "blocks_per_cu": args.blocks_per_cu, "stack_cap": args.stack_cap,
-                  "primary_waves64": args.primary_waves64, "fixed": True}
+                  "primary_waves64": args.primary_waves64, "lds_hist": args.lds_hist, "fixed": True}
     elif args.autotune_sample > 0:
         # the tuning sample holds at least --autotune-sample histories and about
         # 3e9 collisions, at most one timed step: small cascades need more
@@ -244,7 +246,7 @@ def main():
         n_final = int(min(4 * n_sample, args.histories))
         best, rate = eng.autotune(step_problem(prob, 0, n_final).with_(hist0=spare + 2000), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
-                  "primary_waves64": getattr(best, "primary_waves64", 0),
+                  "primary_waves64": getattr(best, "primary_waves64", 0), "lds_hist": best.lds_hist,
                   "sample_histories": n_sample, "final_histories": n_final, "sample_rate": rate}
     wsize = max(1, min(20000, args.histories))
     for i in range(args.warmup):

@@ -1302,15 +1302,35 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // 2.6e6 entries per queue shard against 1.5e6, on some MI355X devices
   // (others ran both as on the sample).  The finalists are the pick of each
   // occupancy within 10 % of the fastest shape without roles, and the roles
-  // pick.  Otherwise the two picks are timed once more each on the sample (a
-  // single timing varies by a few per cent).
+  // pick.  Each finalist also runs with every depth bin in HBM, unless the
+  // caller set lds_hist: LDS atomics on the bins are in order with the stack's
+  // LDS traffic and serialise on a hot bin, HBM atomics are neither but
+  // contend in L2 where a few bins take most counts.  On C5 (vacancies spread
+  // over 1.5 um) the bins in HBM ran 8 % faster at 256 x 3; on C2 (shallow,
+  // dense cascades) a third slower.  Otherwise the two picks are timed once
+  // more each on the sample (a single timing varies by a few per cent).
   mt_primaries s2 = *pk;
   s2.n_hist = std::min(4 * s.n_hist, pk->n_hist);
-  const bool stage2 = s2.n_hist >= 2 * s.n_hist && (per_occ.size() > 1 || best_r[1] > 0.0);
+  const bool stage2 = s2.n_hist >= 2 * s.n_hist;
+  // the better of a finalist with the caller's depth bins and with all bins
+  // in HBM, on the second-stage histories; < 0 on error (rc set)
+  auto final_rate = [&](mt_tuning& t) -> double {
+    double r = rate_of(t, rc, nullptr, &s2);
+    if (r < 0.0 || keep.lds_hist != 0) return r;
+    mt_tuning h = t;
+    h.lds_hist = -1;
+    const double rh = rate_of(h, rc, nullptr, &s2);
+    if (rh < 0.0) return rh;
+    if (rh > r) {
+      r = rh;
+      t = h;
+    }
+    return r;
+  };
   if (stage2) {
     double r0 = 0.0;
-    for (const auto& e : per_occ) {
-      const double r = rate_of(e.first, rc, nullptr, &s2);
+    for (auto& e : per_occ) {
+      const double r = final_rate(e.first);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
@@ -1322,7 +1342,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
     }
     best_r[0] = r0;
     if (best_r[1] > 0.0) {
-      best_r[1] = rate_of(best_t[1], rc, nullptr, &s2);
+      best_r[1] = final_rate(best_t[1]);
       if (best_r[1] < 0.0) {
         ctx->tune = keep;
         return rc;
