@@ -308,10 +308,12 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts
  * fastest with roles) are timed again on up to four times `sample`
  * histories, each also with every depth bin in HBM (lds_hist -1) unless the
  * context's tuning sets lds_hist, and those timings decide; otherwise the two
- * fastest (without and with roles) are timed once more on the sample.  Roles are kept only
- * when the primaries make at least 2 % of the collisions and they run 3 %
- * faster than the fastest shape without them.  The pick stays in the context
- * and is returned in *best, its rate in *best_rate (collisions per second). */
+ * fastest (without and with roles) are timed once more on the sample.
+ * Roles are kept only when the primaries make at least 2 % of the
+ * collisions and their whole launch (drain included) runs 3 % faster than
+ * that of the fastest shape without them.  The pick stays in the context and
+ * is returned in *best, its steady-state rate in *best_rate (collisions per
+ * second). */
 int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, long long sample,
                 int reps, mt_tuning* best, double* best_rate);
 

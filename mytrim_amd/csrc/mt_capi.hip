@@ -1151,9 +1151,13 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // Where the admissions end before half the flights (a sample of fewer
   // histories than the cascades the device holds at once), the whole
   // launch's kernel time is used.
-  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last, const mt_primaries* ps = nullptr) -> double {
+  // `whole` (optional) receives the whole launch's rate, drain included:
+  // collisions per kernel second.
+  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last, const mt_primaries* ps = nullptr,
+                     double* whole = nullptr) -> double {
     ctx->tune = t;
     double r = 0.0;
+    if (whole) *whole = 0.0;
     for (int i = 0; i < reps; ++i) {
       mt_tallies out{};
       rc = mt_run_cascades(ctx, ps ? ps : &s, &ro, &out);
@@ -1163,6 +1167,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       }
       if (rc != MT_OK) return -1.0;
       double ri = out.kernel_ms > 0.0 ? double(out.n_coll) / (1e-3 * out.kernel_ms) : 0.0;
+      if (whole) *whole = std::max(*whole, ri);
       const double sec = ctx->last_tadm > ctx->last_t0 && ctx->last_t0 > 0
                              ? double(ctx->last_tadm - ctx->last_t0) / (ctx->clock_khz * 1e3)
                              : 0.0;
@@ -1314,23 +1319,31 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   const bool stage2 = s2.n_hist >= 2 * s.n_hist;
   // the better of a finalist with the caller's depth bins and with all bins
   // in HBM, on the second-stage histories; < 0 on error (rc set)
-  auto final_rate = [&](mt_tuning& t) -> double {
-    double r = rate_of(t, rc, nullptr, &s2);
+  auto final_rate = [&](mt_tuning& t, double& whole) -> double {
+    double r = rate_of(t, rc, nullptr, &s2, &whole);
     if (r < 0.0 || keep.lds_hist != 0) return r;
     mt_tuning h = t;
     h.lds_hist = -1;
-    const double rh = rate_of(h, rc, nullptr, &s2);
+    double wh = 0.0;
+    const double rh = rate_of(h, rc, nullptr, &s2, &wh);
     if (rh < 0.0) return rh;
     if (rh > r) {
       r = rh;
       t = h;
+      whole = wh;
     }
     return r;
   };
+  // The shapes are ranked on the steady state, roles against no roles on the
+  // whole launch: on C1 (long He primary flights) roles gain most in the
+  // drain, where the last primaries' flights would otherwise hold a few
+  // waves while the rest idle.
+  double whole_r[2] = {0.0, 0.0};
   if (stage2) {
     double r0 = 0.0;
     for (auto& e : per_occ) {
-      const double r = final_rate(e.first);
+      double wr = 0.0;
+      const double r = final_rate(e.first, wr);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
@@ -1338,11 +1351,12 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       if (r > r0) {
         r0 = r;
         best_t[0] = e.first;
+        whole_r[0] = wr;
       }
     }
     best_r[0] = r0;
     if (best_r[1] > 0.0) {
-      best_r[1] = final_rate(best_t[1]);
+      best_r[1] = final_rate(best_t[1], whole_r[1]);
       if (best_r[1] < 0.0) {
         ctx->tune = keep;
         return rc;
@@ -1350,7 +1364,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
     }
   } else if (best_r[0] > 0.0 && best_r[1] > 0.0) {
     for (int w = 0; w < 2; ++w) {
-      const double r = rate_of(best_t[w], rc, nullptr);
+      const double r = rate_of(best_t[w], rc, nullptr, nullptr, &whole_r[w]);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
@@ -1358,7 +1372,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       best_r[w] = std::max(best_r[w], r);
     }
   }
-  const int pick = best_r[1] > 1.03 * best_r[0] ? 1 : 0;
+  const int pick = best_r[0] <= 0.0 || (best_r[1] > 0.0 && whole_r[1] > 1.03 * whole_r[0]) ? 1 : 0;
   const double best_rate_v = best_r[pick];
   const mt_tuning best_v = best_t[pick];
   ctx->tune = best_v;
