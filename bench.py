@@ -242,9 +242,10 @@ def main():
         pilot = eng.run(step_problem(prob, 0, 2000).with_(hist0=spare))
         per_hist = max(pilot["n_coll"] / 2000.0, 1.0)
         n_sample = int(min(max(args.autotune_sample, 3e9 / per_hist), args.histories))
-        # the tuner times its finalists again on up to four times the sample
+        # tuned for a launch of one step's histories: the tuner times its
+        # finalists again on up to four times the sample
         n_final = int(min(4 * n_sample, args.histories))
-        best, rate = eng.autotune(step_problem(prob, 0, n_final).with_(hist0=spare + 2000), sample=n_sample)
+        best, rate = eng.autotune(step_problem(prob, 0, args.histories).with_(hist0=spare + 2000), sample=n_sample)
         tuning = {"block": best.block, "blocks_per_cu": best.blocks_per_cu, "stack_cap": best.stack_cap,
                   "primary_waves64": getattr(best, "primary_waves64", 0), "lds_hist": best.lds_hist,
                   "sample_histories": n_sample, "final_histories": n_final, "sample_rate": rate}

@@ -148,6 +148,12 @@ typedef struct {
   long long coll_primary;/* collisions of gen-0 flights (the primaries' share of the work) */
   double sclk_mhz;       /* mean shader clock of the launch's waves, start to exit (their cycle
                             counters over the constant-rate wall clock) */
+  double half_ms;        /* first wave's start to the admission of the middle primary (the
+                            ramp-up is over); 0 if not recorded */
+  long long flights_at_half; /* flights ended by then (each wave adds its own in batches) */
+  double admit_ms;       /* first wave's start to the admission of the last primary (the drain
+                            follows); 0 if not recorded */
+  long long flights_at_admit; /* flights ended by then */
 } mt_tallies;
 
 /* Tuning knobs of the persistent kernel (0 = default). */
@@ -299,21 +305,22 @@ int mt_run_cascades_log(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts
  * histories of `pkas` once at the default shape (for the primaries' share of
  * the collisions), then under each of 17 candidate launch shapes (block size,
  * LDS stack depth, resident blocks per CU) without wave roles and with feeder
- * waves for 2.5 times that share, then at the fastest shape with roles with
- * three other feeder counts, `reps` times each.  A shape's rate is its
- * launch's steady state: collisions from the start to the admission of the
- * last primary over that time (the drain after it is left out).  When `pkas`
- * holds at least twice `sample` histories, the finalists (the fastest shape
- * of each occupancy within 10 % of the fastest without roles, and the
- * fastest with roles) are timed again on up to four times `sample`
- * histories, each also with every depth bin in HBM (lds_hist -1) unless the
- * context's tuning sets lds_hist, and those timings decide; otherwise the two
- * fastest (without and with roles) are timed once more on the sample.
- * Roles are kept only when the primaries make at least 2 % of the
- * collisions and their whole launch (drain included) runs 3 % faster than
- * that of the fastest shape without them.  The pick stays in the context and
- * is returned in *best, its steady-state rate in *best_rate (collisions per
- * second). */
+ * waves for 2.5 times that share, each with as many depth bins in LDS as fit
+ * and with every bin in HBM (unless the context's tuning sets lds_hist), then
+ * with three other feeder counts at the fastest roles shape of each
+ * occupancy (waves per CU), `reps` times each.  A shape's rate is projected for a launch of all of `pkas`:
+ * the sample launch's steady rate (collisions from the start to the
+ * admission of the last primary, over that time) for the collisions of that
+ * many histories, plus the sample launch's drain (the time after the last
+ * admission).  When `pkas` holds at least twice `sample` histories, the
+ * finalists (the fastest shape of each occupancy within 10 % of the fastest,
+ * without and with roles, and without roles the runner-up of another shape)
+ * are timed again on up to four times `sample`
+ * histories, each with both depth-bin splits, and those timings decide; otherwise the two fastest (without and with roles) are
+ * timed once more on the sample.  Roles are kept only when the primaries
+ * make at least 2 % of the collisions and they run 3 % faster than the
+ * fastest shape without them.  The pick stays in the context and is returned
+ * in *best, its projected rate in *best_rate (collisions per second). */
 int mt_autotune(mt_ctx* ctx, const mt_primaries* pkas, const mt_run_opts* opts, long long sample,
                 int reps, mt_tuning* best, double* best_rate);
 

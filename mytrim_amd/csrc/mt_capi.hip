@@ -222,7 +222,7 @@ struct mt_ctx {
   mt_tuning tune{};
   // the last cascade launch's wall-clock stamps (kT0, kTAdmit) and the
   // flights ended at its last admission (kFlightsAtAdmit)
-  unsigned long long last_t0 = 0, last_tadm = 0, last_fadm = 0;
+  unsigned long long last_t0 = 0, last_tadm = 0, last_fadm = 0, last_thalf = 0, last_fhalf = 0;
   DBuf tables, sh, cl, cx, cy, cz, cr, occ, occ_rank, occ_off, occ_idx;
   DBuf pka_sp, pka_E, pka_pos, pka_dir;
   DBuf res, wave_energy, hist_all, h_out;
@@ -1094,6 +1094,16 @@ static int run_cascades(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* 
   ctx->last_t0 = ctrl[kT0];
   ctx->last_tadm = ctrl[kTAdmit];
   ctx->last_fadm = ctrl[kFlightsAtAdmit];
+  out->admit_ms = ctx->last_tadm > ctx->last_t0 && ctx->last_t0 > 0
+                      ? double(ctx->last_tadm - ctx->last_t0) / ctx->clock_khz
+                      : 0.0;
+  out->flights_at_admit = (long long)ctx->last_fadm;
+  ctx->last_thalf = ctrl[kTHalf];
+  ctx->last_fhalf = ctrl[kFlightsAtHalf];
+  out->half_ms = ctx->last_thalf > ctx->last_t0 && ctx->last_t0 > 0
+                     ? double(ctx->last_thalf - ctx->last_t0) / ctx->clock_khz
+                     : 0.0;
+  out->flights_at_half = (long long)ctx->last_fhalf;
   long long* hdst[5] = {out->range_hist, out->lat_y_hist, out->lat_z_hist, out->vac_hist, out->repl_hist};
   for (int i = 0; i < 5; ++i)
     if (hdst[i])
@@ -1139,40 +1149,53 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   mt_run_opts ro = *o;
   ro.per_history = 0;
   const mt_tuning keep = ctx->tune;
-  // one timed pass of the sample under tuning t: collisions per second (the
-  // best of `reps`), 0 when the shape does not fit (LDS, occupancy), < 0 on
-  // any other error (rc set).  The rate is the launch's steady state: the
-  // flights ended from its start to the admission of its last primary over
-  // that time, in collisions at the launch's collisions per flight.  After
-  // the last admission the launch drains: the last cascades' longest flight
-  // chains run on an emptying device.  The drain is a few tens of ms for any
-  // shape and varies by as much from one launch to the next, a third of a
-  // sample launch of 3e9 collisions but 2 % of a full one, so it is left out.
-  // Where the admissions end before half the flights (a sample of fewer
-  // histories than the cascades the device holds at once), the whole
-  // launch's kernel time is used.
-  // `whole` (optional) receives the whole launch's rate, drain included:
-  // collisions per kernel second.
-  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last, const mt_primaries* ps = nullptr,
-                     double* whole = nullptr) -> double {
+  // one timed pass of a sample under tuning t (the best of `reps`): the rate
+  // projected for a launch of all pk->n_hist histories, in collisions per
+  // second; 0 when the shape does not fit (LDS, occupancy), < 0 on any other
+  // error (rc set).  A launch ramps up until the admission of its middle
+  // primary, runs at a steady rate until the admission of its last one (the
+  // flights ended in between over that time, at the launch's collisions per
+  // flight), then drains: the last cascades' longest flight chains run on an
+  // emptying device.  Ramp and drain take tens of ms each, depend on the
+  // shape (a shape with more waves ramps longer; deep LDS stacks hold work in
+  // a few waves at the end) and the drain varies by as much from one launch
+  // to the next: a third of a sample launch of 3e9 collisions, a few per cent
+  // of a launch of 1e6 C2 histories.  On C5 one shape's start-to-last-
+  // admission rate was 18.2e9 on 3.3e5 histories and 20.3e9 on 1e6, another's
+  // 16.9e9 and 17.2e9.  The projection keeps the sample's ramp and drain
+  // (their time and the collisions done in them) and runs the other
+  // collisions of pk->n_hist histories (at the sample's collisions per
+  // history) at the steady rate; where the sample is all of pk it is the
+  // sample's own rate, and so it is where the steady phase holds under a
+  // quarter of the flights (fewer histories than the cascades the device
+  // holds at once).
+  auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last, const mt_primaries* ps = nullptr) -> double {
+    if (!ps) ps = &s;
     ctx->tune = t;
     double r = 0.0;
-    if (whole) *whole = 0.0;
     for (int i = 0; i < reps; ++i) {
       mt_tallies out{};
-      rc = mt_run_cascades(ctx, ps ? ps : &s, &ro, &out);
+      rc = mt_run_cascades(ctx, ps, &ro, &out);
       if (rc == MT_E_LIMIT) {
         rc = MT_OK;
         return 0.0;
       }
       if (rc != MT_OK) return -1.0;
-      double ri = out.kernel_ms > 0.0 ? double(out.n_coll) / (1e-3 * out.kernel_ms) : 0.0;
-      if (whole) *whole = std::max(*whole, ri);
-      const double sec = ctx->last_tadm > ctx->last_t0 && ctx->last_t0 > 0
-                             ? double(ctx->last_tadm - ctx->last_t0) / (ctx->clock_khz * 1e3)
-                             : 0.0;
-      if (sec > 0.0 && out.n_flight > 0 && 2 * ctx->last_fadm >= (unsigned long long)out.n_flight)
-        ri = double(ctx->last_fadm) * (double(out.n_coll) / double(out.n_flight)) / sec;
+      const double kern = 1e-3 * out.kernel_ms;
+      double ri = kern > 0.0 ? double(out.n_coll) / kern : 0.0;
+      const double hz = ctx->clock_khz * 1e3;
+      const unsigned long long t0 = ctx->last_t0, th = ctx->last_thalf, ta = ctx->last_tadm;
+      const unsigned long long fh = ctx->last_fhalf, fa = ctx->last_fadm;
+      if (ps->n_hist < pk->n_hist && t0 > 0 && th > t0 && ta > th && fa > fh && out.n_flight > 0 &&
+          4 * (fa - fh) >= (unsigned long long)out.n_flight) {
+        const double cpf = double(out.n_coll) / double(out.n_flight);
+        const double ramp = double(th - t0) / hz, adm = double(ta - t0) / hz;
+        const double steady = double(fa - fh) * cpf / (adm - ramp);
+        const double w_ramp = double(fh) * cpf, w_drain = double(out.n_flight - (long long)fa) * cpf;
+        const double drain = std::max(0.0, kern - adm);
+        const double n = double(out.n_coll) / double(ps->n_hist) * double(pk->n_hist);
+        ri = n / (ramp + std::max(0.0, n - w_ramp - w_drain) / steady + drain);
+      }
       r = std::max(r, ri);
       if (last) *last = out;
     }
@@ -1217,23 +1240,30 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // and 1-4 % slower on full launches, three times out of three
   const int k_roles = share >= 0.02 ? feeders(share, 2.5) : 0;
   std::vector<std::pair<mt_tuning, double>> timed[2];  // without, with roles
-  std::vector<int> occ0;  // waves per CU each shape without roles ran at
+  std::vector<int> occ[2];  // waves per CU each timed shape ran at
+  // each shape with the depth bins split between LDS and HBM (as many in LDS
+  // as fit) and, unless the caller set the split, with every bin in HBM: which
+  // is faster depends on the workload and the shape (below)
+  const int n_bins = keep.lds_hist == 0 ? 2 : 1;
   for (const auto& c : kCand) {
     for (int with = 0; with < (k_roles > 0 ? 2 : 1); ++with) {
-      mt_tuning t = t0;
-      t.block = c[0];
-      t.blocks_per_cu = c[1];
-      t.stack_cap = c[2];
-      t.primary_waves64 = with ? k_roles : 0;
-      mt_tallies last{};
-      const double r = rate_of(t, rc, &last);
-      if (r < 0.0) {
-        ctx->tune = keep;
-        return rc;
-      }
-      if (r > 0.0) {
-        timed[with].emplace_back(t, r);
-        if (!with) occ0.push_back(ctx->cus > 0 ? last.waves / ctx->cus : 0);
+      for (int b = 0; b < n_bins; ++b) {
+        mt_tuning t = t0;
+        t.block = c[0];
+        t.blocks_per_cu = c[1];
+        t.stack_cap = c[2];
+        t.primary_waves64 = with ? k_roles : 0;
+        if (b) t.lds_hist = -1;
+        mt_tallies last{};
+        const double r = rate_of(t, rc, &last);
+        if (r < 0.0) {
+          ctx->tune = keep;
+          return rc;
+        }
+        if (r > 0.0) {
+          timed[with].emplace_back(t, r);
+          occ[with].push_back(ctx->cus > 0 ? last.waves / ctx->cus : 0);
+        }
       }
     }
   }
@@ -1258,44 +1288,62 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   double best_r[2] = {0.0, 0.0};
   mt_tuning best_t[2] = {keep, keep};
   for (int w = 0; w < 2; ++w) pick_of(timed[w], best_t[w], best_r[w]);
-  // without roles, the same pick per occupancy: the waves per CU a shape ran
-  // at, which the registers and the LDS (stacks and tables) may hold below
-  // the requested threads per block x blocks per CU
-  std::vector<std::pair<mt_tuning, double>> per_occ;
-  std::vector<int> occs(occ0);
-  std::sort(occs.begin(), occs.end());
-  occs.erase(std::unique(occs.begin(), occs.end()), occs.end());
-  for (int occ : occs) {
-    std::vector<std::pair<mt_tuning, double>> v;
-    for (size_t i = 0; i < timed[0].size(); ++i)
-      if (occ0[i] == occ) v.push_back(timed[0][i]);
-    if (v.empty()) continue;
-    mt_tuning bt = keep;
-    double br = 0.0;
-    pick_of(v, bt, br);
-    if (br >= 0.9 * best_r[0]) per_occ.emplace_back(bt, br);
+  // per group (without, with roles) the same pick per occupancy within 10 %
+  // of the group's fastest: occupancy is the waves per CU a shape ran at,
+  // which the registers and the LDS (stacks, tables, bins) may hold below the
+  // requested threads per block x blocks per CU.  Without roles, also the
+  // runner-up of another shape: a sample of fewer histories than one round
+  // of admissions (64 per wave at the start) is rated on its kernel time,
+  // which varies by several per cent
+  auto same_shape = [](const mt_tuning& a, const mt_tuning& b) {
+    return a.block == b.block && a.blocks_per_cu == b.blocks_per_cu && a.stack_cap == b.stack_cap;
+  };
+  std::vector<std::pair<mt_tuning, double>> per_occ[2];
+  for (int w = 0; w < 2; ++w) {
+    std::vector<int> occs(occ[w]);
+    std::sort(occs.begin(), occs.end());
+    occs.erase(std::unique(occs.begin(), occs.end()), occs.end());
+    for (int oc : occs) {
+      std::vector<std::pair<mt_tuning, double>> v;
+      for (size_t i = 0; i < timed[w].size(); ++i)
+        if (occ[w][i] == oc) v.push_back(timed[w][i]);
+      mt_tuning bt = keep;
+      double br = 0.0;
+      pick_of(v, bt, br);
+      if (br < 0.9 * best_r[w]) continue;
+      per_occ[w].emplace_back(bt, br);
+      if (w == 0) {
+        const std::pair<mt_tuning, double>* second = nullptr;
+        for (const auto& e : v)
+          if (!same_shape(e.first, bt) && e.second >= 0.9 * best_r[w] && (!second || e.second > second->second))
+            second = &e;
+        if (second) per_occ[w].push_back(*second);
+      }
+    }
   }
   if (best_r[0] <= 0.0 && best_r[1] <= 0.0) {
     ctx->tune = keep;
     return fail(MT_E_LIMIT, "no candidate launch shape fits");
   }
-  if (best_r[1] > 0.0) {
-    const int ks[3] = {feeders(share, 1.5), feeders(share, 4.0), 63};
+  // with roles, three more feeder counts at each occupancy's pick
+  const int ks[3] = {feeders(share, 1.5), feeders(share, 4.0), 63};
+  for (auto& e : per_occ[1]) {
     for (int i = 0; i < 3; ++i) {
       bool dup = ks[i] == k_roles;
       for (int j = 0; j < i; ++j) dup = dup || ks[j] == ks[i];
       if (dup) continue;
-      mt_tuning t = best_t[1];
+      mt_tuning t = e.first;
       t.primary_waves64 = ks[i];
       const double r = rate_of(t, rc, nullptr);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
       }
-      if (r > best_r[1]) {
-        best_r[1] = r;
-        best_t[1] = t;
-      }
+      if (r > e.second) e = std::make_pair(t, r);
+    }
+    if (e.second > best_r[1]) {
+      best_r[1] = e.second;
+      best_t[1] = e.first;
     }
   }
   // Second stage, where the primaries hold at least twice the sample: the
@@ -1305,66 +1353,54 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // C2 (Cu -> Cu 500 keV), three waves per SIMD led two by 10 % on a
   // 5e4-history sample and trailed by 6-12 % from 2e5 histories on, with
   // 2.6e6 entries per queue shard against 1.5e6, on some MI355X devices
-  // (others ran both as on the sample).  The finalists are the pick of each
-  // occupancy within 10 % of the fastest shape without roles, and the roles
-  // pick.  Each finalist also runs with every depth bin in HBM, unless the
-  // caller set lds_hist: LDS atomics on the bins are in order with the stack's
-  // LDS traffic and serialise on a hot bin, HBM atomics are neither but
-  // contend in L2 where a few bins take most counts.  On C5 (vacancies spread
-  // over 1.5 um) the bins in HBM ran 8 % faster at 256 x 3; on C2 (shallow,
-  // dense cascades) a third slower.  Otherwise the two picks are timed once
-  // more each on the sample (a single timing varies by a few per cent).
+  // (others ran both as on the sample).  The finalists are the picks of each
+  // occupancy within 10 % of the fastest, without and with roles (C5's best,
+  // 128 x 6 with 14 feeders, trailed 256 x 2 at the first feeder count).
+  // Each finalist runs with its own depth-bin split and with the
+  // other one, unless the caller set lds_hist: LDS atomics on the bins are in
+  // order with the stack's LDS traffic and serialise on a hot bin, HBM
+  // atomics are neither but contend in L2 where a few bins take most counts.
+  // On C5 (vacancies spread over 1.5 um) the bins in HBM ran 8 % faster at
+  // 256 x 3; on C2 (shallow, dense cascades) a third slower.  Otherwise the
+  // two picks are timed once more each on the sample (a single timing varies
+  // by a few per cent).
   mt_primaries s2 = *pk;
   s2.n_hist = std::min(4 * s.n_hist, pk->n_hist);
   const bool stage2 = s2.n_hist >= 2 * s.n_hist;
-  // the better of a finalist with the caller's depth bins and with all bins
-  // in HBM, on the second-stage histories; < 0 on error (rc set)
-  auto final_rate = [&](mt_tuning& t, double& whole) -> double {
-    double r = rate_of(t, rc, nullptr, &s2, &whole);
+  // the better of a finalist with its depth-bin split and with the other one,
+  // on the second-stage histories; < 0 on error (rc set)
+  auto final_rate = [&](mt_tuning& t) -> double {
+    double r = rate_of(t, rc, nullptr, &s2);
     if (r < 0.0 || keep.lds_hist != 0) return r;
     mt_tuning h = t;
-    h.lds_hist = -1;
-    double wh = 0.0;
-    const double rh = rate_of(h, rc, nullptr, &s2, &wh);
+    h.lds_hist = t.lds_hist == 0 ? -1 : 0;
+    const double rh = rate_of(h, rc, nullptr, &s2);
     if (rh < 0.0) return rh;
     if (rh > r) {
       r = rh;
       t = h;
-      whole = wh;
     }
     return r;
   };
-  // The shapes are ranked on the steady state, roles against no roles on the
-  // whole launch: on C1 (long He primary flights) roles gain most in the
-  // drain, where the last primaries' flights would otherwise hold a few
-  // waves while the rest idle.
-  double whole_r[2] = {0.0, 0.0};
   if (stage2) {
-    double r0 = 0.0;
-    for (auto& e : per_occ) {
-      double wr = 0.0;
-      const double r = final_rate(e.first, wr);
-      if (r < 0.0) {
-        ctx->tune = keep;
-        return rc;
+    for (int w = 0; w < 2; ++w) {
+      double rw = 0.0;
+      for (auto& e : per_occ[w]) {
+        const double r = final_rate(e.first);
+        if (r < 0.0) {
+          ctx->tune = keep;
+          return rc;
+        }
+        if (r > rw) {
+          rw = r;
+          best_t[w] = e.first;
+        }
       }
-      if (r > r0) {
-        r0 = r;
-        best_t[0] = e.first;
-        whole_r[0] = wr;
-      }
-    }
-    best_r[0] = r0;
-    if (best_r[1] > 0.0) {
-      best_r[1] = final_rate(best_t[1], whole_r[1]);
-      if (best_r[1] < 0.0) {
-        ctx->tune = keep;
-        return rc;
-      }
+      best_r[w] = rw;
     }
   } else if (best_r[0] > 0.0 && best_r[1] > 0.0) {
     for (int w = 0; w < 2; ++w) {
-      const double r = rate_of(best_t[w], rc, nullptr, nullptr, &whole_r[w]);
+      const double r = rate_of(best_t[w], rc, nullptr);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
@@ -1372,7 +1408,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       best_r[w] = std::max(best_r[w], r);
     }
   }
-  const int pick = best_r[0] <= 0.0 || (best_r[1] > 0.0 && whole_r[1] > 1.03 * whole_r[0]) ? 1 : 0;
+  const int pick = best_r[0] <= 0.0 || (best_r[1] > 0.0 && best_r[1] > 1.03 * best_r[0]) ? 1 : 0;
   const double best_rate_v = best_r[pick];
   const mt_tuning best_v = best_t[pick];
   ctx->tune = best_v;

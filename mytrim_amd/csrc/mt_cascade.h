@@ -48,6 +48,10 @@ enum Ctrl {
   // their ratio is the launch's mean shader clock
   kClkSum = 69,
   kWallSum = 70,
+  // wall-clock stamp of the admission of the middle primary (index
+  // n_hist / 2) and the flights ended by then: the launch's ramp-up is over
+  kTHalf = 71,
+  kFlightsAtHalf = 72,
   kQueuePeak = 80,      // largest tail - head seen at a push
   kLive = 96,           // admitted cascades not yet complete
   kCascDone = 97,       // completed cascades

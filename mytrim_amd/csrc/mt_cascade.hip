@@ -482,6 +482,13 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
           const long long got = min(n_adm, max(0ll, A->n_hist - (long long)first));
           if (lane == 0 && got > 0) {
             atomicAdd(&ctrl[kLive], (unsigned long long)got);
+            const long long mid = A->n_hist / 2;
+            if ((long long)first <= mid && (long long)first + got > mid) {  // the middle primary
+              __hip_atomic_store(&ctrl[kFlightsAtHalf], (unsigned long long)ld_relaxed(&ctrl[kFlightsEnded]),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_store(&ctrl[kTHalf], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+            }
             if ((long long)first + got == A->n_hist) {  // the last primary: the drain starts
               __hip_atomic_store(&ctrl[kFlightsAtAdmit], (unsigned long long)ld_relaxed(&ctrl[kFlightsEnded]), __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);

@@ -83,7 +83,8 @@ class Tallies(C.Structure):
         ("h_y", dp), ("h_z", dp), ("h_state", ip), ("h_Eel", dp), ("kernel_ms", C.c_double),
         ("grid", C.c_int), ("block", C.c_int), ("waves", C.c_int), ("queue_peak", C.c_longlong),
         ("queue_ops", C.c_longlong), ("surrogate", C.c_longlong), ("coll_primary", C.c_longlong),
-        ("sclk_mhz", C.c_double)])
+        ("sclk_mhz", C.c_double), ("half_ms", C.c_double), ("flights_at_half", C.c_longlong),
+        ("admit_ms", C.c_double), ("flights_at_admit", C.c_longlong)])
 
 
 class Tuning(C.Structure):
@@ -277,10 +278,11 @@ class Engine:
         return pk, o, keep
 
     def autotune(self, prob: Problem, sample: int = 2000, reps: int = 1, arith: int = MT_FP64):
-        """Pick the fastest launch shape on the first `sample` histories of `prob`
-        (kept in the context for later runs); when `prob` holds at least twice
-        `sample` histories, the finalists are timed again on up to four times
-        `sample`.  Returns (Tuning, collisions/s)."""
+        """Pick the fastest launch shape for a launch of all of `prob`, timed on
+        its first `sample` histories (kept in the context for later runs); when
+        `prob` holds at least twice `sample` histories, the finalists are timed
+        again on up to four times `sample`.  Returns (Tuning, collisions/s
+        projected for the whole launch)."""
         pk, o, keep = self._launch_args(prob, False, arith)
         best = Tuning()
         rate = C.c_double()
@@ -344,7 +346,9 @@ class Engine:
         res.update(range_n=out.range_n, range_sum=out.range_sum, range_sum2=out.range_sum2,
                    kernel_ms=out.kernel_ms, grid=out.grid, block=out.block, waves=out.waves,
                    queue_peak=out.queue_peak, queue_ops=out.queue_ops, surrogate=out.surrogate,
-                   coll_primary=out.coll_primary, sclk_mhz=out.sclk_mhz)
+                   coll_primary=out.coll_primary, sclk_mhz=out.sclk_mhz, half_ms=out.half_ms,
+                   flights_at_half=out.flights_at_half, admit_ms=out.admit_ms,
+                   flights_at_admit=out.flights_at_admit)
         res.update(hists)
         res.update(per)
         del keep
