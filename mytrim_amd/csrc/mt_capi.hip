@@ -14,6 +14,7 @@
 
 #include "mt_cascade.h"
 #include "mt_flight.h"
+#include "mt_tune.h"
 #include "mt_host.h"
 #include "mytrim_gpu.h"
 
@@ -1150,25 +1151,16 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   ro.per_history = 0;
   const mt_tuning keep = ctx->tune;
   // one timed pass of a sample under tuning t (the best of `reps`): the rate
-  // projected for a launch of all pk->n_hist histories, in collisions per
-  // second; 0 when the shape does not fit (LDS, occupancy), < 0 on any other
-  // error (rc set).  A launch ramps up until the admission of its middle
-  // primary, runs at a steady rate until the admission of its last one (the
-  // flights ended in between over that time, at the launch's collisions per
-  // flight), then drains: the last cascades' longest flight chains run on an
-  // emptying device.  Ramp and drain take tens of ms each, depend on the
-  // shape (a shape with more waves ramps longer; deep LDS stacks hold work in
-  // a few waves at the end) and the drain varies by as much from one launch
-  // to the next: a third of a sample launch of 3e9 collisions, a few per cent
-  // of a launch of 1e6 C2 histories.  On C5 one shape's start-to-last-
-  // admission rate was 18.2e9 on 3.3e5 histories and 20.3e9 on 1e6, another's
-  // 16.9e9 and 17.2e9.  The projection keeps the sample's ramp and drain
-  // (their time and the collisions done in them) and runs the other
-  // collisions of pk->n_hist histories (at the sample's collisions per
-  // history) at the steady rate; where the sample is all of pk it is the
-  // sample's own rate, and so it is where the steady phase holds under a
-  // quarter of the flights (fewer histories than the cascades the device
-  // holds at once).
+  // projected for a launch of all pk->n_hist histories (projected_rate: the
+  // sample's ramp and drain kept, the rest at its steady rate), in
+  // collisions per second; 0 when the shape does not fit (LDS, occupancy),
+  // < 0 on any other error (rc set).  Ramp and drain take tens of ms each,
+  // depend on the shape (a shape with more waves ramps longer; deep LDS
+  // stacks hold work in a few waves at the end) and the drain varies by as
+  // much from one launch to the next: a third of a sample launch of 3e9
+  // collisions, a few per cent of a launch of 1e6 C2 histories.  On C5 one
+  // shape's start-to-last-admission rate was 18.2e9 on 3.3e5 histories and
+  // 20.3e9 on 1e6, another's 16.9e9 and 17.2e9.
   auto rate_of = [&](const mt_tuning& t, int& rc, mt_tallies* last, const mt_primaries* ps = nullptr) -> double {
     if (!ps) ps = &s;
     ctx->tune = t;
@@ -1181,22 +1173,16 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
         return 0.0;
       }
       if (rc != MT_OK) return -1.0;
-      const double kern = 1e-3 * out.kernel_ms;
-      double ri = kern > 0.0 ? double(out.n_coll) / kern : 0.0;
-      const double hz = ctx->clock_khz * 1e3;
-      const unsigned long long t0 = ctx->last_t0, th = ctx->last_thalf, ta = ctx->last_tadm;
-      const unsigned long long fh = ctx->last_fhalf, fa = ctx->last_fadm;
-      if (ps->n_hist < pk->n_hist && t0 > 0 && th > t0 && ta > th && fa > fh && out.n_flight > 0 &&
-          4 * (fa - fh) >= (unsigned long long)out.n_flight) {
-        const double cpf = double(out.n_coll) / double(out.n_flight);
-        const double ramp = double(th - t0) / hz, adm = double(ta - t0) / hz;
-        const double steady = double(fa - fh) * cpf / (adm - ramp);
-        const double w_ramp = double(fh) * cpf, w_drain = double(out.n_flight - (long long)fa) * cpf;
-        const double drain = std::max(0.0, kern - adm);
-        const double n = double(out.n_coll) / double(ps->n_hist) * double(pk->n_hist);
-        ri = n / (ramp + std::max(0.0, n - w_ramp - w_drain) / steady + drain);
-      }
-      r = std::max(r, ri);
+      LaunchPhases ph;
+      ph.kernel_s = 1e-3 * out.kernel_ms;
+      ph.half_s = 1e-3 * out.half_ms;
+      ph.admit_s = 1e-3 * out.admit_ms;
+      ph.n_coll = out.n_coll;
+      ph.n_flight = out.n_flight;
+      ph.flights_half = out.flights_at_half;
+      ph.flights_admit = out.flights_at_admit;
+      ph.n_hist = ps->n_hist;
+      r = std::max(r, projected_rate(ph, pk->n_hist));
       if (last) *last = out;
     }
     return r;
