@@ -1225,8 +1225,7 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
   // add queue traffic: on C2 (1.2 %) roles measured within noise on the sample
   // and 1-4 % slower on full launches, three times out of three
   const int k_roles = share >= 0.02 ? feeders(share, 2.5) : 0;
-  std::vector<std::pair<mt_tuning, double>> timed[2];  // without, with roles
-  std::vector<int> occ[2];  // waves per CU each timed shape ran at
+  std::vector<TimedShape> timed[2];  // without, with roles
   // each shape with the depth bins split between LDS and HBM (as many in LDS
   // as fit) and, unless the caller set the split, with every bin in HBM: which
   // is faster depends on the workload and the shape (below)
@@ -1246,66 +1245,28 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
           ctx->tune = keep;
           return rc;
         }
-        if (r > 0.0) {
-          timed[with].emplace_back(t, r);
-          occ[with].push_back(ctx->cus > 0 ? last.waves / ctx->cus : 0);
-        }
+        if (r > 0.0) timed[with].push_back(TimedShape{t, r, ctx->cus > 0 ? last.waves / ctx->cus : 0});
       }
     }
   }
-  // the fastest of a set of timed shapes, except that among shapes within 2 %
-  // of it (the spread of one timing) the deepest LDS stack wins: on the full
-  // launches shallower stacks than the sample's pick measured slower (C3,
-  // 96 entries picked over 128: -8 %)
-  auto pick_of = [](const std::vector<std::pair<mt_tuning, double>>& v, mt_tuning& bt, double& br) {
-    double top = 0.0;
-    for (const auto& e : v) top = std::max(top, e.second);
-    br = 0.0;
-    for (const auto& e : v) {
-      if (e.second < 0.98 * top) continue;
-      const bool deeper = e.first.stack_cap > bt.stack_cap;
-      const bool same = e.first.stack_cap == bt.stack_cap;
-      if (br == 0.0 || deeper || (same && e.second > br)) {
-        br = e.second;
-        bt = e.first;
-      }
-    }
-  };
+  // per group (without, with roles): the pick (pick_shape: the fastest, the
+  // deepest stack within 2 % of it), and the pick of each occupancy within
+  // 10 % of the group's fastest.  Occupancy is the waves per CU a shape ran
+  // at, which the registers and the LDS (stacks, tables, bins) may hold below
+  // the requested threads per block x blocks per CU.  Without roles, also the
+  // runner-up of another shape at each occupancy: a sample of fewer histories
+  // than one round of admissions (64 per wave at the start) is rated on its
+  // kernel time, which varies by several per cent.
   double best_r[2] = {0.0, 0.0};
   mt_tuning best_t[2] = {keep, keep};
-  for (int w = 0; w < 2; ++w) pick_of(timed[w], best_t[w], best_r[w]);
-  // per group (without, with roles) the same pick per occupancy within 10 %
-  // of the group's fastest: occupancy is the waves per CU a shape ran at,
-  // which the registers and the LDS (stacks, tables, bins) may hold below the
-  // requested threads per block x blocks per CU.  Without roles, also the
-  // runner-up of another shape: a sample of fewer histories than one round
-  // of admissions (64 per wave at the start) is rated on its kernel time,
-  // which varies by several per cent
-  auto same_shape = [](const mt_tuning& a, const mt_tuning& b) {
-    return a.block == b.block && a.blocks_per_cu == b.blocks_per_cu && a.stack_cap == b.stack_cap;
-  };
-  std::vector<std::pair<mt_tuning, double>> per_occ[2];
+  std::vector<TimedShape> per_occ[2];
   for (int w = 0; w < 2; ++w) {
-    std::vector<int> occs(occ[w]);
-    std::sort(occs.begin(), occs.end());
-    occs.erase(std::unique(occs.begin(), occs.end()), occs.end());
-    for (int oc : occs) {
-      std::vector<std::pair<mt_tuning, double>> v;
-      for (size_t i = 0; i < timed[w].size(); ++i)
-        if (occ[w][i] == oc) v.push_back(timed[w][i]);
-      mt_tuning bt = keep;
-      double br = 0.0;
-      pick_of(v, bt, br);
-      if (br < 0.9 * best_r[w]) continue;
-      per_occ[w].emplace_back(bt, br);
-      if (w == 0) {
-        const std::pair<mt_tuning, double>* second = nullptr;
-        for (const auto& e : v)
-          if (!same_shape(e.first, bt) && e.second >= 0.9 * best_r[w] && (!second || e.second > second->second))
-            second = &e;
-        if (second) per_occ[w].push_back(*second);
-      }
+    const int p = pick_shape(timed[w]);
+    if (p >= 0) {
+      best_t[w] = timed[w][size_t(p)].t;
+      best_r[w] = timed[w][size_t(p)].rate;
     }
+    per_occ[w] = occupancy_finalists(timed[w], w == 0);
   }
   if (best_r[0] <= 0.0 && best_r[1] <= 0.0) {
     ctx->tune = keep;
@@ -1318,18 +1279,21 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
       bool dup = ks[i] == k_roles;
       for (int j = 0; j < i; ++j) dup = dup || ks[j] == ks[i];
       if (dup) continue;
-      mt_tuning t = e.first;
+      mt_tuning t = e.t;
       t.primary_waves64 = ks[i];
       const double r = rate_of(t, rc, nullptr);
       if (r < 0.0) {
         ctx->tune = keep;
         return rc;
       }
-      if (r > e.second) e = std::make_pair(t, r);
+      if (r > e.rate) {
+        e.t = t;
+        e.rate = r;
+      }
     }
-    if (e.second > best_r[1]) {
-      best_r[1] = e.second;
-      best_t[1] = e.first;
+    if (e.rate > best_r[1]) {
+      best_r[1] = e.rate;
+      best_t[1] = e.t;
     }
   }
   // Second stage, where the primaries hold at least twice the sample: the
@@ -1372,14 +1336,14 @@ int mt_autotune(mt_ctx* ctx, const mt_primaries* pk, const mt_run_opts* o, long 
     for (int w = 0; w < 2; ++w) {
       double rw = 0.0;
       for (auto& e : per_occ[w]) {
-        const double r = final_rate(e.first);
+        const double r = final_rate(e.t);
         if (r < 0.0) {
           ctx->tune = keep;
           return rc;
         }
         if (r > rw) {
           rw = r;
-          best_t[w] = e.first;
+          best_t[w] = e.t;
         }
       }
       best_r[w] = rw;
