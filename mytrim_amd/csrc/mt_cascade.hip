@@ -490,8 +490,8 @@ __global__ void __launch_bounds__(kCascadeMaxBlock, (ARITH == 2 ? 2 : 3)) cascad
                                  __HIP_MEMORY_SCOPE_AGENT);
             }
             if ((long long)first + got == A->n_hist) {  // the last primary: the drain starts
-              __hip_atomic_store(&ctrl[kFlightsAtAdmit], (unsigned long long)ld_relaxed(&ctrl[kFlightsEnded]), __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_store(&ctrl[kFlightsAtAdmit], (unsigned long long)ld_relaxed(&ctrl[kFlightsEnded]),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               __hip_atomic_store(&ctrl[kTAdmit], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
             }
